@@ -1,0 +1,183 @@
+"""Benchmark: subdomain bounds per second of the batched B&B loop on one GPU.
+
+Workload: seeded nonconvex QCQP, n = 50, m = 20, coefficients N(0,1)/sqrt(n),
+x in [-1, 1]^50; the loop bounds (CROWN, FP64), branches on the widest
+variable, prunes against the ALM-PGD incumbent and the constraint rows, and
+appends survivors to a device frontier of at most 2^18 domains.  One step =
+one B&B iteration: `batch` parents picked from the frontier -> 2 * batch
+children bounded.  The root ALM-PGD search runs once before the warm-up.
+
+Prints one JSON line (see the keys below).  The CPU baseline times the FP64
+oracle on the host cores over a bounded sample of the same children; the
+plain FP64 GPU path (op-by-op PyTorch) is timed on the same children too.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+FP64_PEAK_TFLOPS = 78.6
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--batch", type=int, default=16384, help="parents per step")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-alm", action="store_true")
+    return ap.parse_args()
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    from gear_amd.bab import BaBConfig, QCQPBranchAndBound
+    from gear_amd.crown import input_split_branching, input_split_parallel
+    from gear_amd.problems import config2_problem, flops_per_box, bytes_per_box
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    p = config2_problem()
+    n, S = p.n, p.S
+    cap = 1 << 18
+    need = args.batch * (args.warmup + args.steps) + 1
+    if need > cap:
+        raise SystemExit(f"batch * (warmup + steps) = {need} exceeds the frontier cap {cap}")
+    bb = QCQPBranchAndBound(p, BaBConfig(batch=args.batch, cap=cap), device=dev)
+    t_alm = time.time()
+    bb.root(alm=not args.no_alm)
+    torch.cuda.synchronize()
+    t_alm = time.time() - t_alm
+    stream = torch.cuda.current_stream()
+
+    # bound-kernel timing: events on the stream the kernel is launched on
+    events = []
+    orig_bound = bb.net.bound
+
+    def timed_bound(x_L, x_U, **kw):
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        out = orig_bound(x_L, x_U, **kw)
+        e1.record(stream)
+        events.append((e0, e1, x_L.shape[0]))
+        return out
+
+    bb.net.bound = timed_bound
+    for _ in range(args.warmup):
+        bb.step()
+    events.clear()
+    sample_parents = None
+    if len(bb.frontier) >= args.batch:
+        s0 = len(bb.frontier) - args.batch
+        sample_parents = (bb.frontier.L[s0:].clone(), bb.frontier.U[s0:].clone())
+    torch.cuda.synchronize()
+    boxes = 0
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        boxes += bb.step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    bb.update_dual()
+    wall = t1 - t0
+    kt = [(a.elapsed_time(b) * 1e-3, nb) for a, b, nb in events]
+    k_time = sum(t for t, _ in kt)
+    k_boxes = sum(nb for _, nb in kt)
+    avg_launch = k_time / max(len(kt), 1)
+    avg_boxes = k_boxes / max(len(kt), 1)
+    fpb = flops_per_box(n, S)
+    achieved_tf = fpb * avg_boxes / avg_launch / 1e12
+    value = boxes / wall
+
+    result = {
+        "metric": "subdomain bounds/sec (full batched B&B loop, FP64 CROWN)",
+        "value": round(value, 1),
+        "unit": "subdomain bounds/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded QCQP generator, default_rng(3))",
+        "config": {"workload": "qcqp_n50_m20 batched B&B: bound (CROWN) -> widest branch -> "
+                               "prune -> append, frontier cap 2^18",
+                   "n": n, "m": S - 1, "parents_per_step": args.batch,
+                   "children_per_step": 2 * args.batch},
+        "roofline": {
+            "bound": "fp64",
+            "kernel": "qcqp_crown_f64_kernel",
+            "achieved": round(achieved_tf, 3),
+            "peak": FP64_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": round(achieved_tf / FP64_PEAK_TFLOPS, 4),
+            "traffic": None,
+            "flops_per_box": fpb,
+            "bytes_per_box": bytes_per_box(n, S),
+            "avg_launch_ms": round(avg_launch * 1e3, 4),
+            "boxes_per_launch": avg_boxes,
+        },
+        "bab": {"frontier": len(bb.frontier), "P": bb.P, "D": bb.D,
+                "root_alm_s": round(t_alm, 3)},
+    }
+
+    if sample_parents is not None and not args.no_plain:
+        from gear_amd.plain import plain_qcqp_crown
+        pL, pU = sample_parents
+        idx = input_split_branching(None, None, pL, pU, None, None, "naive")
+        cL, cU, _ = input_split_parallel(pL, pU, i_idx=idx)
+        plain_qcqp_crown(bb.net.dev, cL, cU)
+        torch.cuda.synchronize()
+        tp = time.perf_counter()
+        reps = 3
+        for _ in range(reps):
+            plain_qcqp_crown(bb.net.dev, cL, cU)
+        torch.cuda.synchronize()
+        tp = (time.perf_counter() - tp) / reps
+        fused = orig_bound(cL, cU)
+        torch.cuda.synchronize()
+        tf = time.perf_counter()
+        for _ in range(reps):
+            orig_bound(cL, cU, out=fused)
+        torch.cuda.synchronize()
+        tf = (time.perf_counter() - tf) / reps
+        result["plain_fp64_gpu"] = {"value": round(cL.shape[0] / tp, 1),
+                                    "unit": "subdomain bounds/s",
+                                    "fused_same_boxes": round(cL.shape[0] / tf, 1),
+                                    "speedup": round(tp / tf, 2)}
+
+    if sample_parents is not None and not args.no_cpu:
+        from oracle import oracle as O
+        pL, pU = sample_parents
+        idx = input_split_branching(None, None, pL, pU, None, None, "naive")
+        cL, cU, _ = input_split_parallel(pL, pU, i_idx=idx)
+        L = cL.cpu().numpy()
+        U = cU.cpu().numpy()
+        cores = min(16, os.cpu_count() or 1)
+        O.qcqp_crown(p.Q, p.c, p.d, L[:64], U[:64], threads=cores)
+        tc = time.perf_counter()
+        O.qcqp_crown(p.Q, p.c, p.d, L, U, threads=cores)
+        tc = time.perf_counter() - tc
+        result["cpu_baseline"] = {"value": round(L.shape[0] / tc, 1),
+                                  "unit": "subdomain bounds/s", "cores": cores, "kind": "port",
+                                  "sample": f"{L.shape[0]} children of one step, FP64 oracle "
+                                            f"({tc:.2f} s wall)"}
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,143 @@
+"""Batched branch and bound over input boxes on one device.
+
+One iteration follows the reference's regular order
+(complete_verifier/input_split/batch_branch_and_bound.py:275-652):
+pick out the last `batch` domains -> branch on the widest variable -> bisect
+-> bound the children (CROWN) -> prune against the thresholds -> append the
+survivors; then the global lower bound is the frontier minimum capped at the
+discarded-threshold minimum (:620-634), and the gap test is
+is_primal_dual_converged with rel_tol (complete_verifier/solving_utils.py:
+17-49).  ALM-PGD refreshes the incumbent every `alm_interval` iterations
+(nlp_in_bab.py:1078-1231).
+"""
+import time
+from dataclasses import dataclass, field
+
+import torch
+
+from .alm import ALMConfig, ALMPGDSolver, replay_float64
+from .crown import BoundedQCQP, DeviceFrontier, input_split_branching, input_split_parallel
+
+
+def is_primal_dual_converged(P, D, abs_tol=1e-5, rel_tol=0.0):
+    """solving_utils.py:17-49."""
+    if not (abs(P) < float("inf") and abs(D) < float("inf")):
+        return False
+    gap = P - D
+    if gap < -abs_tol:
+        return False
+    if gap <= abs_tol:
+        return True
+    return gap / max(1.0, abs(P), abs(D)) <= rel_tol
+
+
+@dataclass
+class BaBConfig:
+    batch: int = 16384             # parents per iteration (2x children bounded)
+    cap: int = 1 << 18             # frontier capacity
+    rel_tol: float = 1e-4
+    abs_tol: float = 1e-5
+    constr_viol_tol: float = 1e-6
+    max_iters: int = 1000
+    time_limit: float = 60.0
+    alm_interval: int = 0          # 0: ALM only at the root
+    alm: ALMConfig = field(default_factory=ALMConfig)
+
+
+@dataclass
+class BaBStats:
+    iterations: int = 0
+    boxes_bounded: int = 0
+    bound_time: float = 0.0
+    wall: float = 0.0
+    history: list = field(default_factory=list)
+
+
+class QCQPBranchAndBound:
+    """Device-resident B&B over one QCQP instance."""
+
+    def __init__(self, problem, cfg: BaBConfig = None, device="cuda"):
+        self.p = problem
+        self.cfg = cfg or BaBConfig()
+        self.net = BoundedQCQP(problem, device)
+        self.device = self.net.device
+        self.frontier = DeviceFrontier(problem.n, problem.S, self.cfg.cap, self.device)
+        self.alm = ALMPGDSolver(self.cfg.alm)
+        self.P = float("inf")
+        self.x_best = None
+        self.D = -float("inf")
+        self.stats = BaBStats()
+        thr = torch.as_tensor(problem.rhs, dtype=torch.float64).clone()
+        thr[0] = float("inf")
+        thr[1:] += self.cfg.constr_viol_tol
+        self.thr = thr.to(self.device)
+        self._glb = torch.empty(1, dtype=torch.float64, device=self.device)
+
+    def set_incumbent(self, obj, x):
+        if obj < self.P:
+            self.P, self.x_best = obj, x
+            self.thr[0] = obj
+
+    def run_alm(self, x_L, x_U, restarts=None):
+        res = self.alm.solve_with_restarts(self.net.dev, x_L, x_U, num_restarts=restarts)
+        if res.x is not None:
+            obj, viol, feas = replay_float64(self.p, res.x, self.cfg.constr_viol_tol)
+            if feas:
+                self.set_incumbent(obj, res.x)
+        return res
+
+    def root(self, alm=True):
+        L = torch.as_tensor(self.p.xl, dtype=torch.float64, device=self.device)[None]
+        U = torch.as_tensor(self.p.xu, dtype=torch.float64, device=self.device)[None]
+        if alm:
+            self.run_alm(L, U)
+        lb = self.net.bound(L, U)
+        self.frontier.filter_append(lb, self.thr, L, U)
+        self.update_dual()
+
+    def update_dual(self):
+        if len(self.frontier) == 0:
+            self.D = float(self.frontier.disc_min.item())
+            if self.D == float("inf"):
+                self.D = self.P
+        else:
+            self.D = float(self.frontier.global_lb(self._glb).item())
+        return self.D
+
+    def step(self, batch=None, sync=True):
+        """One iteration; returns the number of children bounded."""
+        f = self.frontier
+        # children (2 nb) go where the nb parents were plus free rows
+        nb = min(batch or self.cfg.batch, len(f), f.cap - len(f))
+        if nb <= 0:
+            return 0
+        pL, pU, _ = f.pick_out(nb)
+        idx = input_split_branching(self.net, None, pL, pU, None, None, "naive")
+        cL, cU, _ = input_split_parallel(pL, pU, i_idx=idx)
+        lb = self.net.bound(cL, cU)
+        f.filter_append(lb, self.thr, cL, cU, sync=sync)
+        self.stats.iterations += 1
+        self.stats.boxes_bounded += 2 * nb
+        return 2 * nb
+
+    def solve(self):
+        cfg = self.cfg
+        t0 = time.time()
+        self.root()
+        while len(self.frontier) > 0 and self.stats.iterations < cfg.max_iters:
+            if is_primal_dual_converged(self.P, self.D, cfg.abs_tol, cfg.rel_tol):
+                break
+            if time.time() - t0 > cfg.time_limit:
+                break
+            if cfg.alm_interval and self.stats.iterations % cfg.alm_interval == 0 and \
+                    self.stats.iterations > 0:
+                k = min(len(self.frontier), 64)
+                s = len(self.frontier) - k
+                self.run_alm(self.frontier.L[s:s + k], self.frontier.U[s:s + k], restarts=8)
+            self.step()
+            self.update_dual()
+            self.stats.history.append((time.time() - t0, self.P, self.D, len(self.frontier)))
+        self.stats.wall = time.time() - t0
+        if len(self.frontier) == 0:
+            self.D = min(self.P, float(self.frontier.disc_min.item()))
+        return self

@@ -1,0 +1,195 @@
+"""Host-side mirror of the reference's bound / split / branch / prune calls,
+dispatching to the gfx950 kernels of libgearhip.so.
+
+Call shapes follow the reference so a caller reads the same:
+  * BoundedQCQP.compute_bounds(x=(BoundedTensor,), C=None, method='CROWN',
+    bound_upper=False, return_A=True, needed_A_dict={out: {in}}) returns
+    (lb (B,S), None, A_dict) like BoundedModule.compute_bounds
+    (auto_LiRPA/bound_general.py:1275-1286; backward_bound.py:1098-1111);
+  * input_split_branching(net, dom_lb, x_L, x_U, lA, thresholds, 'naive')
+    (complete_verifier/input_split/branching_heuristics.py:59-86);
+  * input_split_parallel(x_L, x_U, shape, split_depth=1, i_idx=...)
+    (complete_verifier/input_split/split.py:26-122);
+  * filter_verified_domains-style pruning into a DeviceFrontier
+    (complete_verifier/input_split/branching_domains.py:238-558).
+Every entry point runs the HIP kernels; there is no CPU or torch fallback.
+"""
+import torch
+
+from . import _lib
+from ._lib import check, lib, ptr, stream_ptr
+from .problems import DeviceQCQP, QCQP
+
+
+class PerturbationLpNorm:
+    """Box perturbation (norm = inf) given by x_L, x_U
+    (auto_LiRPA/perturbations.py:129-500); only the box form is supported."""
+
+    def __init__(self, eps=None, norm=float("inf"), x_L=None, x_U=None):
+        if norm != float("inf"):
+            raise ValueError("only the L-inf box perturbation is supported")
+        if x_L is None or x_U is None:
+            raise ValueError("x_L and x_U are required")
+        self.norm, self.x_L, self.x_U = norm, x_L, x_U
+
+
+class BoundedTensor:
+    """A tensor with a box perturbation (auto_LiRPA/bounded_tensor.py)."""
+
+    def __init__(self, x, ptb: PerturbationLpNorm):
+        self.data, self.ptb = x, ptb
+
+
+class BoundedQCQP:
+    """A QCQP program bound on one device.  `compute_bounds` is the CROWN
+    lower bound of every row over every box (C = eye(S))."""
+
+    output_name = "/out"
+    input_name = "/x"
+
+    def __init__(self, problem: QCQP, device="cuda"):
+        if problem.n > 64:
+            raise _lib.GearHipError("the fused FP64 kernel covers n <= 64")
+        self.problem = problem
+        self.dev = DeviceQCQP(problem, device)
+        self.device = self.dev.device
+        self.n, self.S = problem.n, problem.S
+        lib()
+
+    def bound(self, x_L, x_U, return_A=False, out=None, stream=None):
+        """lb (B, S) [, lA (B, S, n), lbias (B, S)] for boxes x_L, x_U (B, n)."""
+        x_L = x_L.contiguous()
+        x_U = x_U.contiguous()
+        if x_L.dtype != torch.float64 or x_U.dtype != torch.float64:
+            raise TypeError("the FP64 path takes float64 boxes")
+        B = x_L.shape[0]
+        if x_L.shape != (B, self.n) or x_U.shape != (B, self.n):
+            raise ValueError(f"boxes must be (B, {self.n})")
+        lb = out if out is not None else torch.empty(B, self.S, dtype=torch.float64,
+                                                     device=self.device)
+        lA = lbias = None
+        if return_A:
+            lA = torch.empty(B, self.S, self.n, dtype=torch.float64, device=self.device)
+            lbias = torch.empty(B, self.S, dtype=torch.float64, device=self.device)
+        d = self.dev
+        check(lib().gh_qcqp_crown_f64(B, self.n, self.S, ptr(d.Q), ptr(d.c), ptr(d.d),
+                                      ptr(x_L), ptr(x_U), ptr(lb), ptr(lA), ptr(lbias),
+                                      stream_ptr(stream)), "gh_qcqp_crown_f64")
+        return (lb, lA, lbias) if return_A else lb
+
+    def compute_bounds(self, x=None, C=None, method="CROWN", bound_lower=True,
+                       bound_upper=False, return_A=False, needed_A_dict=None, **kwargs):
+        if method.lower() not in ("crown", "backward"):
+            raise NotImplementedError(f"method {method!r}: only CROWN is on this path")
+        if bound_upper or not bound_lower:
+            raise NotImplementedError("only lower bounds are on this path")
+        if C is not None:
+            raise NotImplementedError("C must be the default eye(S)")
+        bt = x[0] if isinstance(x, (tuple, list)) else x
+        res = self.bound(bt.ptb.x_L, bt.ptb.x_U, return_A=return_A)
+        if not return_A:
+            return res, None
+        lb, lA, lbias = res
+        A_dict = {self.output_name: {self.input_name: {
+            "lA": lA, "uA": None, "lbias": lbias, "ubias": None, "unstable_idx": None}}}
+        return lb, None, A_dict
+
+
+def input_split_branching(net, dom_lb, x_L, x_U, lA, thresholds, branching_method="naive",
+                          split_depth=1, **kwargs):
+    """(B, 1) int64 indices of the widest input per box (ties: lowest index)."""
+    if branching_method != "naive" or split_depth != 1:
+        raise NotImplementedError("only naive (widest) branching at depth 1 is on this path")
+    B, n = x_L.shape
+    idx = torch.empty(B, dtype=torch.int32, device=x_L.device)
+    check(lib().gh_branch_widest_f64(B, n, ptr(x_L.contiguous()), ptr(x_U.contiguous()),
+                                     ptr(idx), stream_ptr()), "gh_branch_widest_f64")
+    return idx.long().unsqueeze(1)
+
+
+def input_split_parallel(x_L, x_U, shape=None, split_depth=1, i_idx=None, split_partitions=2,
+                         split_hint=None):
+    """Bisect every box on i_idx[:, 0]: returns (new_x_L, new_x_U, 1) with the
+    upper halves first, then the lower halves."""
+    if split_depth != 1 or split_partitions != 2 or split_hint is not None:
+        raise NotImplementedError("depth-1 bisection only")
+    B, n = x_L.shape
+    idx = i_idx[:, 0].to(torch.int32).contiguous()
+    oL = torch.empty(2 * B, n, dtype=x_L.dtype, device=x_L.device)
+    oU = torch.empty_like(oL)
+    check(lib().gh_split_f64(B, n, ptr(x_L.contiguous()), ptr(x_U.contiguous()), ptr(idx),
+                             ptr(oL), ptr(oU), stream_ptr()), "gh_split_f64")
+    return oL, oU, 1
+
+
+class DeviceFrontier:
+    """Device-resident domain storage with stack order (pop from the end),
+    as StackTensorStorage (complete_verifier/tensor_storage.py:351-406).
+    Rows: x_L, x_U (n), lb (S)."""
+
+    def __init__(self, n, S, cap, device="cuda"):
+        self.n, self.S, self.cap = n, S, int(cap)
+        self.device = torch.device(device)
+        self.L = torch.empty(self.cap, n, dtype=torch.float64, device=self.device)
+        self.U = torch.empty_like(self.L)
+        self.lb = torch.empty(self.cap, S, dtype=torch.float64, device=self.device)
+        self.count = 0
+        self.total = torch.zeros(1, dtype=torch.int64, device=self.device)
+        self.disc_min = torch.full((1,), float("inf"), dtype=torch.float64, device=self.device)
+        self._ws = None
+
+    def __len__(self):
+        return self.count
+
+    def push(self, x_L, x_U, lb):
+        """Unconditional append (root)."""
+        B = x_L.shape[0]
+        if self.count + B > self.cap:
+            raise RuntimeError("frontier capacity exceeded")
+        self.L[self.count:self.count + B] = x_L
+        self.U[self.count:self.count + B] = x_U
+        self.lb[self.count:self.count + B] = lb
+        self.count += B
+
+    def pick_out(self, batch):
+        """Views of the last `batch` rows (the parents of the next split);
+        the rows stay valid until the next append overwrites them."""
+        nb = min(batch, self.count)
+        s = self.count - nb
+        self.count = s
+        return self.L[s:s + nb], self.U[s:s + nb], self.lb[s:s + nb]
+
+    def workspace(self, B):
+        need = int(lib().gh_filter_workspace_bytes(B))
+        if self._ws is None or self._ws.numel() < need:
+            self._ws = torch.empty(need, dtype=torch.uint8, device=self.device)
+        return self._ws
+
+    def filter_append(self, lb, thr, x_L, x_U, sync=True):
+        """Prune boxes verified against thresholds thr (S,) and append the
+        survivors in order.  Returns the survivor count (a host sync) when
+        sync is set, else None (call finish_append later)."""
+        B = x_L.shape[0]
+        ws = self.workspace(B)
+        check(lib().gh_filter_append_f64(B, self.S, self.n, ptr(lb), ptr(thr), ptr(x_L),
+                                         ptr(x_U), ptr(self.L), ptr(self.U), ptr(self.lb),
+                                         self.count, self.cap, ptr(self.total),
+                                         ptr(self.disc_min), ptr(ws), stream_ptr()),
+              "gh_filter_append_f64")
+        if sync:
+            return self.finish_append()
+        return None
+
+    def finish_append(self):
+        kept = int(self.total.item())
+        self.count += kept
+        return kept
+
+    def global_lb(self, out=None):
+        """min(lb[:, 0] over the frontier, discarded-threshold minimum) as a
+        device scalar."""
+        out = out if out is not None else torch.empty(1, dtype=torch.float64, device=self.device)
+        out.copy_(self.disc_min)
+        check(lib().gh_min_strided_f64(self.count, self.S, ptr(self.lb), ptr(out), stream_ptr()),
+              "gh_min_strided_f64")
+        return out

@@ -1,0 +1,121 @@
+/*
+ * gearhip -- C ABI of the MI355X (gfx950) batched CROWN bound / branch /
+ * prune path.
+ *
+ * Conventions (all entry points):
+ *   - every pointer argument is a device pointer unless stated otherwise;
+ *     buffers are caller-owned (the library allocates nothing);
+ *   - arrays are dense row-major, FP64 unless the name says f32;
+ *   - work is enqueued on `stream` (0 = the null stream) and is
+ *     asynchronous; the return value reports argument and launch errors
+ *     only;
+ *   - return codes: GH_OK, GH_EINVAL (bad size or null pointer),
+ *     GH_EUNSUPPORTED (a size this build has no kernel for), GH_ELAUNCH.
+ *
+ * Reference citations are paths relative to the reference's solver/ tree.
+ */
+#ifndef GEARHIP_H
+#define GEARHIP_H
+
+#include <stdint.h>
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GH_OK 0
+#define GH_EINVAL 1
+#define GH_EUNSUPPORTED 2
+#define GH_ELAUNCH 3
+
+/* Library version, major * 10000 + minor * 100 + patch (host only). */
+int gh_version(void);
+
+/*
+ * CROWN lower bounds of a QCQP over B boxes, FP64.
+ *
+ * Replaces, for programs of the form f_s(x) = x^T Q_s x + c_s^T x + d_s
+ * (s = 0 objective, s >= 1 constraints), the call
+ *   BoundedModule.compute_bounds(x=(BoundedTensor(x, PerturbationLpNorm(
+ *       x_L=xL, x_U=xU)),), C=eye(S), method='CROWN', bound_upper=False,
+ *       return_A=True, needed_A_dict={out: {in}})
+ * (auto_LiRPA/bound_general.py:1275-1593, backward_bound.py:880-1206), as
+ * bound by input_split/bounding.py:316-326.
+ *
+ *   Q (S, n, n), Cm (S, n), d (S): problem data;
+ *   xL, xU (B, n): boxes;
+ *   lb (B, S): lower bounds (out);
+ *   lA (B, S, n), lbias (B, S): the linear lower planes lA x + lbias
+ *   before concretization (out, either may be NULL).
+ * Supported: 1 <= n <= 64.
+ */
+int gh_qcqp_crown_f64(int B, int n, int S, const double *Q, const double *Cm,
+                      const double *d, const double *xL, const double *xU,
+                      double *lb, double *lA, double *lbias,
+                      hipStream_t stream);
+
+/*
+ * Widest-variable branching: idx[b] = argmax_i (xU[b,i] - xL[b,i]), the
+ * lowest index on ties.  Replaces input_split_branching(...,
+ * branching_method='naive') (complete_verifier/input_split/
+ * branching_heuristics.py:59-86).  idx is int32 (B).
+ */
+int gh_branch_widest_f64(int B, int n, const double *xL, const double *xU,
+                         int32_t *idx, hipStream_t stream);
+
+/*
+ * Bisection of each box on dimension idx[b] at (xL + xU) / 2: child 0 is
+ * the upper half, child 1 the lower half; children are written
+ * [all child 0; all child 1] into oL, oU (2B, n).  Replaces
+ * input_split_parallel(..., split_depth=1, split_partitions=2)
+ * (complete_verifier/input_split/split.py:26-122).
+ */
+int gh_split_f64(int B, int n, const double *xL, const double *xU,
+                 const int32_t *idx, double *oL, double *oU,
+                 hipStream_t stream);
+
+/* Bytes of device workspace gh_filter_append_f64 needs for B boxes. */
+size_t gh_filter_workspace_bytes(int B);
+
+/*
+ * Prune verified boxes and append the survivors to a frontier.  A box is
+ * dropped when some row has finite lb and threshold with lb > thr, when
+ * the objective row has lb >= thr, when the objective lb is +inf with a
+ * finite threshold, or when its box is invalid (xL > xU, all finite).
+ * Replaces UnsortedInputDomainList.filter_verified_domains /
+ * get_remaining_index (complete_verifier/input_split/
+ * branching_domains.py:238-558) followed by add (:560-830).
+ *
+ *   lb (B, S), thr (S): thresholds shared by all boxes (row 0 the
+ *   incumbent, +inf if none; rows >= 1 rhs + constr_viol_tol);
+ *   cL, cU (B, n): boxes;
+ *   FL, FU (cap, n), Flb (cap, S): frontier; survivors are written in
+ *   ascending box order starting at row `base` (host value);
+ *   total (device int64): number of survivors (out);
+ *   disc_min (device double, in/out): lowered to the smallest finite
+ *   objective threshold of a dropped box
+ *   (branching_domains.py:489-496);
+ *   ws: gh_filter_workspace_bytes(B) bytes.
+ * The caller guarantees base + B <= cap.
+ */
+int gh_filter_append_f64(int B, int S, int n, const double *lb,
+                         const double *thr, const double *cL,
+                         const double *cU, double *FL, double *FU,
+                         double *Flb, int64_t base, int64_t cap,
+                         int64_t *total, double *disc_min, void *ws,
+                         hipStream_t stream);
+
+/*
+ * out (device double, in/out) = min(out, min_b x[b * stride]) over
+ * b < N.  Used for the global lower bound over the frontier's objective
+ * row (batch_branch_and_bound.py:620-634).
+ */
+int gh_min_strided_f64(int64_t N, int64_t stride, const double *x,
+                       double *out, hipStream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GEARHIP_H */

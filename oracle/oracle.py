@@ -1,0 +1,98 @@
+"""CPU oracle -- test infrastructure only.
+
+Python face of the C restatement in oracle/crown_qcqp.c (FP64 CROWN of a
+QCQP with the reference's rounding enclosures, widest branching, bisection,
+prune).  Only tests/, the smoke check and the benchmark's CPU-baseline leg
+may import this module; the product path never does.
+
+Every function is pinned in tests/test_oracle.py against what fixes it
+independently: sampled soundness (lb <= f(x) at vertices and random points),
+closed forms (a pure linear row gives the exact box minimum within the
+enclosure margin; a single bilinear term gives the exact McCormick minimum),
+brute force on tiny inputs, and the reference's own explicit expectations
+for split / branch / prune (CV/tests/test_input_split_split.py and
+branching_domains.py semantics).
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+_lib = None
+_D = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_I32 = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_U8 = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    return LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        h = ctypes.CDLL(LIB_PATH)
+        h.gho_qcqp_crown.argtypes = [ctypes.c_int] * 3 + [_D] * 5 + [ctypes.c_void_p] * 3 + [ctypes.c_int]
+        h.gho_qcqp_crown.restype = ctypes.c_int
+        h.gho_branch_widest.argtypes = [ctypes.c_int, ctypes.c_int, _D, _D, _I32]
+        h.gho_branch_widest.restype = None
+        h.gho_split.argtypes = [ctypes.c_int, ctypes.c_int, _D, _D, _I32, _D, _D]
+        h.gho_split.restype = None
+        h.gho_prune.argtypes = [ctypes.c_int] * 3 + [_D] * 4 + [_U8, ctypes.POINTER(ctypes.c_double)]
+        h.gho_prune.restype = ctypes.c_long
+        _lib = h
+    return _lib
+
+
+def _c(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def qcqp_crown(Q, c, d, xL, xU, return_A=False, threads=1):
+    """lb (B, S) [, lA (B, S, n), lbias (B, S)] -- FP64 oracle bound."""
+    Q, c, d, xL, xU = map(_c, (Q, c, d, xL, xU))
+    S, n = Q.shape[0], Q.shape[1]
+    B = xL.shape[0]
+    lb = np.empty((B, S))
+    lA = np.empty((B, S, n)) if return_A else None
+    lbias = np.empty((B, S)) if return_A else None
+    p = lambda a: None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+    lib().gho_qcqp_crown(B, n, S, Q, c, d, xL, xU, p(lb), p(lA), p(lbias), threads)
+    if return_A:
+        return lb, lA, lbias
+    return lb
+
+
+def branch_widest(xL, xU):
+    xL, xU = _c(xL), _c(xU)
+    idx = np.empty(xL.shape[0], dtype=np.int32)
+    lib().gho_branch_widest(xL.shape[0], xL.shape[1], xL, xU, idx)
+    return idx
+
+
+def split(xL, xU, idx):
+    xL, xU = _c(xL), _c(xU)
+    B, n = xL.shape
+    oL = np.empty((2 * B, n))
+    oU = np.empty((2 * B, n))
+    lib().gho_split(B, n, xL, xU, np.ascontiguousarray(idx, dtype=np.int32), oL, oU)
+    return oL, oU
+
+
+def prune(lb, thr_rows, xL, xU, disc_min=np.inf):
+    """keep mask (B,), kept count, updated discarded-threshold minimum.
+    thr_rows (S,) is shared by every box."""
+    lb, xL, xU = _c(lb), _c(xL), _c(xU)
+    B, S = lb.shape
+    thr = np.ascontiguousarray(np.broadcast_to(np.asarray(thr_rows, dtype=np.float64), (B, S)))
+    keep = np.empty(B, dtype=np.uint8)
+    dm = ctypes.c_double(disc_min)
+    kept = lib().gho_prune(B, S, xL.shape[1], lb, thr, xL, xU, keep, ctypes.byref(dm))
+    return keep.astype(bool), int(kept), dm.value

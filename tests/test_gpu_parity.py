@@ -1,0 +1,150 @@
+"""HIP path vs the oracle, through the C ABI: FP64 bounds, planes, branch,
+split and prune are compared bit for bit."""
+import numpy as np
+import pytest
+import torch
+
+from gear_amd.problems import QCQP, qcqp_dense, random_bisection_boxes
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _dev(a, cuda):
+    return torch.as_tensor(np.ascontiguousarray(a), dtype=torch.float64, device=cuda)
+
+
+def _bits_equal(a, b):
+    a = np.ascontiguousarray(a, dtype=np.float64).view(np.int64)
+    b = np.ascontiguousarray(b, dtype=np.float64).view(np.int64)
+    return np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("n,m,B", [(50, 20, 1000), (8, 3, 1024), (1, 0, 33), (7, 2, 65),
+                                   (16, 4, 31), (17, 1, 100), (33, 5, 257), (64, 3, 200)])
+def test_qcqp_crown_f64_bit_exact(cuda, n, m, B):
+    from gear_amd.crown import BoundedQCQP
+    print(f"case n={n} m={m} B={B}", flush=True)
+    p = qcqp_dense(n, m, seed=n + m)
+    L, U = random_bisection_boxes(p.xl, p.xu, B, seed=B)
+    lb_o, lA_o, lbias_o = O.qcqp_crown(p.Q, p.c, p.d, L, U, return_A=True)
+    net = BoundedQCQP(p, cuda)
+    lb, lA, lbias = net.bound(_dev(L, cuda), _dev(U, cuda), return_A=True)
+    torch.cuda.synchronize()
+    assert _bits_equal(lb.cpu().numpy(), lb_o)
+    assert _bits_equal(lA.cpu().numpy(), lA_o)
+    assert _bits_equal(lbias.cpu().numpy(), lbias_o)
+
+
+def test_compute_bounds_interface(cuda):
+    """compute_bounds returns (lb, None, A_dict) shaped like the reference."""
+    from gear_amd.crown import BoundedQCQP, BoundedTensor, PerturbationLpNorm
+    p = qcqp_dense(10, 3, seed=1)
+    L, U = random_bisection_boxes(p.xl, p.xu, 12, seed=2)
+    net = BoundedQCQP(p, cuda)
+    xl, xu = _dev(L, cuda), _dev(U, cuda)
+    x = BoundedTensor((xl + xu) / 2, PerturbationLpNorm(x_L=xl, x_U=xu))
+    lb, ub, A = net.compute_bounds(x=(x,), method="CROWN", bound_upper=False, return_A=True,
+                                   needed_A_dict={net.output_name: [net.input_name]})
+    assert ub is None and lb.shape == (12, 4)
+    rec = A[net.output_name][net.input_name]
+    assert rec["lA"].shape == (12, 4, 10) and rec["lbias"].shape == (12, 4)
+    assert _bits_equal(lb.cpu().numpy(), O.qcqp_crown(p.Q, p.c, p.d, L, U))
+
+
+def test_degenerate_and_single_boxes(cuda):
+    from gear_amd.crown import BoundedQCQP
+    p = qcqp_dense(12, 2, seed=4)
+    L, U = random_bisection_boxes(p.xl, p.xu, 40, seed=3)
+    L[5] = U[5]            # a point box
+    L[6, 3] = U[6, 3]      # one zero-width coordinate
+    net = BoundedQCQP(p, cuda)
+    lb = net.bound(_dev(L, cuda), _dev(U, cuda)).cpu().numpy()
+    assert _bits_equal(lb, O.qcqp_crown(p.Q, p.c, p.d, L, U))
+    one = net.bound(_dev(L[:1], cuda), _dev(U[:1], cuda)).cpu().numpy()
+    assert _bits_equal(one, lb[:1])
+    empty = net.bound(_dev(L[:0], cuda), _dev(U[:0], cuda))
+    assert empty.shape == (0, 3)
+
+
+def test_batch_independence(cuda):
+    """A box's bound does not depend on the batch it is bound in."""
+    from gear_amd.crown import BoundedQCQP
+    p = qcqp_dense(50, 20, seed=3)
+    L, U = random_bisection_boxes(p.xl, p.xu, 300, seed=9)
+    net = BoundedQCQP(p, cuda)
+    full = net.bound(_dev(L, cuda), _dev(U, cuda)).cpu().numpy()
+    part = net.bound(_dev(L[117:250], cuda), _dev(U[117:250], cuda)).cpu().numpy()
+    assert _bits_equal(full[117:250], part)
+
+
+def test_branch_split_bit_exact(cuda):
+    from gear_amd.crown import input_split_branching, input_split_parallel
+    rng = np.random.default_rng(0)
+    B, n = 777, 50
+    L = rng.standard_normal((B, n))
+    U = L + np.abs(rng.standard_normal((B, n)))
+    U[::7, 3] = L[::7, 3] + (U[::7, 0] - L[::7, 0])   # ties
+    idx = input_split_branching(None, None, _dev(L, cuda), _dev(U, cuda), None, None, "naive")
+    idx_o = O.branch_widest(L, U)
+    assert np.array_equal(idx[:, 0].cpu().numpy(), idx_o)
+    oL, oU, depth = input_split_parallel(_dev(L, cuda), _dev(U, cuda), i_idx=idx)
+    eL, eU = O.split(L, U, idx_o)
+    assert depth == 1
+    assert _bits_equal(oL.cpu().numpy(), eL) and _bits_equal(oU.cpu().numpy(), eU)
+
+
+@pytest.mark.parametrize("B", [1, 255, 256, 257, 5000, 70000])
+def test_filter_append_matches_oracle(cuda, B):
+    from gear_amd.crown import DeviceFrontier
+    rng = np.random.default_rng(B)
+    n, S = 6, 4
+    L = rng.standard_normal((B, n))
+    U = L + np.abs(rng.standard_normal((B, n)))
+    U[::13, 2] = L[::13, 2] - 1.0                    # invalid boxes
+    lb = rng.standard_normal((B, S))
+    lb[::17, 0] = np.inf
+    lb[::19, 1] = np.nan
+    thr = np.array([0.8, 1.0, 1.2, np.inf])
+    keep, kept, dm = O.prune(lb, thr, L, U)
+    f = DeviceFrontier(n, S, cap=B + 10, device=cuda)
+    f.push(_dev(np.zeros((3, n)), cuda), _dev(np.ones((3, n)), cuda),
+           _dev(np.zeros((3, S)), cuda))
+    got = f.filter_append(_dev(lb, cuda), _dev(thr, cuda), _dev(L, cuda), _dev(U, cuda))
+    assert got == kept and len(f) == 3 + kept
+    assert _bits_equal(f.L[3:3 + kept].cpu().numpy(), L[keep])
+    assert _bits_equal(f.U[3:3 + kept].cpu().numpy(), U[keep])
+    assert _bits_equal(f.lb[3:3 + kept].cpu().numpy(), lb[keep])
+    assert float(f.disc_min.item()) == dm
+    glb = float(f.global_lb().item())
+    assert glb == min(dm, np.nanmin(np.concatenate([np.zeros(3), lb[keep][:, 0]])))
+
+
+def test_bab_iterations_match_oracle_loop(cuda):
+    """Three B&B iterations on the device reproduce the oracle-driven loop
+    (pick from the end, widest branch, bisect, bound, prune, append)."""
+    from gear_amd.bab import BaBConfig, QCQPBranchAndBound
+    p = qcqp_dense(10, 3, seed=11)
+    bb = QCQPBranchAndBound(p, BaBConfig(batch=64, cap=4096), device=cuda)
+    bb.root(alm=False)
+    bb.set_incumbent(-1.0, None)       # an injected incumbent prunes part of the tree
+    # oracle mirror
+    FL, FU = p.xl[None].copy(), p.xu[None].copy()
+    Flb = O.qcqp_crown(p.Q, p.c, p.d, FL, FU)
+    thr = np.concatenate([[-1.0], p.rhs[1:] + 1e-6])
+    for it in range(6):
+        nb = min(64, len(FL))
+        pL, pU = FL[-nb:], FU[-nb:]
+        FL, FU, Flb = FL[:-nb], FU[:-nb], Flb[:-nb]
+        idx = O.branch_widest(pL, pU)
+        cL, cU = O.split(pL, pU, idx)
+        lb = O.qcqp_crown(p.Q, p.c, p.d, cL, cU)
+        keep, _, _ = O.prune(lb, thr, cL, cU)
+        FL = np.vstack([FL, cL[keep]])
+        FU = np.vstack([FU, cU[keep]])
+        Flb = np.vstack([Flb, lb[keep]])
+        bb.step()
+        c = len(bb.frontier)
+        assert c == len(FL), it
+        assert _bits_equal(bb.frontier.L[:c].cpu().numpy(), FL)
+        assert _bits_equal(bb.frontier.lb[:c].cpu().numpy(), Flb)
