@@ -5,7 +5,8 @@ x in [-1, 1]^50; the loop bounds (CROWN, FP64), branches on the widest
 variable, prunes against the ALM-PGD incumbent and the constraint rows, and
 appends survivors to a device frontier of at most 2^18 domains.  One step =
 one B&B iteration: `batch` parents picked from the frontier -> 2 * batch
-children bounded.  The root ALM-PGD search runs once before the warm-up.
+children bounded.  The loop starts from the root bisected (widest variable
+first) into `batch` leaves; the root ALM-PGD search runs once before that.
 
 Prints one JSON line (see the keys below).  The CPU baseline times the FP64
 oracle on the host cores over a bounded sample of the same children; the
@@ -32,7 +33,21 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-alm", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB device-to-device 5 times (a known-size "
+                         "reference for the HBM byte counters) and exit")
     return ap.parse_args()
+
+
+def copy_probe(mib):
+    import torch
+    n = mib * (1 << 20) // 8
+    a = torch.ones(n, dtype=torch.float64, device="cuda:0")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(f"copy probe: {mib} MiB read + {mib} MiB written per launch", flush=True)
 
 
 def main():
@@ -40,6 +55,9 @@ def main():
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
         sys.exit(2)
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
     import numpy as np
     import torch
 
@@ -52,12 +70,13 @@ def main():
     p = config2_problem()
     n, S = p.n, p.S
     cap = 1 << 18
-    need = args.batch * (args.warmup + args.steps) + 1
+    need = args.batch * (args.warmup + args.steps + 1)
     if need > cap:
         raise SystemExit(f"batch * (warmup + steps) = {need} exceeds the frontier cap {cap}")
     bb = QCQPBranchAndBound(p, BaBConfig(batch=args.batch, cap=cap), device=dev)
     t_alm = time.time()
-    bb.root(alm=not args.no_alm)
+    # presplit start: `batch` leaves so every step runs a full batch
+    bb.presplit(args.batch, alm=not args.no_alm)
     torch.cuda.synchronize()
     t_alm = time.time() - t_alm
     stream = torch.cuda.current_stream()

@@ -95,6 +95,23 @@ class QCQPBranchAndBound:
         self.frontier.filter_append(lb, self.thr, L, U)
         self.update_dual()
 
+    def presplit(self, leaves, alm=True):
+        """Start from the root box bisected on the widest variable level by
+        level into `leaves` boxes (a power of two), all bound and filtered in
+        one batch -- the presplit-domains start of the reference
+        (batch_branch_and_bound.py:2964-3061)."""
+        L = torch.as_tensor(self.p.xl, dtype=torch.float64, device=self.device)[None]
+        U = torch.as_tensor(self.p.xu, dtype=torch.float64, device=self.device)[None]
+        if alm:
+            self.run_alm(L, U)
+        while L.shape[0] < leaves:
+            idx = input_split_branching(self.net, None, L, U, None, None, "naive")
+            L, U, _ = input_split_parallel(L, U, i_idx=idx)
+        lb = self.net.bound(L, U)
+        self.frontier.filter_append(lb, self.thr, L, U)
+        self.stats.boxes_bounded += L.shape[0]
+        self.update_dual()
+
     def update_dual(self):
         if len(self.frontier) == 0:
             self.D = float(self.frontier.disc_min.item())

@@ -269,8 +269,8 @@ extern "C" int gh_qcqp_crown_f64(int B, int n, int S, const double* Q, const dou
                                  double* lA, double* lbias, hipStream_t stream)
 {
     if (B < 0 || n <= 0 || S <= 0) return GH_EINVAL;
-    if (!Q || !Cm || !d || !xL || !xU || !lb) return GH_EINVAL;
     if (B == 0) return GH_OK;
+    if (!Q || !Cm || !d || !xL || !xU || !lb) return GH_EINVAL;
     if (n > 64) return GH_EUNSUPPORTED;
     const double eps = 2.220446049250313080847e-16;
     QcqpConsts k;
