@@ -101,14 +101,22 @@ def main():
     sample_parents = None
     if len(bb.frontier) >= args.batch:
         s0 = len(bb.frontier) - args.batch
-        sample_parents = (bb.frontier.L[s0:].clone(), bb.frontier.U[s0:].clone())
+        sample_parents = (bb.frontier.L[s0:s0 + args.batch].clone(),
+                          bb.frontier.U[s0:s0 + args.batch].clone())
+    import gc
+    gc.collect()
+    gc.disable()    # no collector pause inside the timed steps
     torch.cuda.synchronize()
     boxes = 0
+    step_ms = []
     t0 = time.perf_counter()
     for _ in range(args.steps):
+        ts = time.perf_counter()
         boxes += bb.step()
+        step_ms.append(round((time.perf_counter() - ts) * 1e3, 3))
     torch.cuda.synchronize()
     t1 = time.perf_counter()
+    gc.enable()
     bb.update_dual()
     wall = t1 - t0
     kt = [(a.elapsed_time(b) * 1e-3, nb) for a, b, nb in events]
@@ -150,6 +158,7 @@ def main():
             "avg_launch_ms": round(avg_launch * 1e3, 4),
             "boxes_per_launch": avg_boxes,
         },
+        "step_ms": step_ms,
         "bab": {"frontier": len(bb.frontier), "P": bb.P, "D": bb.D,
                 "root_alm_s": round(t_alm, 3)},
     }

@@ -9,7 +9,7 @@ import os
 import torch
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libgearhip.so")
+LIB_PATH = os.environ.get("GEAR_AMD_LIB") or os.path.join(HERE, "libgearhip.so")
 
 GH_OK, GH_EINVAL, GH_EUNSUPPORTED, GH_ELAUNCH = 0, 1, 2, 3
 _ERRORS = {GH_EINVAL: "invalid argument", GH_EUNSUPPORTED: "unsupported size",
@@ -22,6 +22,8 @@ _L = ctypes.c_int64
 # name -> argtypes (restype int unless listed in _RESTYPES)
 SIGNATURES = {
     "gh_version": [],
+    "gh_qcqp_pack_bytes": [_I, _I],
+    "gh_qcqp_pack_f64": [_I, _I, _P, _P, _P],
     "gh_qcqp_crown_f64": [_I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P],
     "gh_branch_widest_f64": [_I, _I, _P, _P, _P, _P],
     "gh_split_f64": [_I, _I, _P, _P, _P, _P, _P, _P],
@@ -29,7 +31,7 @@ SIGNATURES = {
     "gh_filter_append_f64": [_I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _L, _L, _P, _P, _P, _P],
     "gh_min_strided_f64": [_L, _L, _P, _P, _P],
 }
-_RESTYPES = {"gh_filter_workspace_bytes": ctypes.c_size_t}
+_RESTYPES = {"gh_filter_workspace_bytes": ctypes.c_size_t, "gh_qcqp_pack_bytes": ctypes.c_size_t}
 
 _lib = None
 

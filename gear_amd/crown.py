@@ -54,7 +54,10 @@ class BoundedQCQP:
         self.dev = DeviceQCQP(problem, device)
         self.device = self.dev.device
         self.n, self.S = problem.n, problem.S
-        lib()
+        nbytes = int(lib().gh_qcqp_pack_bytes(self.n, self.S))
+        self.packed = torch.empty(nbytes, dtype=torch.uint8, device=self.device)
+        check(lib().gh_qcqp_pack_f64(self.n, self.S, ptr(self.dev.Q), ptr(self.packed),
+                                     stream_ptr()), "gh_qcqp_pack_f64")
 
     def bound(self, x_L, x_U, return_A=False, out=None, stream=None):
         """lb (B, S) [, lA (B, S, n), lbias (B, S)] for boxes x_L, x_U (B, n)."""
@@ -72,7 +75,7 @@ class BoundedQCQP:
             lA = torch.empty(B, self.S, self.n, dtype=torch.float64, device=self.device)
             lbias = torch.empty(B, self.S, dtype=torch.float64, device=self.device)
         d = self.dev
-        check(lib().gh_qcqp_crown_f64(B, self.n, self.S, ptr(d.Q), ptr(d.c), ptr(d.d),
+        check(lib().gh_qcqp_crown_f64(B, self.n, self.S, ptr(self.packed), ptr(d.c), ptr(d.d),
                                       ptr(x_L), ptr(x_U), ptr(lb), ptr(lA), ptr(lbias),
                                       stream_ptr(stream)), "gh_qcqp_crown_f64")
         return (lb, lA, lbias) if return_A else lb

@@ -30,18 +30,42 @@ __device__ __forceinline__ double add_enclosed(double a, double b, double sign, 
     return widen(r, finite_abs(r) * k1, sign);
 }
 
-// xor tree over the 16 lanes of a group: p[l] += p[l^8], p[l^4], p[l^2],
-// p[l^1].  Every lane of the group ends with the same value.
+// DPP lane move of a double (two 32-bit halves).
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v)
+{
+    int2 h = __builtin_bit_cast(int2, v);
+    h.x = __builtin_amdgcn_update_dpp(0, h.x, CTRL, 0xF, 0xF, false);
+    h.y = __builtin_amdgcn_update_dpp(0, h.y, CTRL, 0xF, 0xF, false);
+    return __builtin_bit_cast(double, h);
+}
+template <int CTRL>
+__device__ __forceinline__ float dpp_f32(float v)
+{
+    return __builtin_bit_cast(
+        float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, 0xF, 0xF, false));
+}
+
+// xor tree over the 16 lanes of a group: p[l] += p[l^8]; p[l] += p[l^4];
+// p[l] += p[l^2]; p[l] += p[l^1].  Every lane ends with the same value.
+// The moves are DPP (no LDS): row_ror:8 is exactly lane l^8; after that
+// level p[x] == p[x^8], so row_ror:4 delivers a value equal to p[l^4];
+// quad_perm [2,3,0,1] and [1,0,3,2] are l^2 and l^1.  Each addition is
+// p[l] + p[l^off], the same operands as the oracle's tree.
 __device__ __forceinline__ double tree16(double v)
 {
-#pragma unroll
-    for (int off = 8; off >= 1; off >>= 1) v = v + __shfl_xor(v, off, GH_LANES);
+    v = v + dpp_f64<0x128>(v);   // row_ror:8
+    v = v + dpp_f64<0x124>(v);   // row_ror:4
+    v = v + dpp_f64<0x4E>(v);    // quad_perm 2,3,0,1
+    v = v + dpp_f64<0xB1>(v);    // quad_perm 1,0,3,2
     return v;
 }
 __device__ __forceinline__ float tree16(float v)
 {
-#pragma unroll
-    for (int off = 8; off >= 1; off >>= 1) v = v + __shfl_xor(v, off, GH_LANES);
+    v = v + dpp_f32<0x128>(v);
+    v = v + dpp_f32<0x124>(v);
+    v = v + dpp_f32<0x4E>(v);
+    v = v + dpp_f32<0xB1>(v);
     return v;
 }
 

@@ -33,6 +33,16 @@ extern "C" {
 int gh_version(void);
 
 /*
+ * Problem setup for the QCQP bound: packs Q (S, n, n) once into the MFMA
+ * operand order the bound kernels stream (the immutable per-program plan
+ * of this ABI).  `packed` holds gh_qcqp_pack_bytes(n, S) bytes (0 when the
+ * size is unsupported).  Supported: 1 <= n <= 64.
+ */
+size_t gh_qcqp_pack_bytes(int n, int S);
+int gh_qcqp_pack_f64(int n, int S, const double *Q, void *packed,
+                     hipStream_t stream);
+
+/*
  * CROWN lower bounds of a QCQP over B boxes, FP64.
  *
  * Replaces, for programs of the form f_s(x) = x^T Q_s x + c_s^T x + d_s
@@ -43,16 +53,16 @@ int gh_version(void);
  * (auto_LiRPA/bound_general.py:1275-1593, backward_bound.py:880-1206), as
  * bound by input_split/bounding.py:316-326.
  *
- *   Q (S, n, n), Cm (S, n), d (S): problem data;
+ *   packed: gh_qcqp_pack_f64 of Q (S, n, n); Cm (S, n), d (S);
  *   xL, xU (B, n): boxes;
  *   lb (B, S): lower bounds (out);
  *   lA (B, S, n), lbias (B, S): the linear lower planes lA x + lbias
  *   before concretization (out, either may be NULL).
- * Supported: 1 <= n <= 64.
+ * Supported: 1 <= n <= 64.  Results are bit-identical to the FP64 oracle.
  */
-int gh_qcqp_crown_f64(int B, int n, int S, const double *Q, const double *Cm,
-                      const double *d, const double *xL, const double *xU,
-                      double *lb, double *lA, double *lbias,
+int gh_qcqp_crown_f64(int B, int n, int S, const void *packed,
+                      const double *Cm, const double *d, const double *xL,
+                      const double *xU, double *lb, double *lA, double *lbias,
                       hipStream_t stream);
 
 /*
