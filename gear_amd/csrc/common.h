@@ -69,4 +69,42 @@ __device__ __forceinline__ float tree16(float v)
     return v;
 }
 
+// Values of lane l^16 and lane l^32 (gfx950 permlane swaps, no LDS).
+__device__ __forceinline__ double xor16_f64(double v)
+{
+    int2 h = __builtin_bit_cast(int2, v);
+    auto lo = __builtin_amdgcn_permlane16_swap((unsigned)h.x, (unsigned)h.x, false, false);
+    auto hi = __builtin_amdgcn_permlane16_swap((unsigned)h.y, (unsigned)h.y, false, false);
+    // after the swap: element 0 holds rows (0,0,2,2), element 1 rows (1,1,3,3)
+    bool even_row = ((threadIdx.x >> 4) & 1) == 0;
+    int2 r;
+    r.x = (int)(even_row ? lo[1] : lo[0]);
+    r.y = (int)(even_row ? hi[1] : hi[0]);
+    return __builtin_bit_cast(double, r);
+}
+__device__ __forceinline__ double xor32_f64(double v)
+{
+    int2 h = __builtin_bit_cast(int2, v);
+    auto lo = __builtin_amdgcn_permlane32_swap((unsigned)h.x, (unsigned)h.x, false, false);
+    auto hi = __builtin_amdgcn_permlane32_swap((unsigned)h.y, (unsigned)h.y, false, false);
+    // after the swap: element 0 holds halves (lo, lo), element 1 (hi, hi)
+    bool low_half = (threadIdx.x & 63) < 32;
+    int2 r;
+    r.x = (int)(low_half ? lo[1] : lo[0]);
+    r.y = (int)(low_half ? hi[1] : hi[0]);
+    return __builtin_bit_cast(double, r);
+}
+
+// The red16 tree when partial l = q + 4v lives in lane row q (lane / 16)
+// and register v: levels l^8 and l^4 are register pairs (v^2, v^1), levels
+// l^2 and l^1 are lanes ^32 and ^16.  Same operand pairs as the oracle.
+__device__ __forceinline__ double tree4x4(const double p[4])
+{
+    double q0 = p[0] + p[2], q1 = p[1] + p[3];
+    double r = q0 + q1;
+    r = r + xor32_f64(r);
+    r = r + xor16_f64(r);
+    return r;
+}
+
 }  // namespace gh

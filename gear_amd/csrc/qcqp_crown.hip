@@ -10,17 +10,19 @@
 // box concretization; fp_enclosure.py and backward_bound.py:550-823
 // rounding enclosures).
 //
-// Mapping.  One wavefront per 16 boxes (one block each, so there is no
-// block-level synchronisation).  Q is packed once per problem into MFMA
-// operand order (gh_qcqp_pack_f64), so every B operand is one coalesced
-// load that stays L2-resident across the batch.  Per row s:
-//   1. IBP of y_s = Q_s x on FP64 MFMA (16x16x4): rows = boxes, columns =
-//      j, contraction over i; centre, radius and magnitude share Q;
-//   2. McCormick plane choice and coefficients and the interval of z on the
-//      MFMA output layout (lane (q, r), item v <-> box q + 4v, j = 16c + r);
-//   3. beta -> LDS, backward A = beta^T Q_s on MFMA (contraction over j);
-//   4. merges, A67 charges and concretization as 16-lane partials (column
-//      r of every 16-wide tile) plus an xor tree over the 16 lanes of a box.
+// Mapping.  One wavefront per 16 boxes; lane 16q + r owns box r and its
+// variable indices q + 4m.  Q is packed once per problem into MFMA operand
+// order (gh_qcqp_pack_f64): every A operand is one coalesced, L2-resident
+// load, and the B operands are the lane's own box vectors, so nothing goes
+// through LDS.  Per row s:
+//   1. IBP of y_s = Q_s x on FP64 MFMA (16x16x4): D[j][box], contraction
+//      over i; centre, radius and magnitude share each Q tile;
+//   2. McCormick plane choice and coefficients and the interval of z, per
+//      (box, j) in the lane that owns them;
+//   3. backward A = beta^T Q_s on MFMA with the lane's beta as B operand;
+//   4. merges, A67 charges and concretization: partial sums per lane row q
+//      and register v (index class q + 4v mod 16), then the red16 tree
+//      (register pairs for l^8, l^4; lane swaps for l^2, l^1).
 // An FP64 MFMA k-step is the FMA chain over its 4 k in order, so each GEMM
 // output is the sequential FMA chain the oracle (oracle/crown_qcqp.c)
 // prescribes; with the same reduction tree the result is bit-identical.
@@ -68,181 +70,190 @@ __global__ void qcqp_pack_kernel(int n, int S, int nch, int kt, const double* __
     QA[u] = (j2 < n && i2 < n) ? Qs[(size_t)j2 * n + i2] : 0.0;
 }
 
-// One wavefront per 16 boxes; no block-level synchronisation.  LDS per wave:
-// beta and alpha (16 x K4 each, zero padded); box rows are read from global
-// memory (L1/L2 resident for the wave's lifetime).
-template <int NCH>
+// One wavefront per 16 boxes, one box per lane column: lane (q, r) = lane
+// 16q + r owns box r and the variable indices q + 4m, m < KT = ceil(n/4).
+// The MFMAs take Q tiles as the A operand and the lane's box vectors as the
+// B operand, so every per-box vector (box rows, Cm + alpha, beta) stays in
+// the lane's registers and no LDS is used.  IBP tiles are processed in
+// groups of two to bound the live accumulators.
+template <int KT>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void qcqp_crown_f64_kernel(
-    int B, int n, int S, int kt, const double* __restrict__ QB, const double* __restrict__ QA,
+    int B, int n, int S, const double* __restrict__ QB, const double* __restrict__ QA,
     const double* __restrict__ Cm, const double* __restrict__ d, const double* __restrict__ xL,
     const double* __restrict__ xU, double* __restrict__ lb, double* __restrict__ lA,
     double* __restrict__ lbias, QcqpConsts k)
 {
-    extern __shared__ double smem[];
-    const int K4 = 4 * kt;
-    double* sBeta = smem;              // 16 * K4
-    double* sAlpha = sBeta + 16 * K4;  // 16 * K4
-
+    constexpr int NCH = (KT + 3) / 4;    // 16-wide tiles of j (and of i)
+    constexpr int NM = KT;               // per-lane indices q + 4m
+    constexpr int G = (KT > 12) ? 1 : 2; // IBP tiles per group
     const int l = threadIdx.x;
     const int lr = l & 15, lq = l >> 4;
-    const int box0 = blockIdx.x * 16;
+    const int box = blockIdx.x * 16 + lr;
+    const int bo = min(box, B - 1);      // rows past B compute on box B-1, never stored
+    const bool live = box < B;
 
-    for (int t = l; t < 16 * K4; t += 64) sBeta[t] = 0.0;
-    __syncthreads();
-    // Box rows this lane reads: the A-operand box (lr) and, on the output
-    // layout, boxes lq + 4v.  Rows past B read box B-1 (never stored).
-    const int bA = min(box0 + lr, B - 1);
-    const double* aL = xL + (size_t)bA * n;
-    const double* aU = xU + (size_t)bA * n;
+    double xl[NM], xu[NM];
+#pragma unroll
+    for (int m = 0; m < NM; ++m) {
+        int i = lq + 4 * m;
+        xl[m] = (i < n) ? xL[(size_t)bo * n + i] : 0.0;
+        xu[m] = (i < n) ? xU[(size_t)bo * n + i] : 0.0;
+    }
 
-    const size_t rowstride = (size_t)NCH * kt * 64;
+    const size_t rowstride = (size_t)NCH * KT * 64;
     for (int s = 0; s < S; ++s) {
         const double* qb = QB + (size_t)s * rowstride + l;
         const double* qa = QA + (size_t)s * rowstride + l;
-
-        // ---- 1. IBP of y_s on MFMA: rows = the 16 boxes, columns = j,
-        // contraction over i (concretize_func.py:548-551, linear.py:607-653,
-        // backward_bound.py:739-741 for centre / radius / magnitude).
-        d4 Dc[NCH], Dd[NCH], Dm[NCH];
-#pragma unroll
-        for (int c = 0; c < NCH; ++c) Dc[c] = Dd[c] = Dm[c] = (d4){0.0, 0.0, 0.0, 0.0};
-        double qn[NCH];
-#pragma unroll
-        for (int c = 0; c < NCH; ++c) qn[c] = qb[(size_t)c * kt * 64];
-        for (int t = 0; t < kt; ++t) {
-            double q[NCH];
-#pragma unroll
-            for (int c = 0; c < NCH; ++c) q[c] = qn[c];
-            if (t + 1 < kt) {
-#pragma unroll
-                for (int c = 0; c < NCH; ++c) qn[c] = qb[((size_t)c * kt + t + 1) * 64];
-            }
-            int i = 4 * t + lq;
-            double lo = 0.0, hi = 0.0;
-            if (i < n) { lo = aL[i]; hi = aU[i]; }
-            double x0 = (hi + lo) / 2.0;
-            double ep = (hi - lo) / 2.0;
-            double mx = gh::finite_abs(fmax(-lo, hi));
-#pragma unroll
-            for (int c = 0; c < NCH; ++c) {
-                double qa_ = fabs(q[c]);
-                Dc[c] = mfma16(x0, q[c], Dc[c]);
-                Dd[c] = mfma16(ep, qa_, Dd[c]);
-                Dm[c] = mfma16(mx, qa_, Dm[c]);
-            }
-        }
-
-        // ---- 2. Elementwise on the MFMA output layout: item v of lane
-        // (lq, lr) is box lq + 4v, column j = 16c + lr.
         const double* cs = Cm + (size_t)s * n;
+
+        double acm[NM], beta[NM];
         double p_ql[4], p_qu[4], p_g[4], p_own[4];
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            const int slot = lq + 4 * v;
-            const int bo = min(box0 + slot, B - 1);
-            p_ql[v] = p_qu[v] = p_g[v] = p_own[v] = 0.0;
+        for (int v = 0; v < 4; ++v) p_ql[v] = p_qu[v] = p_g[v] = p_own[v] = 0.0;
+
 #pragma unroll
-            for (int c = 0; c < NCH; ++c) {
-                const int j = 16 * c + lr;
-                if (j < n) {
-                    double cc = Dc[c][v], dv = Dd[c][v], mg = Dm[c][v];
-                    double e = mg * k.kn;
-                    double yl = gh::widen((cc - dv) + 0.0, e, -1.0);
-                    double yu = gh::widen((cc + dv) + 0.0, e, 1.0);
-                    double xl = xL[(size_t)bo * n + j], xu = xU[(size_t)bo * n + j];
-                    // interval of z_j (bivariate.py:563-597)
-                    double r0 = xl * yl, r1 = xl * yu, r2 = xu * yl, r3 = xu * yu;
-                    p_ql[v] = p_ql[v] + fmin(fmin(r0, r1), fmin(r2, r3));
-                    p_qu[v] = p_qu[v] + fmax(fmax(r0, r1), fmax(r2, r3));
-                    // coeff-aware McCormick lower plane (bivariate.py:190-261, 40-75)
-                    double a1 = yl, b1 = xl, c1 = (-xl) * yl;
-                    double a2 = yu, b2 = xu, c2 = (-xu) * yu;
-                    double m1 = a1 * (a1 >= 0 ? xl : xu);
-                    m1 = m1 + b1 * (b1 >= 0 ? yl : yu);
-                    m1 = m1 + c1;
-                    double m2 = a2 * (a2 >= 0 ? xl : xu);
-                    m2 = m2 + b2 * (b2 >= 0 ? yl : yu);
-                    m2 = m2 + c2;
-                    double rr = (m1 >= m2) ? 1.0 : 0.0;
-                    double t = (yl - yu) * rr;
-                    double al = t + yu;
-                    t = (xl - xu) * rr;
-                    double be = t + xu;
-                    double pp = yu * xu, qq = yl * xl;
-                    t = (pp - qq) * rr;
-                    double ga = t - pp;
-                    sAlpha[slot * K4 + j] = al;
-                    sBeta[slot * K4 + j] = be;
-                    p_g[v] = p_g[v] + ga;
-                    p_own[v] = fma(fabs(be), k.hsn * mg, p_own[v]);
+        for (int c0 = 0; c0 < NCH; c0 += G) {
+            // ---- 1. IBP of y_s for tiles c0..c0+G-1: D[j][box] = sum_i
+            // Q[j][i] x[box][i] on MFMA (A = Q tile, B = the lane's x at
+            // index q + 4t); centre, radius and magnitude as in
+            // concretize_func.py:548-551, linear.py:607-653,
+            // backward_bound.py:739-741.
+            d4 Dc[G], Dd[G], Dm[G];
+#pragma unroll
+            for (int g = 0; g < G; ++g) Dc[g] = Dd[g] = Dm[g] = (d4){0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int t = 0; t < KT; ++t) {
+                double x0 = (xu[t] + xl[t]) / 2.0;
+                double ep = (xu[t] - xl[t]) / 2.0;
+                double mx = gh::finite_abs(fmax(-xl[t], xu[t]));
+#pragma unroll
+                for (int g = 0; g < G; ++g) {
+                    if (c0 + g < NCH) {
+                        double q = qb[((size_t)(c0 + g) * KT + t) * 64];
+                        double qa_ = fabs(q);
+                        Dc[g] = mfma16(q, x0, Dc[g]);
+                        Dd[g] = mfma16(qa_, ep, Dd[g]);
+                        Dm[g] = mfma16(qa_, mx, Dm[g]);
+                    }
+                }
+                if (t % 4 == 3) __builtin_amdgcn_sched_barrier(0);
+            }
+            // ---- 2. Elementwise: item v of tile c is index j = q + 4m,
+            // m = 4c + v, of box r.
+#pragma unroll
+            for (int g = 0; g < G; ++g) {
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int c = c0 + g, m = 4 * c + v;
+                    if (c < NCH && m < NM) {
+                        const int j = lq + 4 * m;
+                        acm[m] = 0.0;
+                        beta[m] = 0.0;
+                        if (j < n) {
+                            double cc = Dc[g][v], dv = Dd[g][v], mg = Dm[g][v];
+                            double e = mg * k.kn;
+                            double yl = gh::widen((cc - dv) + 0.0, e, -1.0);
+                            double yu = gh::widen((cc + dv) + 0.0, e, 1.0);
+                            double xlj = xl[m], xuj = xu[m];
+                            // interval of z_j (bivariate.py:563-597)
+                            double r0 = xlj * yl, r1 = xlj * yu, r2 = xuj * yl, r3 = xuj * yu;
+                            p_ql[v] = p_ql[v] + fmin(fmin(r0, r1), fmin(r2, r3));
+                            p_qu[v] = p_qu[v] + fmax(fmax(r0, r1), fmax(r2, r3));
+                            // coeff-aware McCormick lower plane
+                            // (bivariate.py:190-261, 40-75)
+                            double a1 = yl, b1 = xlj, c1 = (-xlj) * yl;
+                            double a2 = yu, b2 = xuj, c2 = (-xuj) * yu;
+                            double m1 = a1 * (a1 >= 0 ? xlj : xuj);
+                            m1 = m1 + b1 * (b1 >= 0 ? yl : yu);
+                            m1 = m1 + c1;
+                            double m2 = a2 * (a2 >= 0 ? xlj : xuj);
+                            m2 = m2 + b2 * (b2 >= 0 ? yl : yu);
+                            m2 = m2 + c2;
+                            double rr = (m1 >= m2) ? 1.0 : 0.0;
+                            double t = (yl - yu) * rr;
+                            double al = t + yu;
+                            t = (xlj - xuj) * rr;
+                            double be = t + xuj;
+                            double pp = yu * xuj, qq = yl * xlj;
+                            t = (pp - qq) * rr;
+                            double ga = t - pp;
+                            acm[m] = cs[j] + al;
+                            beta[m] = be;
+                            p_g[v] = p_g[v] + ga;
+                            p_own[v] = fma(fabs(be), k.hsn * mg, p_own[v]);
+                        }
+                    }
                 }
             }
         }
-        __syncthreads();   // one wave per block: orders the beta writes and reads
+        const double ql = gh::tree4x4(p_ql), qu = gh::tree4x4(p_qu);
+        const double gsum = gh::tree4x4(p_g), own = gh::tree4x4(p_own);
 
-        // ---- 3. Backward product A = beta^T Q_s on MFMA: rows = boxes,
-        // columns = i, contraction over j.
+        // ---- 3. Backward product: D[i][box] = sum_j Q[j][i] beta[box][j]
+        // (A = Q^T tile c, B = the lane's beta at index q + 4t).
         d4 Da[NCH];
 #pragma unroll
         for (int c = 0; c < NCH; ++c) Da[c] = (d4){0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-        for (int c = 0; c < NCH; ++c) qn[c] = qa[(size_t)c * kt * 64];
-        for (int t = 0; t < kt; ++t) {
-            double q[NCH];
+        for (int t = 0; t < KT; ++t) {
 #pragma unroll
-            for (int c = 0; c < NCH; ++c) q[c] = qn[c];
-            if (t + 1 < kt) {
-#pragma unroll
-                for (int c = 0; c < NCH; ++c) qn[c] = qa[((size_t)c * kt + t + 1) * 64];
-            }
-            double a = sBeta[lr * K4 + 4 * t + lq];
-#pragma unroll
-            for (int c = 0; c < NCH; ++c) Da[c] = mfma16(a, q[c], Da[c]);
+            for (int c = 0; c < NCH; ++c)
+                Da[c] = mfma16(qa[((size_t)c * KT + t) * 64], beta[t], Da[c]);
+            // keep the operand loads of later steps from being hoisted here
+            // (they would hold KT * NCH registers)
+            if (t % 4 == 3) __builtin_amdgcn_sched_barrier(0);
         }
 
-        // ---- 4. Merges, charges and concretization per box.
-        double csv[NCH];
+        // ---- 4. Merges, charges and concretization of box r, in two passes
+        // over the lane's indices to bound the live partial sums.
+        double p_lc[4], p_ldv[4], p_lmg[4], p_m1[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) p_lc[v] = p_ldv[v] = p_lmg[v] = p_m1[v] = 0.0;
 #pragma unroll
         for (int c = 0; c < NCH; ++c) {
-            int i = 16 * c + lr;
-            csv[c] = (i < n) ? cs[i] : 0.0;
-        }
-        const double ds = d[s];
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            const int slot = lq + 4 * v, b = box0 + slot;
-            const int bo = min(b, B - 1);
-            double p_lc = 0.0, p_ldv = 0.0, p_lmg = 0.0, p_m1 = 0.0, p_m2 = 0.0;
-            double p_base = 0.0, p_et = 0.0, p_ma = 0.0;
-            double Af[NCH];
-#pragma unroll
-            for (int c = 0; c < NCH; ++c) {
-                int i = 16 * c + lr;
-                Af[c] = 0.0;
-                if (i < n) {
-                    double lo = xL[(size_t)bo * n + i], hi = xU[(size_t)bo * n + i];
+            for (int v = 0; v < 4; ++v) {
+                const int m = 4 * c + v, i = lq + 4 * m;
+                if (m < NM && i < n) {
+                    double lo = xl[m], hi = xu[m];
                     double x0 = (hi + lo) / 2.0, ep = (hi - lo) / 2.0;
                     double mx = gh::finite_abs(fmax(-lo, hi));
-                    double ca = fabs(csv[c]);
-                    p_lc = fma(csv[c], x0, p_lc);
-                    p_ldv = fma(ca, ep, p_ldv);
-                    p_lmg = fma(ca, mx, p_lmg);
-                    double acm = csv[c] + sAlpha[slot * K4 + i];
-                    p_m1 = fma(fabs(acm), mx, p_m1);
-                    double a = acm + Da[c][v];
-                    Af[c] = a;
-                    p_m2 = fma(fabs(a), mx, p_m2);
-                    p_base = fma(a, x0, p_base);
-                    p_et = fma(fabs(a), ep, p_et);
-                    p_ma = fma(fabs(a), mx, p_ma);
+                    double csi = cs[i];
+                    double ca = fabs(csi);
+                    p_lc[v] = fma(csi, x0, p_lc[v]);
+                    p_ldv[v] = fma(ca, ep, p_ldv[v]);
+                    p_lmg[v] = fma(ca, mx, p_lmg[v]);
+                    p_m1[v] = fma(fabs(acm[m]), mx, p_m1[v]);
                 }
             }
-            double ql = gh::tree16(p_ql[v]), qu = gh::tree16(p_qu[v]);
-            double lc = gh::tree16(p_lc), ldv = gh::tree16(p_ldv), lmg = gh::tree16(p_lmg);
-            double gsum = gh::tree16(p_g[v]), own = gh::tree16(p_own[v]);
-            double merge1 = gh::tree16(p_m1), merge2 = gh::tree16(p_m2);
-            double base = gh::tree16(p_base), et = gh::tree16(p_et), ma = gh::tree16(p_ma);
-            if (b >= B) continue;
+        }
+        const double lc = gh::tree4x4(p_lc), ldv = gh::tree4x4(p_ldv), lmg = gh::tree4x4(p_lmg);
+        const double merge1 = gh::tree4x4(p_m1);
+        double p_m2[4], p_base[4], p_et[4], p_ma[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) p_m2[v] = p_base[v] = p_et[v] = p_ma[v] = 0.0;
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int m = 4 * c + v, i = lq + 4 * m;
+                if (m < NM && i < n) {
+                    double lo = xl[m], hi = xu[m];
+                    double x0 = (hi + lo) / 2.0, ep = (hi - lo) / 2.0;
+                    double mx = gh::finite_abs(fmax(-lo, hi));
+                    double a = acm[m] + Da[c][v];
+                    p_m2[v] = fma(fabs(a), mx, p_m2[v]);
+                    p_base[v] = fma(a, x0, p_base[v]);
+                    p_et[v] = fma(fabs(a), ep, p_et[v]);
+                    p_ma[v] = fma(fabs(a), mx, p_ma[v]);
+                    if (lA && live) lA[((size_t)box * S + s) * n + i] = a;
+                }
+            }
+        }
+        const double merge2 = gh::tree4x4(p_m2);
+        const double base = gh::tree4x4(p_base), et = gh::tree4x4(p_et), ma = gh::tree4x4(p_ma);
+        if (live && lq == 0) {
+            const double ds = d[s];
             double le = (lmg + fabs(ds)) * k.kn;
             double ll = gh::widen((lc - ldv) + ds, le, -1.0);
             double lu = gh::widen((lc + ldv) + ds, le, 1.0);
@@ -261,33 +272,20 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void qc
             double bias = gh::widen(a, gh::finite_abs(a) * k.kn, -1.0);
             double root = gh::widen(base - et, ma * k.kn, -1.0);
             if (!isfinite(root)) root = -INFINITY;
-            double out = gh::add_enclosed(bias, root, -1.0, k.k1);
-            if (lr == 0) {
-                lb[(size_t)b * S + s] = out;
-                if (lbias) lbias[(size_t)b * S + s] = a;
-            }
-            if (lA) {
-#pragma unroll
-                for (int c = 0; c < NCH; ++c) {
-                    int i = 16 * c + lr;
-                    if (i < n) lA[((size_t)b * S + s) * n + i] = Af[c];
-                }
-            }
+            lb[(size_t)box * S + s] = gh::add_enclosed(bias, root, -1.0, k.k1);
+            if (lbias) lbias[(size_t)box * S + s] = a;
         }
-        __syncthreads();   // beta of this row is consumed before the next row writes it
     }
 }
 
-template <int NCH>
+template <int KT>
 int launch_f64(int B, int n, int S, const double* QB, const double* QA, const double* Cm,
                const double* d, const double* xL, const double* xU, double* lb, double* lA,
                double* lbias, const QcqpConsts& k, hipStream_t stream)
 {
-    const int kt = (n + 3) / 4;
-    size_t lds = sizeof(double) * (size_t)2 * 16 * 4 * kt;
     dim3 grid((B + 15) / 16);
-    hipLaunchKernelGGL(qcqp_crown_f64_kernel<NCH>, grid, dim3(64), lds, stream, B, n, S, kt, QB,
-                       QA, Cm, d, xL, xU, lb, lA, lbias, k);
+    hipLaunchKernelGGL(qcqp_crown_f64_kernel<KT>, grid, dim3(64), 0, stream, B, n, S, QB, QA,
+                       Cm, d, xL, xU, lb, lA, lbias, k);
     return hipGetLastError() == hipSuccess ? GH_OK : GH_ELAUNCH;
 }
 
@@ -333,10 +331,12 @@ extern "C" int gh_qcqp_crown_f64(int B, int n, int S, const void* packed, const 
     int nch = nch_of(n), kt = (n + 3) / 4;
     const double* QB = (const double*)packed;
     const double* QA = QB + (size_t)S * nch * kt * 64;
-    switch (nch) {
-    case 1: return launch_f64<1>(B, n, S, QB, QA, Cm, d, xL, xU, lb, lA, lbias, k, stream);
-    case 2: return launch_f64<2>(B, n, S, QB, QA, Cm, d, xL, xU, lb, lA, lbias, k, stream);
-    case 3: return launch_f64<3>(B, n, S, QB, QA, Cm, d, xL, xU, lb, lA, lbias, k, stream);
-    default: return launch_f64<4>(B, n, S, QB, QA, Cm, d, xL, xU, lb, lA, lbias, k, stream);
+#define GH_KT(K) \
+    case K: return launch_f64<K>(B, n, S, QB, QA, Cm, d, xL, xU, lb, lA, lbias, k, stream);
+    switch (kt) {
+        GH_KT(1) GH_KT(2) GH_KT(3) GH_KT(4) GH_KT(5) GH_KT(6) GH_KT(7) GH_KT(8)
+        GH_KT(9) GH_KT(10) GH_KT(11) GH_KT(12) GH_KT(13) GH_KT(14) GH_KT(15) GH_KT(16)
+    default: return GH_EUNSUPPORTED;
     }
+#undef GH_KT
 }
