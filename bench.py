@@ -22,6 +22,10 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
 
 FP64_PEAK_TFLOPS = 78.6
+# HBM bytes per bound launch of 32,768 boxes, measured with rocprofv3
+# FETCH_SIZE (x2.00) + WRITE_SIZE (x1.00), scale factors from a 256 MiB copy
+# (profiles/qcqp_crown_f64_n50_m20.md); scaled to the launch size below.
+MEASURED_TRAFFIC_BYTES_32K = 74.1e6
 
 
 def parse():
@@ -152,7 +156,8 @@ def main():
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": round(achieved_tf / FP64_PEAK_TFLOPS, 4),
-            "traffic": None,
+            "traffic": round(MEASURED_TRAFFIC_BYTES_32K * avg_boxes / 32768),
+            "traffic_source": "rocprofv3 FETCH_SIZE/WRITE_SIZE, profiles/qcqp_crown_f64_n50_m20.md",
             "flops_per_box": fpb,
             "bytes_per_box": bytes_per_box(n, S),
             "avg_launch_ms": round(avg_launch * 1e3, 4),
