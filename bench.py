@@ -188,10 +188,30 @@ def main():
             orig_bound(cL, cU, out=fused)
         torch.cuda.synchronize()
         tf = (time.perf_counter() - tf) / reps
+        # FP32-sound form on the same children
+        from gear_amd.crown import BoundedQCQP
+        net32 = BoundedQCQP(p, dev, precision="fp32s")
+        lb32 = net32.bound(cL, cU)
+        torch.cuda.synchronize()
+        t3 = time.perf_counter()
+        for _ in range(reps):
+            net32.bound(cL, cU, out=lb32)
+        torch.cuda.synchronize()
+        t3 = (time.perf_counter() - t3) / reps
+        viol = int((lb32 > fused).sum())
+        loose = ((fused - lb32) / (fused.abs() + 1.0))
         result["plain_fp64_gpu"] = {"value": round(cL.shape[0] / tp, 1),
                                     "unit": "subdomain bounds/s",
                                     "fused_same_boxes": round(cL.shape[0] / tf, 1),
                                     "speedup": round(tp / tf, 2)}
+        result["fp32_sound"] = {"value": round(cL.shape[0] / t3, 1),
+                                "unit": "subdomain bounds/s",
+                                "vs_plain_fp64": round(tp / t3, 2),
+                                "vs_fused_fp64": round(tf / t3, 2),
+                                "violations_vs_fp64": viol,
+                                "near_tie_boxes_settled_fp64": int(net32.last_flag.sum()),
+                                "looseness_max": float(loose.max()),
+                                "looseness_mean": float(loose.mean())}
 
     if sample_parents is not None and not args.no_cpu:
         from oracle import oracle as O

@@ -25,13 +25,17 @@ SIGNATURES = {
     "gh_qcqp_pack_bytes": [_I, _I],
     "gh_qcqp_pack_f64": [_I, _I, _P, _P, _P],
     "gh_qcqp_crown_f64": [_I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P],
+    "gh_qcqp_pack_f32_bytes": [_I, _I],
+    "gh_qcqp_pack_f32": [_I, _I, _P, _P, _P],
+    "gh_qcqp_crown_f32s": [_I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P],
     "gh_branch_widest_f64": [_I, _I, _P, _P, _P, _P],
     "gh_split_f64": [_I, _I, _P, _P, _P, _P, _P, _P],
     "gh_filter_workspace_bytes": [_I],
     "gh_filter_append_f64": [_I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _L, _L, _P, _P, _P, _P],
     "gh_min_strided_f64": [_L, _L, _P, _P, _P],
 }
-_RESTYPES = {"gh_filter_workspace_bytes": ctypes.c_size_t, "gh_qcqp_pack_bytes": ctypes.c_size_t}
+_RESTYPES = {"gh_filter_workspace_bytes": ctypes.c_size_t, "gh_qcqp_pack_bytes": ctypes.c_size_t,
+             "gh_qcqp_pack_f32_bytes": ctypes.c_size_t}
 
 _lib = None
 

@@ -47,10 +47,15 @@ class BoundedQCQP:
     output_name = "/out"
     input_name = "/x"
 
-    def __init__(self, problem: QCQP, device="cuda"):
+    def __init__(self, problem: QCQP, device="cuda", precision="fp64"):
+        """precision: "fp64" (bit-identical to the FP64 oracle) or "fp32s"
+        (FP32 with a rigorous outward margin; never above the FP64 bound)."""
         if problem.n > 64:
-            raise _lib.GearHipError("the fused FP64 kernel covers n <= 64")
+            raise _lib.GearHipError("the fused QCQP kernels cover n <= 64")
+        if precision not in ("fp64", "fp32s"):
+            raise ValueError(f"precision {precision!r}")
         self.problem = problem
+        self.precision = precision
         self.dev = DeviceQCQP(problem, device)
         self.device = self.dev.device
         self.n, self.S = problem.n, problem.S
@@ -58,9 +63,18 @@ class BoundedQCQP:
         self.packed = torch.empty(nbytes, dtype=torch.uint8, device=self.device)
         check(lib().gh_qcqp_pack_f64(self.n, self.S, ptr(self.dev.Q), ptr(self.packed),
                                      stream_ptr()), "gh_qcqp_pack_f64")
+        self.packed32 = None
+        if precision == "fp32s":
+            nb32 = int(lib().gh_qcqp_pack_f32_bytes(self.n, self.S))
+            self.packed32 = torch.empty(nb32, dtype=torch.uint8, device=self.device)
+            check(lib().gh_qcqp_pack_f32(self.n, self.S, ptr(self.dev.Q), ptr(self.packed32),
+                                         stream_ptr()), "gh_qcqp_pack_f32")
+        self.last_flagged = 0
 
     def bound(self, x_L, x_U, return_A=False, out=None, stream=None):
         """lb (B, S) [, lA (B, S, n), lbias (B, S)] for boxes x_L, x_U (B, n)."""
+        if self.precision == "fp32s" and not return_A:
+            return self.bound_fp32s(x_L, x_U, out=out, stream=stream)
         x_L = x_L.contiguous()
         x_U = x_U.contiguous()
         if x_L.dtype != torch.float64 or x_U.dtype != torch.float64:
@@ -79,6 +93,27 @@ class BoundedQCQP:
                                       ptr(x_L), ptr(x_U), ptr(lb), ptr(lA), ptr(lbias),
                                       stream_ptr(stream)), "gh_qcqp_crown_f64")
         return (lb, lA, lbias) if return_A else lb
+
+    def bound_fp32s(self, x_L, x_U, out=None, stream=None):
+        """FP32-sound lower bounds (B, S) as float64 values, each <= the FP64
+        bound of the same box.  `last_flag` marks boxes where a near-tie
+        McCormick choice was settled in FP64 (diagnostic)."""
+        x_L = x_L.contiguous()
+        x_U = x_U.contiguous()
+        B = x_L.shape[0]
+        if x_L.shape != (B, self.n) or x_U.shape != (B, self.n):
+            raise ValueError(f"boxes must be (B, {self.n})")
+        if self.packed32 is None:
+            raise _lib.GearHipError("constructed without precision='fp32s'")
+        lb = out if out is not None else torch.empty(B, self.S, dtype=torch.float64,
+                                                     device=self.device)
+        flag = torch.empty(B, dtype=torch.int32, device=self.device)
+        d = self.dev
+        check(lib().gh_qcqp_crown_f32s(B, self.n, self.S, ptr(self.packed32), ptr(d.Q), ptr(d.c),
+                                       ptr(d.d), ptr(x_L), ptr(x_U), ptr(lb), ptr(flag),
+                                       stream_ptr(stream)), "gh_qcqp_crown_f32s")
+        self.last_flag = flag
+        return lb
 
     def compute_bounds(self, x=None, C=None, method="CROWN", bound_lower=True,
                        bound_upper=False, return_A=False, needed_A_dict=None, **kwargs):

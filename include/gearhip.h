@@ -66,6 +66,29 @@ int gh_qcqp_crown_f64(int B, int n, int S, const void *packed,
                       hipStream_t stream);
 
 /*
+ * FP32-sound form of gh_qcqp_crown_f64: the same relaxation evaluated in
+ * FP32 on MFMA with a rigorous outward error term, so each bound is a valid
+ * lower bound and not above the FP64 bound of gh_qcqp_crown_f64.  McCormick
+ * choices within the FP32/FP64 difference of a tie are settled with the
+ * FP64 path's exact arithmetic for that element; flag[b] = 1 records that
+ * this happened for box b (diagnostic only).  This form is new surface (the
+ * reference has FP64 only, CV/gear/config.py:219-221).
+ *
+ *   packed: gh_qcqp_pack_f32 of Q (gh_qcqp_pack_f32_bytes(n, S) bytes);
+ *   Q (S, n, n) FP64 (for the tie cases), Cm, d, xL, xU as
+ *   gh_qcqp_crown_f64 (FP64; boxes are rounded outward to FP32);
+ *   lb (B, S) FP64 (FP32 values); flag (B) int32.
+ * Supported: 1 <= n <= 64.
+ */
+size_t gh_qcqp_pack_f32_bytes(int n, int S);
+int gh_qcqp_pack_f32(int n, int S, const double *Q, void *packed,
+                     hipStream_t stream);
+int gh_qcqp_crown_f32s(int B, int n, int S, const void *packed,
+                       const double *Q, const double *Cm, const double *d,
+                       const double *xL, const double *xU, double *lb,
+                       int32_t *flag, hipStream_t stream);
+
+/*
  * Widest-variable branching: idx[b] = argmax_i (xU[b,i] - xL[b,i]), the
  * lowest index on ties.  Replaces input_split_branching(...,
  * branching_method='naive') (complete_verifier/input_split/

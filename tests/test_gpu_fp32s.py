@@ -1,0 +1,65 @@
+"""FP32-sound form: every bound is <= the FP64 oracle bound (zero
+violations), sound on sampled points, and its looseness is small."""
+import numpy as np
+import pytest
+import torch
+
+from gear_amd.problems import qcqp_dense, random_bisection_boxes
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("n,m,B", [(50, 20, 8192), (8, 3, 2048), (17, 4, 1000), (33, 6, 1500),
+                                   (64, 3, 1000), (1, 1, 64)])
+def test_fp32s_below_fp64_oracle(cuda, n, m, B):
+    from gear_amd.crown import BoundedQCQP
+    print(f"case n={n} m={m} B={B}", flush=True)
+    p = qcqp_dense(n, m, seed=100 + n)
+    L, U = random_bisection_boxes(p.xl, p.xu, B, seed=B + n)
+    ref = O.qcqp_crown(p.Q, p.c, p.d, L, U, threads=8)
+    net = BoundedQCQP(p, cuda, precision="fp32s")
+    lb = net.bound(torch.as_tensor(L, device=cuda), torch.as_tensor(U, device=cuda)).cpu().numpy()
+    assert np.all(np.isfinite(lb))
+    viol = lb > ref
+    assert not viol.any(), f"{viol.sum()} bounds above the FP64 oracle"
+    scale = np.abs(ref) + 1.0
+    loose = (ref - lb) / scale
+    flagged = int(net.last_flag.sum())
+    print(f"looseness max {loose.max():.3e} mean {loose.mean():.3e}, "
+          f"near-tie boxes settled in FP64 {flagged}/{B}", flush=True)
+    assert loose.max() < 1e-3
+
+
+def test_fp32s_sound_on_samples(cuda):
+    from gear_amd.crown import BoundedQCQP
+    p = qcqp_dense(12, 4, seed=7)
+    L, U = random_bisection_boxes(p.xl, p.xu, 256, seed=3)
+    net = BoundedQCQP(p, cuda, precision="fp32s")
+    lb = net.bound(torch.as_tensor(L, device=cuda), torch.as_tensor(U, device=cuda)).cpu().numpy()
+    rng = np.random.default_rng(0)
+    for b in range(len(L)):
+        x = L[b] + rng.random((200, 12)) * (U[b] - L[b])
+        assert np.all(p.values(x) >= lb[b])
+
+
+def test_fp32s_exact_ties(cuda):
+    """Symmetric boxes put McCormick choices exactly on ties; they are
+    settled with the FP64 choice, so the bound stays below FP64."""
+    from gear_amd.crown import BoundedQCQP
+    p = qcqp_dense(6, 2, seed=5)
+    # symmetric boxes around 0 make x-ranges and y-ranges sign-symmetric,
+    # which puts many McCormick choices exactly on a tie
+    B = 512
+    rng = np.random.default_rng(1)
+    w = rng.random((B, 6)) + 0.1
+    L, U = -w, w
+    net = BoundedQCQP(p, cuda, precision="fp32s")
+    tl, tu = torch.as_tensor(L, device=cuda), torch.as_tensor(U, device=cuda)
+    lb = net.bound(tl, tu)
+    flag = net.last_flag.bool().cpu().numpy()
+    ref = O.qcqp_crown(p.Q, p.c, p.d, L, U)
+    got = lb.cpu().numpy()
+    assert flag.any()
+    assert np.all(got <= ref)
+    assert np.all((ref - got) / (np.abs(ref) + 1) < 1e-3)
