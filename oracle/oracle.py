@@ -1,9 +1,12 @@
 """CPU oracle -- test infrastructure only.
 
-Python face of the C restatement in oracle/crown_qcqp.c (FP64 CROWN of a
+Python face of the C restatements in oracle/crown_qcqp.c (FP64 CROWN of a
 QCQP with the reference's rounding enclosures, widest branching, bisection,
-prune).  Only tests/, the smoke check and the benchmark's CPU-baseline leg
-may import this module; the product path never does.
+prune) and oracle/crown_dag.c (FP64 CROWN of expression programs: per-node
+intermediate bounds, the reference's relaxations of mul / sqr / pow / exp /
+log / reciprocal / sin, interval fallback for rows with no relaxation).
+Only tests/, the smoke check and the benchmark's CPU-baseline leg may import
+this module; the product path never does.
 
 Every function is pinned in tests/test_oracle.py against what fixes it
 independently: sampled soundness (lb <= f(x) at vertices and random points),
@@ -11,7 +14,13 @@ closed forms (a pure linear row gives the exact box minimum within the
 enclosure margin; a single bilinear term gives the exact McCormick minimum),
 brute force on tiny inputs, and the reference's own explicit expectations
 for split / branch / prune (CV/tests/test_input_split_split.py and
-branching_domains.py semantics).
+branching_domains.py semantics).  The program oracle is pinned in
+tests/test_oracle_dag.py: the reference's golden CROWN bounds of x^4 and x^6
+(solver/tests/data_64/pow_test_data, extracted to tests/golden/), closed
+forms of the exp / log / reciprocal / square planes, and sampled soundness of
+every operator and of random programs.  x^3 (exact tangent points instead of
+the reference's tables) and sin (secant / tangent on one convexity piece)
+are pinned by soundness only.
 """
 import ctypes
 import os
@@ -96,3 +105,46 @@ def prune(lb, thr_rows, xL, xU, disc_min=np.inf):
     dm = ctypes.c_double(disc_min)
     kept = lib().gho_prune(B, S, xL.shape[1], lb, thr, xL, xU, keep, ctypes.byref(dm))
     return keep.astype(bool), int(kept), dm.value
+
+
+# ------------------------------------------------------------ expression DAGs
+class _GhProg(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int32), ("V", ctypes.c_int32), ("R", ctypes.c_int32),
+                ("pad", ctypes.c_int32), ("nodes", ctypes.c_void_p), ("out", ctypes.c_void_p)]
+
+
+def _prog_struct(prog):
+    nodes, out = prog.arrays()
+    st = _GhProg(prog.n, prog.V, prog.R, 0, nodes.ctypes.data_as(ctypes.c_void_p),
+                 out.ctypes.data_as(ctypes.c_void_p))
+    return st, (nodes, out)   # keep the arrays alive with the struct
+
+
+def _dag_lib():
+    h = lib()
+    if not hasattr(h, "_dag_ready"):
+        h.gho_dag_crown.argtypes = [ctypes.POINTER(_GhProg), ctypes.c_int, _D, _D, _D, ctypes.c_int]
+        h.gho_dag_crown.restype = ctypes.c_int
+        h.gho_dag_ibp.argtypes = [ctypes.POINTER(_GhProg), ctypes.c_int, _D, _D, _D, _D]
+        h.gho_dag_ibp.restype = ctypes.c_int
+        h._dag_ready = True
+    return h
+
+
+def dag_crown(prog, xL, xU, threads=1):
+    """lb (B, R): FP64 oracle CROWN lower bound of every row of an
+    expression program (gear_amd.program.Program) over boxes xL, xU (B, n)."""
+    xL, xU = _c(np.atleast_2d(xL)), _c(np.atleast_2d(xU))
+    st, keep = _prog_struct(prog)
+    lb = np.empty((xL.shape[0], prog.R))
+    _dag_lib().gho_dag_crown(ctypes.byref(st), xL.shape[0], xL, xU, lb, threads)
+    return lb
+
+
+def dag_ibp(prog, xL, xU):
+    xL, xU = _c(np.atleast_2d(xL)), _c(np.atleast_2d(xU))
+    st, keep = _prog_struct(prog)
+    lo = np.empty((xL.shape[0], prog.R))
+    hi = np.empty((xL.shape[0], prog.R))
+    _dag_lib().gho_dag_ibp(ctypes.byref(st), xL.shape[0], xL, xU, lo, hi)
+    return lo, hi
