@@ -1,0 +1,348 @@
+// Plan builder for expression programs: compiles every row of every program
+// into the tape of dag_tape.h and uploads it to the current device.
+//
+// The backward-pass bookkeeping resolved here follows the reference's
+// backward_general queue (auto_LiRPA/backward_bound.py:989-1125): a node is
+// visited once all of its consumers have pushed their A into it, ancestors
+// in reverse topological order; an A merge into a node that already holds
+// one is charged gamma_1 <|A|, magnitude> (backward_bound.py:749-823), at
+// once for an earlier contribution and deferred to the node's own visit for
+// the last one (:801-806).  Which case applies depends on the program's
+// structure only, so it is fixed here and the kernel reads it off the tape.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "dag_tape.h"
+#include "gearhip_internal.h"
+
+namespace {
+
+using gh::DagFwd;
+using gh::DagJob;
+using gh::DagPass;
+using gh::DagStep;
+
+bool is_relaxed(int op)
+{
+    return op == GH_OP_MUL || op == GH_OP_SQR || op == GH_OP_POW || op == GH_OP_EXP ||
+           op == GH_OP_LOG || op == GH_OP_RECIP || op == GH_OP_SIN;
+}
+
+bool is_binary(int op) { return op == GH_OP_ADD || op == GH_OP_SUB || op == GH_OP_MUL; }
+
+constexpr int kMaxRowNodes = 8192;
+
+struct Builder {
+    std::vector<DagJob> jobs;
+    std::vector<DagFwd> fwd;
+    std::vector<DagPass> passes;
+    std::vector<DagStep> steps;
+    std::vector<int32_t> cin;
+    std::vector<int32_t> gin;
+
+    struct Local {
+        int op, a, b;
+        double c;
+        int gid;
+    };
+
+    // A-slot allocator of one pass (pairs: lower at s, upper at s + 1)
+    struct Slots {
+        int base, used = 0, peak = 0;
+        std::vector<int> free_list;
+        int get()
+        {
+            int s;
+            if (!free_list.empty()) {
+                s = free_list.back();
+                free_list.pop_back();
+            } else {
+                s = used++;
+            }
+            peak = std::max(peak, used);
+            return base + 2 * s;
+        }
+        void put(int slot) { free_list.push_back((slot - base) / 2); }
+    };
+
+    // one backward pass from local node s; returns the pass index
+    int add_pass(const std::vector<Local> &N, int ni, int s, int want_u, int abase, int *apeak,
+                 const std::vector<int> &rs)
+    {
+        const int V = (int)N.size();
+        std::vector<char> anc(s + 1, 0), has(s + 1, 0), df(s + 1, 0);
+        std::vector<int> rem(s + 1, 0), slot(s + 1, -1);
+        anc[s] = 1;
+        for (int v = s; v >= ni; --v) {
+            if (!anc[v] || N[v].op == GH_OP_CONST) continue;
+            anc[N[v].a] = 1;
+            rem[N[v].a]++;
+            if (is_binary(N[v].op)) {
+                anc[N[v].b] = 1;
+                rem[N[v].b]++;
+            }
+        }
+        Slots sl;
+        sl.base = abase;
+        has[s] = 1;
+        slot[s] = sl.get();
+        DagPass P{};
+        P.step_off = (int32_t)steps.size();
+        P.start = s;
+        P.want_u = want_u;
+        P.sstart = slot[s];
+        auto push = [&](int u, int later) -> uint8_t {
+            if (has[u]) {
+                bool last = !later && rem[u] == 0;
+                if (last) {
+                    df[u] = 1;
+                    return gh::PUSH_DEFER;
+                }
+                return gh::PUSH_CHARGE;
+            }
+            has[u] = 1;
+            slot[u] = sl.get();
+            return gh::PUSH_FIRST;
+        };
+        bool first_input = true;
+        std::vector<std::pair<int, int>> ins;
+        for (int v = s; v >= 0; --v) {
+            if (!anc[v]) continue;
+            const Local &nd = N[v];
+            DagStep st{};
+            st.op = (uint8_t)nd.op;
+            st.v = (int16_t)v;
+            st.sv = (int16_t)slot[v];
+            st.c = nd.c;
+            st.rs = (int16_t)(rs[v] >= 0 ? rs[v] : 0);
+            if (nd.op == GH_OP_INPUT) {
+                if (first_input) st.flags |= gh::STEP_FIRST_INPUT;
+                first_input = false;
+                ins.emplace_back(v, slot[v]);
+                steps.push_back(st);
+                continue;  // its A stays live until concretization
+            }
+            if (nd.op == GH_OP_CONST) {
+                steps.push_back(st);
+                sl.put(slot[v]);
+                continue;
+            }
+            if (df[v]) st.flags |= gh::STEP_DEFERRED;
+            int a = nd.a, b = nd.b;
+            rem[a]--;
+            if (is_binary(nd.op)) rem[b]--;
+            st.a = (int16_t)a;
+            st.b = (int16_t)b;
+            switch (nd.op) {
+            case GH_OP_ADD:
+            case GH_OP_SUB:
+            case GH_OP_MUL:
+                st.ka = push(a, b == a);
+                st.kb = push(b, 0);
+                st.sb = (int16_t)slot[b];
+                break;
+            default:
+                st.ka = push(a, 0);
+                break;
+            }
+            st.sa = (int16_t)slot[a];
+            steps.push_back(st);
+            sl.put(slot[v]);
+        }
+        std::sort(ins.begin(), ins.end());
+        P.cin_off = (int32_t)cin.size();
+        P.ncin = (int32_t)ins.size();
+        for (auto &p : ins) cin.push_back((p.first << 16) | p.second);
+        P.nsteps = (int32_t)steps.size() - P.step_off;
+        *apeak = std::max(*apeak, sl.peak);
+        passes.push_back(P);
+        (void)V;
+        return (int)passes.size() - 1;
+    }
+
+    int add_program(int p, const gh_prog &G)
+    {
+        const int n = G.n, Vg = G.V;
+        if (n < 0 || Vg < n || G.R < 0 || (Vg > 0 && !G.nodes) || (G.R > 0 && !G.out))
+            return GH_EINVAL;
+        // inputs of relaxed nodes get CROWN intermediate bounds
+        std::vector<char> needs(Vg, 0);
+        for (int v = n; v < Vg; ++v) {
+            const gh_node &nd = G.nodes[v];
+            if (nd.op < GH_OP_CONST || nd.op > GH_OP_SIN) return GH_EINVAL;
+            if (nd.op != GH_OP_CONST && (nd.a < 0 || nd.a >= v)) return GH_EINVAL;
+            if (is_binary(nd.op) && (nd.b < 0 || nd.b >= v)) return GH_EINVAL;
+            if (nd.op == GH_OP_POW) {
+                int e = (int)nd.c;
+                if ((double)e != nd.c || (e != 3 && e != 4 && e != 6)) return GH_EUNSUPPORTED;
+            }
+            if (!is_relaxed(nd.op)) continue;
+            if (nd.a >= n) needs[nd.a] = 1;
+            if (nd.op == GH_OP_MUL && nd.b >= n) needs[nd.b] = 1;
+        }
+        for (int v = 0; v < n; ++v)
+            if (G.nodes[v].op != GH_OP_INPUT || G.nodes[v].a != v) return GH_EINVAL;
+        for (int r = 0; r < G.R; ++r) {
+            int o = G.out[r];
+            if (o < 0 || o >= Vg) return GH_EINVAL;
+            // the row's ancestors
+            std::vector<char> in_row(o + 1, 0);
+            std::vector<int> stack{o};
+            while (!stack.empty()) {
+                int v = stack.back();
+                stack.pop_back();
+                if (in_row[v]) continue;
+                in_row[v] = 1;
+                const gh_node &nd = G.nodes[v];
+                if (nd.op == GH_OP_INPUT || nd.op == GH_OP_CONST) continue;
+                stack.push_back(nd.a);
+                if (is_binary(nd.op)) stack.push_back(nd.b);
+            }
+            std::vector<int> order, local(o + 1, -1);
+            for (int v = 0; v < std::min(n, o + 1); ++v)
+                if (in_row[v]) order.push_back(v);
+            int ni = (int)order.size();
+            for (int v = n; v <= o; ++v)
+                if (in_row[v]) order.push_back(v);
+            const int V = (int)order.size();
+            if (V > kMaxRowNodes) return GH_EUNSUPPORTED;
+            for (int k = 0; k < V; ++k) local[order[k]] = k;
+            std::vector<Local> N(V);
+            for (int k = 0; k < V; ++k) {
+                const gh_node &nd = G.nodes[order[k]];
+                N[k].op = nd.op;
+                N[k].c = nd.c;
+                N[k].gid = order[k];
+                N[k].a = (nd.op == GH_OP_INPUT || nd.op == GH_OP_CONST) ? 0 : local[nd.a];
+                N[k].b = is_binary(nd.op) ? local[nd.b] : 0;
+            }
+            DagJob J{};
+            J.prog = p;
+            J.row = r;
+            J.V = V;
+            J.ni = ni;
+            J.n = n;
+            J.R = G.R;
+            J.out = local[o];
+            J.gin_off = (int32_t)gin.size();
+            for (int k = 0; k < ni; ++k) gin.push_back(order[k]);
+            // relaxation slots after L (0..V-1) and U (V..2V-1)
+            std::vector<int> rs(V, -1);
+            int next = 2 * V;
+            for (int k = ni; k < V; ++k) {
+                if (!is_relaxed(N[k].op)) continue;
+                rs[k] = next;
+                next += (N[k].op == GH_OP_MUL) ? 7 : 5;
+            }
+            const int abase = next;
+            int apeak = 0;
+            J.fwd_off = (int32_t)fwd.size();
+            J.pass_off = (int32_t)passes.size();
+            int64_t cost = V;
+            for (int k = ni; k < V; ++k) {
+                DagFwd f{};
+                f.op = N[k].op;
+                f.a = N[k].a;
+                f.b = N[k].b;
+                f.rs = rs[k];
+                f.c = N[k].c;
+                f.pass = -1;
+                if (needs[N[k].gid] && N[k].op != GH_OP_CONST) {
+                    f.pass = add_pass(N, ni, k, 1, abase, &apeak, rs);
+                    cost += passes.back().nsteps;
+                }
+                fwd.push_back(f);
+            }
+            if (o < n) {
+                J.final_pass = -1;
+            } else {
+                J.final_pass = add_pass(N, ni, J.out, 0, abase, &apeak, rs);
+                cost += passes.back().nsteps;
+            }
+            J.slots = abase + 2 * apeak;
+            if (J.slots > 32767) return GH_EUNSUPPORTED;
+            J.cost = cost;
+            jobs.push_back(J);
+        }
+        return GH_OK;
+    }
+};
+
+template <class T>
+size_t append(std::vector<char> &buf, const std::vector<T> &v)
+{
+    size_t off = (buf.size() + 255) & ~size_t(255);
+    buf.resize(off + v.size() * sizeof(T));
+    if (!v.empty()) std::memcpy(buf.data() + off, v.data(), v.size() * sizeof(T));
+    return off;
+}
+
+}  // namespace
+
+extern "C" int gh_dag_plan_create(int nprog, const gh_prog *progs, gh_dag_plan **plan)
+{
+    if (!plan || nprog < 0 || (nprog > 0 && !progs)) return GH_EINVAL;
+    *plan = nullptr;
+    Builder B;
+    for (int p = 0; p < nprog; ++p) {
+        int rc = B.add_program(p, progs[p]);
+        if (rc != GH_OK) return rc;
+    }
+    // largest rows first: the queue hands them out before the short ones
+    std::stable_sort(B.jobs.begin(), B.jobs.end(),
+                     [](const DagJob &x, const DagJob &y) { return x.cost > y.cost; });
+    std::vector<char> buf;
+    size_t o_jobs = append(buf, B.jobs), o_fwd = append(buf, B.fwd), o_pass = append(buf, B.passes);
+    size_t o_steps = append(buf, B.steps), o_cin = append(buf, B.cin), o_gin = append(buf, B.gin);
+    buf.resize(std::max<size_t>(buf.size(), 256));
+    auto *P = new gh_dag_plan{};
+    if (hipGetDevice(&P->device) != hipSuccess || hipMalloc(&P->dbuf, buf.size()) != hipSuccess) {
+        delete P;
+        return GH_ELAUNCH;
+    }
+    if (hipMemcpy(P->dbuf, buf.data(), buf.size(), hipMemcpyHostToDevice) != hipSuccess) {
+        (void)hipFree(P->dbuf);
+        delete P;
+        return GH_ELAUNCH;
+    }
+    char *d = static_cast<char *>(P->dbuf);
+    P->jobs = reinterpret_cast<const DagJob *>(d + o_jobs);
+    P->fwd = reinterpret_cast<const DagFwd *>(d + o_fwd);
+    P->passes = reinterpret_cast<const DagPass *>(d + o_pass);
+    P->steps = reinterpret_cast<const DagStep *>(d + o_steps);
+    P->cin = reinterpret_cast<const int32_t *>(d + o_cin);
+    P->gin = reinterpret_cast<const int32_t *>(d + o_gin);
+    P->njobs = (int64_t)B.jobs.size();
+    P->nfwd = (int64_t)B.fwd.size();
+    P->npass = (int64_t)B.passes.size();
+    P->nsteps = (int64_t)B.steps.size();
+    P->ncin = (int64_t)B.cin.size();
+    P->ngin = (int64_t)B.gin.size();
+    P->nprog = nprog;
+    P->max_slots = 1;
+    for (auto &J : B.jobs) P->max_slots = std::max<int64_t>(P->max_slots, J.slots);
+    P->tape_bytes = (int64_t)buf.size();
+    *plan = P;
+    return GH_OK;
+}
+
+extern "C" int gh_dag_plan_destroy(gh_dag_plan *plan)
+{
+    if (!plan) return GH_OK;
+    hipError_t e = hipFree(plan->dbuf);
+    delete plan;
+    return e == hipSuccess ? GH_OK : GH_ELAUNCH;
+}
+
+extern "C" int gh_dag_plan_info(const gh_dag_plan *plan, int64_t *jobs, int64_t *max_slots,
+                                int64_t *tape_bytes)
+{
+    if (!plan) return GH_EINVAL;
+    if (jobs) *jobs = plan->njobs;
+    if (max_slots) *max_slots = plan->max_slots;
+    if (tape_bytes) *tape_bytes = plan->tape_bytes;
+    return GH_OK;
+}

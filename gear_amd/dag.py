@@ -1,0 +1,105 @@
+"""Expression programs on the device: plan, batches and the persistent
+CROWN bound (include/gearhip.h, gh_dag_*).
+
+`DagPlan(programs)` compiles a list of `gear_amd.program.Program` into one
+device-resident plan; `bound(xLs, xUs)` bounds every row of every program
+over its own batch of boxes in one persistent launch.  It replaces, for
+programs in this node vocabulary, BoundedModule.compute_bounds(...,
+method='CROWN', bound_upper=False) (auto_LiRPA/bound_general.py:1275-1593)
+as called per instance by input_split/bounding.py:316-326.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+
+
+class GhProg(ctypes.Structure):
+    """gh_prog of include/gearhip_program.h (host pointers)."""
+    _fields_ = [("n", ctypes.c_int32), ("V", ctypes.c_int32), ("R", ctypes.c_int32),
+                ("pad", ctypes.c_int32), ("nodes", ctypes.c_void_p), ("out", ctypes.c_void_p)]
+
+
+def default_waves(device=None):
+    """Resident wavefronts of a full persistent launch: 12 per CU (3 per
+    SIMD, the kernel's register-limited occupancy)."""
+    props = torch.cuda.get_device_properties(device or torch.cuda.current_device())
+    return 12 * props.multi_processor_count
+
+
+class DagPlan:
+    def __init__(self, programs, device="cuda"):
+        self.programs = list(programs)
+        self.device = torch.device(device)
+        self.n = [P.n for P in self.programs]
+        self.R = [P.R for P in self.programs]
+        arrs = [P.arrays() for P in self.programs]
+        progs = (GhProg * max(1, len(arrs)))()
+        for k, (P, (nodes, out)) in enumerate(zip(self.programs, arrs)):
+            progs[k] = GhProg(P.n, P.V, P.R, 0, nodes.ctypes.data, out.ctypes.data)
+        handle = ctypes.c_void_p()
+        with torch.cuda.device(self.device):
+            _lib.check(_lib.lib().gh_dag_plan_create(len(arrs), ctypes.addressof(progs),
+                                                     ctypes.byref(handle)), "dag plan")
+        self._h = handle
+        jobs, slots, tape = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        _lib.check(_lib.lib().gh_dag_plan_info(handle, ctypes.byref(jobs), ctypes.byref(slots),
+                                               ctypes.byref(tape)), "dag plan info")
+        self.jobs, self.max_slots, self.tape_bytes = jobs.value, slots.value, tape.value
+        self._ws = None
+        self._ws_waves = None
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.lib().gh_dag_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- device buffers
+    def workspace(self, waves):
+        if self._ws is None or self._ws_waves != waves:
+            nbytes = _lib.lib().gh_dag_workspace_bytes(self._h, waves)
+            self._ws = torch.empty(max(nbytes, 256), dtype=torch.uint8, device=self.device)
+            self._ws_waves = waves
+        return self._ws
+
+    def make_batches(self, xLs, xUs, lbs, counts):
+        """Device array of gh_dag_batch: per program (xL, xU, lb, count*)."""
+        rows = []
+        for k, (xL, xU, lb, cnt) in enumerate(zip(xLs, xUs, lbs, counts)):
+            for t in (xL, xU, lb):
+                if t.dtype != torch.float64 or not t.is_contiguous() or t.device != self.device:
+                    raise _lib.GearHipError("batch tensors must be contiguous float64 on the plan's device")
+            if xL.shape[-1] != self.n[k] or lb.shape[-1] != self.R[k]:
+                raise _lib.GearHipError(f"program {k}: box width {xL.shape[-1]} / rows {lb.shape[-1]} "
+                                        f"do not match n={self.n[k]}, R={self.R[k]}")
+            rows.append([xL.data_ptr(), xU.data_ptr(), lb.data_ptr(), cnt.data_ptr()])
+        return torch.tensor(np.asarray(rows, dtype=np.uint64).view(np.int64).reshape(-1, 4),
+                            dtype=torch.int64).to(self.device)
+
+    def launch(self, batches, waves, stream=None):
+        ws = self.workspace(waves)
+        _lib.check(_lib.lib().gh_dag_crown_f64(self._h, batches.data_ptr(), ws.data_ptr(),
+                                               ws.numel(), waves, _lib.stream_ptr(stream)),
+                   "dag crown")
+
+    def bound(self, xLs, xUs, waves=None):
+        """lb per program (B_k, R_k), FP64."""
+        waves = waves or default_waves(self.device)
+        xLs = [x.contiguous() for x in xLs]
+        xUs = [x.contiguous() for x in xUs]
+        lbs = [torch.empty(x.shape[0], r, dtype=torch.float64, device=self.device)
+               for x, r in zip(xLs, self.R)]
+        counts = torch.tensor([x.shape[0] for x in xLs], dtype=torch.int64, device=self.device)
+        cnt_views = [counts[k:k + 1] for k in range(len(xLs))]
+        batches = self.make_batches(xLs, xUs, lbs, cnt_views)
+        self.launch(batches, waves)
+        self._keep = (batches, counts)
+        return lbs

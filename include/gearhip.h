@@ -4,7 +4,9 @@
  *
  * Conventions (all entry points):
  *   - every pointer argument is a device pointer unless stated otherwise;
- *     buffers are caller-owned (the library allocates nothing);
+ *     buffers are caller-owned (the library allocates device memory only
+ *     for program plans, in gh_dag_plan_create, and frees it in
+ *     gh_dag_plan_destroy);
  *   - arrays are dense row-major, FP64 unless the name says f32;
  *   - work is enqueued on `stream` (0 = the null stream) and is
  *     asynchronous; the return value reports argument and launch errors
@@ -19,6 +21,8 @@
 
 #include <stdint.h>
 #include <hip/hip_runtime_api.h>
+
+#include "gearhip_program.h"
 
 #ifdef __cplusplus
 extern "C" {
@@ -146,6 +150,56 @@ int gh_filter_append_f64(int B, int S, int n, const double *lb,
  */
 int gh_min_strided_f64(int64_t N, int64_t stride, const double *x,
                        double *out, hipStream_t stream);
+
+/* ------------------------------------------------ expression programs */
+
+/*
+ * Program plan: the immutable, device-resident form of one or more
+ * expression programs (gearhip_program.h), built once per program set.
+ * Replaces the per-model graph that BoundedModule.__init__ builds
+ * (auto_LiRPA/bound_general.py:50-160) for programs in the node vocabulary
+ * of gearhip_program.h.  `progs` and the arrays they point to are HOST
+ * memory; the plan uploads what the kernels read to the current device.
+ * Returns GH_EUNSUPPORTED for programs past the plan limits (a row with
+ * more than 8192 nodes).
+ */
+typedef struct gh_dag_plan gh_dag_plan;
+int gh_dag_plan_create(int nprog, const gh_prog *progs, gh_dag_plan **plan);
+int gh_dag_plan_destroy(gh_dag_plan *plan);
+/* Rows (jobs) in the plan and per-lane state slots of the largest row. */
+int gh_dag_plan_info(const gh_dag_plan *plan, int64_t *jobs, int64_t *max_slots,
+                     int64_t *tape_bytes);
+
+/* One program's batch of boxes (device pointers). */
+typedef struct {
+    const double *xL, *xU;   /* (cap, n) */
+    double *lb;              /* (cap, R) out */
+    const int64_t *count;    /* device scalar: boxes to bound, <= cap */
+} gh_dag_batch;
+
+/* Workspace bytes for gh_dag_crown_f64 with `waves` resident wavefronts. */
+size_t gh_dag_workspace_bytes(const gh_dag_plan *plan, int waves);
+
+/*
+ * CROWN lower bounds of every row of every program over its batch, FP64,
+ * in one persistent launch: `waves` wavefronts (64 boxes each) pull
+ * (row, 64-box chunk) items from a device queue, largest rows first, until
+ * the queue drains.  Same contract as gh_qcqp_crown_f64 without A:
+ *   compute_bounds(..., method='CROWN', bound_upper=False)
+ * (auto_LiRPA/bound_general.py:1275-1593): intermediate bounds of every
+ * input of a relaxed node by CROWN intersected with intervals
+ * (bound_general.py:1032-1222), the operators' relaxations
+ * (operators/bivariate.py, activations.py, s_shaped.py, convex_concave.py,
+ * trigonometric.py), A67 rounding enclosure (backward_bound.py:550-823),
+ * box concretization (concretize_bounds.py:267-321) and the interval
+ * fallback of rows with no relaxation (backward_bound.py:826-877).
+ *   batches: device array of nprog gh_dag_batch, in plan order;
+ *   ws: gh_dag_workspace_bytes(plan, waves) bytes.
+ * Counts are read on the device, so a captured launch follows the batch
+ * sizes at replay.  Calls sharing `ws` must be ordered on one stream.
+ */
+int gh_dag_crown_f64(const gh_dag_plan *plan, const gh_dag_batch *batches, void *ws,
+                     size_t ws_bytes, int waves, hipStream_t stream);
 
 #ifdef __cplusplus
 }

@@ -54,6 +54,7 @@
 #endif
 
 #include "crown_dag.h"
+#include "gearhip_math.h"
 
 #define EPS64 2.220446049250313080847e-16
 #define EPSILON 1e-12  /* auto_LiRPA operators/base.py:32 */
@@ -135,23 +136,23 @@ static void ibp_node(const gh_node *nd, const double *L, const double *U, double
         u = fmax(la * la, ua * ua);
         break;
     case GH_OP_POW: {
-        double pl = pow(la, c), pu = pow(ua, c);
+        double pl = gh_powi(la, (int)c), pu = gh_powi(ua, (int)c);
         if (((int)c) % 2 == 0) {
             l = (la <= 0 && ua >= 0) ? 0.0 : fmin(pl, pu);
             u = fmax(pl, pu);
         } else { l = pl; u = pu; }
         break;
     }
-    case GH_OP_EXP: l = exp(la); u = exp(ua); break;  /* convex_concave.py:418-427 */
+    case GH_OP_EXP: l = gh_exp(la); u = gh_exp(ua); break;  /* convex_concave.py:418-427 */
     case GH_OP_LOG: /* convex_concave.py:92-103: restrict to x >= 0 */
-        l = log(la > 0 ? la : 0.0); u = log(ua); break;
+        l = gh_log(la > 0 ? la : 0.0); u = gh_log(ua); break;
     case GH_OP_RECIP: /* convex_concave.py:286-313 */
         l = 1.0 / ua; u = 1.0 / la;
         if ((la < 0 && ua > 0) || (la < 0 && ua == 0) || (la == 0 && ua == 0)) l = -INFINITY;
         if ((la < 0 && ua > 0) || (la == 0 && ua == 0) || (la == 0 && ua > 0)) u = INFINITY;
         break;
     case GH_OP_SIN: { /* trigonometric.py:445-467 */
-        double sl = sin(la), su = sin(ua);
+        double sl = gh_sin(la), su = gh_sin(ua);
         u = crosses(la, ua, M_PI / 2) ? 1.0 : fmax(sl, su);
         l = crosses(la, ua, 3 * M_PI / 2) ? -1.0 : fmin(sl, su);
         break;
@@ -202,13 +203,13 @@ static int relax_node(const gh_node *nd, const double *L, const double *U, doubl
         if (!isfinite(l) || !isfinite(u)) return (p % 2 == 0) ? 2 : 3;
         u = fmax(u, l + 1e-8);  /* s_shaped.py:795 */
         if (p % 2 == 0) {  /* bound_relax_even_power, s_shaped.py:940-972 */
-            double yl_ = pow(l, p), yu_ = pow(u, p);
+            double yl_ = gh_powi(l, p), yu_ = gh_powi(u, p);
             double eps = fmax(fabs(yl_), fabs(yu_)) * 1e-6 + 1e-12;
             double k = (yu_ - yl_) / fmax(u - l, 1e-8);
             R[2] = k; R[3] = -l * k + (yl_ + eps);
             double mid = l >= 0 ? l : (u <= 0 ? u : 0.0);
-            double dk = p * pow(mid, p - 1);
-            R[0] = dk; R[1] = -mid * dk + (pow(mid, p) - eps);
+            double dk = p * gh_powi(mid, p - 1);
+            R[0] = dk; R[1] = -mid * dk + (gh_powi(mid, p) - eps);
         } else {  /* x^3: inverse S (s_shaped.py:325-549 with inflection 0) */
             double fl = l * l * l, fu = u * u * u;
             double ks = (fu - fl) / (u - l);
@@ -234,18 +235,18 @@ static int relax_node(const gh_node *nd, const double *L, const double *U, doubl
     case GH_OP_EXP: { /* convex_concave.py:569-625 */
         if (!isfinite(l) || !isfinite(u)) return 3;
         double du = u - l;
-        double ne = -expm1(l - u);
-        double mp = u + log(fmax(ne, EPSILON)) - log(fmax(du, EPSILON));
+        double ne = -gh_expm1(l - u);
+        double mp = u + gh_log(fmax(ne, EPSILON)) - gh_log(fmax(du, EPSILON));
         double m = du > 1e-6 ? mp : (l + u) / 2;
         m = fmin(fmax(m, l), u);
-        double el = exp(l), em = exp(m), eu = exp(u);
+        double el = gh_exp(l), em = gh_exp(m), eu = gh_exp(u);
         double lk = isfinite(em) ? em : 0.0;
         double lx = isfinite(m) ? m : 0.0;
         double lbb = -lx * lk + lk;
         int ls = isfinite(lk) && isfinite(lbb);
         R[0] = ls ? lk : 0.0; R[1] = ls ? lbb : 0.0;
         double d = u - l, k;
-        if (d < 1) k = el * (d > 0 ? expm1(fmin(d, 1.0)) / fmax(d, 1e-300) : 1.0);
+        if (d < 1) k = el * (d > 0 ? gh_expm1(fmin(d, 1.0)) / fmax(d, 1e-300) : 1.0);
         else k = (eu - el) / fmax(d, 1.0);
         double ubb = -l * k + el;
         int us = isfinite(k) && isfinite(ubb);
@@ -255,14 +256,14 @@ static int relax_node(const gh_node *nd, const double *L, const double *U, doubl
     case GH_OP_LOG: { /* convex_concave.py:64-92 */
         double lr = l > 0 ? l : 0.0;
         if (!(lr > 0) || !isfinite(u)) return 3;
-        double rl = log(lr), ru = log(u), du = u - lr;
+        double rl = gh_log(lr), ru = gh_log(u), du = u - lr;
         double ku = (ru - rl) / fmax(du, EPSILON);
         R[0] = ku; R[1] = -lr * ku + rl;
         double dl = fmax(ru - rl, EPSILON);
         double m = du > 1e-6 ? du / dl : (lr + u) / 2;
         m = fmin(fmax(m, lr), u);
         double k = 1.0 / m;
-        R[2] = k; R[3] = -m * k + log(m);
+        R[2] = k; R[3] = -m * k + gh_log(m);
         return 0;
     }
     case GH_OP_RECIP: { /* convex_concave.py:315-375 */
@@ -285,9 +286,9 @@ static int relax_node(const gh_node *nd, const double *L, const double *U, doubl
         double kpi_l = floor(l / M_PI), kpi_u = floor(u / M_PI);
         if (kpi_l == kpi_u && u - l < M_PI) {
             /* one piece: sin is concave where floor(x/pi) is even */
-            double sl = sin(l), su = sin(u);
-            double ks = (u > l) ? (su - sl) / (u - l) : cos(l);
-            double m = (l + u) / 2, km = cos(m), bm = -m * km + sin(m);
+            double sl = gh_sin(l), su = gh_sin(u);
+            double ks = (u > l) ? (su - sl) / (u - l) : gh_cos(l);
+            double m = (l + u) / 2, km = gh_cos(m), bm = -m * km + gh_sin(m);
             double bs = -l * ks + sl;
             if (((long)kpi_l) % 2 == 0) { R[0] = ks; R[1] = bs; R[2] = km; R[3] = bm; }
             else { R[0] = km; R[1] = bm; R[2] = ks; R[3] = bs; }
@@ -641,5 +642,26 @@ int gho_dag_eval(const gh_prog *P, int N_, const double *X, double *F)
         for (int r = 0; r < P->R; ++r) F[(size_t)k * P->R + r] = Y[P->out[r]];
     }
     free(Y);
+    return 0;
+}
+
+/* The shared elementary functions (gearhip_math.h) over an array, for their
+ * accuracy tests: fn 0 exp, 1 expm1, 2 log, 3 sin, 4 cos, 10 + p x^p. */
+int gho_math(int fn, int N, const double *x, double *y)
+{
+    for (int k = 0; k < N; ++k) {
+        double v = x[k], r;
+        switch (fn) {
+        case 0: r = gh_exp(v); break;
+        case 1: r = gh_expm1(v); break;
+        case 2: r = gh_log(v); break;
+        case 3: r = gh_sin(v); break;
+        case 4: r = gh_cos(v); break;
+        default:
+            if (fn < 10) return 1;
+            r = gh_powi(v, fn - 10);
+        }
+        y[k] = r;
+    }
     return 0;
 }

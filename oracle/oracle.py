@@ -148,3 +148,17 @@ def dag_ibp(prog, xL, xU):
     hi = np.empty((xL.shape[0], prog.R))
     _dag_lib().gho_dag_ibp(ctypes.byref(st), xL.shape[0], xL, xU, lo, hi)
     return lo, hi
+
+
+def math_fn(name, x):
+    """The shared elementary functions of include/gearhip_math.h on an
+    array (name: exp, expm1, log, sin, cos, pow2 .. pow6)."""
+    codes = {"exp": 0, "expm1": 1, "log": 2, "sin": 3, "cos": 4}
+    code = codes[name] if name in codes else 10 + int(name[3:])
+    x = _c(x)
+    y = np.empty_like(x)
+    h = lib()
+    h.gho_math.argtypes = [ctypes.c_int, ctypes.c_int, _D, _D]
+    h.gho_math.restype = ctypes.c_int
+    h.gho_math(code, x.size, x, y)
+    return y

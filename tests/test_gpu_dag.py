@@ -1,0 +1,140 @@
+"""Expression-program HIP path vs the FP64 oracle, through the C ABI.
+
+Bounds are identical: the kernel performs the oracle's operations in the
+oracle's order, and both use the elementary functions of
+include/gearhip_math.h.  Rows with no relaxation fall back to the same
+interval, and -inf rows match.  (The bar of the bound is 1e-9 relative; the
+test asks for equality and reports the relative error it would have had.)
+"""
+import numpy as np
+import pytest
+import torch
+
+from gear_amd.problems import random_bisection_boxes
+from gear_amd.program import Program, config1_problem, config3_instances, random_nlp
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+RTOL, ATOL = 1e-9, 1e-12
+
+
+def _dev(a, cuda):
+    return torch.as_tensor(np.ascontiguousarray(a), dtype=torch.float64, device=cuda)
+
+
+def _check(g, o, what):
+    g = np.asarray(g)
+    assert g.shape == o.shape, what
+    ninf_g, ninf_o = np.isneginf(g), np.isneginf(o)
+    assert np.array_equal(ninf_g, ninf_o), f"{what}: -inf rows differ"
+    f = ~ninf_o
+    assert np.all(np.isfinite(g[f])), f"{what}: non-finite bound"
+    err = np.abs(g[f] - o[f])
+    tol = RTOL * np.abs(o[f]) + ATOL
+    worst = np.max(err / tol) if err.size else 0.0
+    exact = np.mean(g[f] == o[f]) if err.size else 1.0
+    print(f"{what}: {err.size} bounds, max err/tol {worst:.3g}, equal {exact:.4f}", flush=True)
+    assert np.all(err <= tol), what
+    assert np.array_equal(g, o), f"{what}: not identical"
+
+
+def _bound(plans_progs, boxes, cuda, waves=None):
+    from gear_amd.dag import DagPlan
+    plan = DagPlan(plans_progs, device=cuda)
+    lbs = plan.bound([_dev(L, cuda) for L, _ in boxes], [_dev(U, cuda) for _, U in boxes],
+                     waves=waves)
+    torch.cuda.synchronize()
+    return [lb.cpu().numpy() for lb in lbs]
+
+
+def test_config1_vs_oracle(cuda):
+    print("case config1 n=8 m=3 B=1024", flush=True)
+    P = config1_problem()
+    L, U = random_bisection_boxes(P.xl, P.xu, 1024, seed=11)
+    lb = _bound([P], [(L, U)], cuda)[0]
+    _check(lb, O.dag_crown(P, L, U, threads=8), "config1")
+
+
+def test_random_programs_uneven_batches(cuda):
+    progs, boxes = [], []
+    counts = [0, 1, 63, 64, 65, 200, 7, 130]
+    for s, B in enumerate(counts):
+        P = random_nlp(10 + s, 3, seed=s)
+        progs.append(P)
+        boxes.append(random_bisection_boxes(P.xl, P.xu, max(B, 1), seed=s) if B else
+                     (np.zeros((0, P.n)), np.zeros((0, P.n))))
+    print("case random programs counts", counts, flush=True)
+    lbs = _bound(progs, boxes, cuda)
+    for s, (P, (L, U)) in enumerate(zip(progs, boxes)):
+        assert lbs[s].shape == (len(L), P.R)
+        if len(L):
+            _check(lbs[s], O.dag_crown(P, L, U, threads=8), f"random program {s}")
+
+
+def _edge_program():
+    P = Program(3)
+    P.output(P.mul(P.x(0), P.x(1)))                      # plain bilinear
+    P.output(P.x(2))                                     # a bare input row
+    P.output(P.const(2.5))                               # a constant row
+    P.output(P.recip(P.x(0)))                            # pole inside some boxes
+    P.output(P.add(P.mul(0, 0), P.sub(1, 1)))            # same operand twice
+    P.output(P.neg(P.pow(P.addc(0, 0.3), 3)))            # odd power
+    P.output(P.pow(P.mulc(1, 0.7), 4))                   # even powers
+    P.output(P.pow(2, 6))
+    P.output(P.sin(P.mulc(2, 5.0)))                      # sin across periods
+    P.output(P.log(P.addc(P.sqr(0), 1e-3)))              # log near 0
+    P.output(P.mul(P.exp(0), P.sin(P.mul(1, 2))))        # nested relaxed inputs
+    P.output(P.div(P.x(1), P.addc(P.sqr(2), 0.5)))       # division
+    return P
+
+
+def test_edge_program(cuda):
+    print("case edge program", flush=True)
+    P = _edge_program()
+    xl, xu = np.array([-2.0, -1.5, -3.0]), np.array([2.0, 1.0, 3.0])
+    L, U = random_bisection_boxes(xl, xu, 500, seed=3)
+    # a few degenerate boxes
+    L[:4] = U[:4]
+    lb = _bound([P], [(L, U)], cuda)[0]
+    _check(lb, O.dag_crown(P, L, U, threads=8), "edge program")
+
+
+def test_config3_subset(cuda):
+    insts = config3_instances()[:24]
+    rng = np.random.default_rng(0)
+    boxes = []
+    for k, P in enumerate(insts):
+        B = int(rng.integers(1, 150))
+        boxes.append(random_bisection_boxes(P.xl, P.xu, B, seed=k))
+    print("case config3 subset", [P.n for P in insts], flush=True)
+    lbs = _bound(insts, boxes, cuda)
+    for k, (P, (L, U)) in enumerate(zip(insts, boxes)):
+        _check(lbs[k], O.dag_crown(P, L, U, threads=8), f"instance {k} n={P.n}")
+
+
+def test_wave_count_does_not_change_results(cuda):
+    P = random_nlp(14, 4, seed=21)
+    L, U = random_bisection_boxes(P.xl, P.xu, 300, seed=2)
+    a = _bound([P], [(L, U)], cuda, waves=1)[0]
+    b = _bound([P], [(L, U)], cuda, waves=37)[0]
+    c = _bound([P], [(L, U)], cuda)[0]
+    assert np.array_equal(a.view(np.int64), b.view(np.int64))
+    assert np.array_equal(a.view(np.int64), c.view(np.int64))
+
+
+def test_device_counts_follow_batch_size(cuda):
+    """Counts are read on the device: shrinking the count leaves the rows
+    past it untouched."""
+    from gear_amd.dag import DagPlan
+    P = random_nlp(9, 2, seed=5)
+    L, U = random_bisection_boxes(P.xl, P.xu, 130, seed=1)
+    plan = DagPlan([P], device=cuda)
+    xL, xU = _dev(L, cuda), _dev(U, cuda)
+    lb = torch.full((130, P.R), 7.0, dtype=torch.float64, device=cuda)
+    cnt = torch.tensor([70], dtype=torch.int64, device=cuda)
+    batches = plan.make_batches([xL], [xU], [lb], [cnt])
+    plan.launch(batches, 16)
+    torch.cuda.synchronize()
+    got = lb.cpu().numpy()
+    _check(got[:70], O.dag_crown(P, L[:70], U[:70]), "first 70")
+    assert np.all(got[70:] == 7.0)
