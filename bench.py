@@ -26,6 +26,11 @@ FP64_PEAK_TFLOPS = 78.6
 # FETCH_SIZE (x2.00) + WRITE_SIZE (x1.00), scale factors from a 256 MiB copy
 # (profiles/qcqp_crown_f64_n50_m20.md); scaled to the launch size below.
 MEASURED_TRAFFIC_BYTES_32K = 74.1e6
+HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of the expression-program bound on the 256-instance
+# workload (rocprofv3 FETCH_SIZE x2.00 + WRITE_SIZE x1.00,
+# profiles/dag_crown_config3.md).
+MEASURED_DAG_TRAFFIC_BYTES = 32.58e9
 
 
 def parse():
@@ -37,6 +42,9 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-alm", action="store_true")
+    ap.add_argument("--no-dag", action="store_true", help="skip the expression-program section")
+    ap.add_argument("--dag-only", action="store_true",
+                    help="run only the expression-program section (256 uneven instances)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device 5 times (a known-size "
                          "reference for the HBM byte counters) and exit")
@@ -54,6 +62,93 @@ def copy_probe(mib):
     print(f"copy probe: {mib} MiB read + {mib} MiB written per launch", flush=True)
 
 
+def bench_uneven_dag(reps=20, cpu=True):
+    """Expression programs: 256 seeded instances (n lognormal in [5, 200],
+    m = n // 4, random expression rows of depth 2-5), each with its own
+    batch of 64 * 2^k boxes (k uniform in 0..5), bounded in one persistent
+    launch replayed from a captured graph with the autotuned wave count."""
+    import numpy as np
+    import torch
+
+    from gear_amd.dag import DagPlan
+    from gear_amd.problems import random_bisection_boxes
+    from gear_amd.program import config3_instances
+
+    dev = torch.device("cuda:0")
+    insts = config3_instances()
+    rng = np.random.default_rng(0)
+    counts = [int(64 * 2 ** rng.integers(0, 6)) for _ in insts]
+    boxes = [random_bisection_boxes(P.xl, P.xu, B, seed=k)
+             for k, (P, B) in enumerate(zip(insts, counts))]
+    t_plan = time.perf_counter()
+    plan = DagPlan(insts, device=dev)
+    t_plan = time.perf_counter() - t_plan
+    xL = [torch.as_tensor(L, device=dev) for L, _ in boxes]
+    xU = [torch.as_tensor(U, device=dev) for _, U in boxes]
+    lbs = [torch.empty(B, P.R, dtype=torch.float64, device=dev) for B, P in zip(counts, insts)]
+    cnt = torch.tensor(counts, dtype=torch.int64, device=dev)
+    batches = plan.make_batches(xL, xU, lbs, [cnt[k:k + 1] for k in range(len(insts))])
+    total = sum(counts)
+    waves = plan.tune_waves(batches, total)
+    g = plan.graph(batches, waves)
+    stream = torch.cuda.current_stream()
+    for _ in range(3):
+        g.replay()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        g.replay()
+    e1.record(stream)
+    e1.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    state = sum(c * b for c, b in zip(counts, plan.state_bytes))
+    io = sum(c * b for c, b in zip(counts, plan.io_bytes))
+    achieved = (state + io) / (ms * 1e-3) / 1e9
+    out = {
+        "value": round(total / (ms * 1e-3), 1),
+        "unit": "subdomain bounds/s (every row of the instance)",
+        "row_bounds_per_s": round(sum(c * P.R for c, P in zip(counts, insts)) / (ms * 1e-3), 1),
+        "ms_per_launch": round(ms, 4),
+        "dtype": "f64",
+        "config": {"workload": "256 expression-program instances, n lognormal(ln 40, 0.8) "
+                               "clipped to [5, 200], m = n // 4, boxes per instance "
+                               "64 * 2^U{0..5}, one persistent launch (graph replay)",
+                   "instances": len(insts), "boxes": total, "rows": sum(P.R for P in insts),
+                   "waves": waves, "plan_s": round(t_plan, 3), "jobs": plan.jobs},
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "dag_crown_kernel",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": round(MEASURED_DAG_TRAFFIC_BYTES) if MEASURED_DAG_TRAFFIC_BYTES else None,
+            "traffic_source": "rocprofv3 FETCH_SIZE/WRITE_SIZE, profiles/dag_crown_config3.md",
+            "algorithmic_bytes": state + io,
+            "algorithmic": "per-box state the backward passes read and write "
+                           "(A, relaxation planes, node bounds) plus box / bound I/O",
+        },
+        "parity": "bounds identical to the FP64 oracle (tests/test_gpu_dag.py)",
+    }
+    if cpu:
+        from oracle import oracle as O
+        cores = min(16, os.cpu_count() or 1)
+        tc = 0.0
+        nb = 0
+        for k, P in enumerate(insts):
+            L, U = boxes[k]
+            t = time.perf_counter()
+            O.dag_crown(P, L[:32], U[:32], threads=cores)
+            tc += time.perf_counter() - t
+            nb += min(32, len(L))
+        out["cpu_baseline"] = {"value": round(nb / tc, 1), "unit": "subdomain bounds/s",
+                               "cores": cores, "kind": "port",
+                               "sample": f"first 32 boxes of each of the 256 instances "
+                                         f"({nb} boxes, {tc:.2f} s wall), FP64 oracle"}
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -61,6 +156,9 @@ def main():
         sys.exit(2)
     if args.copy_probe:
         copy_probe(args.copy_probe)
+        return
+    if args.dag_only:
+        print(json.dumps(bench_uneven_dag(cpu=not args.no_cpu)), flush=True)
         return
     import numpy as np
     import torch
@@ -150,7 +248,7 @@ def main():
                    "n": n, "m": S - 1, "parents_per_step": args.batch,
                    "children_per_step": 2 * args.batch},
         "roofline": {
-            "bound": "fp64",
+            "bound": "mfma",
             "kernel": "qcqp_crown_f64_kernel",
             "achieved": round(achieved_tf, 3),
             "peak": FP64_PEAK_TFLOPS,
@@ -229,6 +327,8 @@ def main():
                                   "unit": "subdomain bounds/s", "cores": cores, "kind": "port",
                                   "sample": f"{L.shape[0]} children of one step, FP64 oracle "
                                             f"({tc:.2f} s wall)"}
+    if not args.no_dag:
+        result["uneven_dag"] = bench_uneven_dag(cpu=not args.no_cpu)
     print(json.dumps(result), flush=True)
 
 

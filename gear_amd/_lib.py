@@ -36,6 +36,7 @@ SIGNATURES = {
     "gh_dag_plan_create": [_I, _P, ctypes.POINTER(_P)],
     "gh_dag_plan_destroy": [_P],
     "gh_dag_plan_info": [_P, _P, _P, _P],
+    "gh_dag_plan_stats": [_P, _P, _P, _P],
     "gh_dag_workspace_bytes": [_P, _I],
     "gh_dag_crown_f64": [_P, _P, _P, ctypes.c_size_t, _I, _P],
 }

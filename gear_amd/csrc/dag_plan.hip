@@ -42,6 +42,39 @@ struct Builder {
     std::vector<DagStep> steps;
     std::vector<int32_t> cin;
     std::vector<int32_t> gin;
+    // per program: state words, I/O words and backward steps per box
+    std::vector<int64_t> prog_words, prog_io, prog_steps;
+
+    // 8-byte slot accesses per lane of one pass, as the kernel performs them
+    int64_t pass_words(int pidx) const
+    {
+        const DagPass &P = passes[pidx];
+        const int sides = P.want_u ? 2 : 1;
+        auto pw = [](uint8_t kind) -> int64_t {
+            return kind == gh::PUSH_FIRST ? 1 : (kind == gh::PUSH_DEFER ? 2 : 4);
+        };
+        int64_t w = sides;  // the start node's A
+        for (int k = 0; k < P.nsteps; ++k) {
+            const DagStep &st = steps[P.step_off + k];
+            if (st.op == GH_OP_INPUT) {
+                w += 2 * sides + 2;
+                continue;
+            }
+            w += sides;
+            if (st.op == GH_OP_CONST) continue;
+            if (st.flags & gh::STEP_DEFERRED) w += 2;
+            switch (st.op) {
+            case GH_OP_ADD:
+            case GH_OP_SUB: w += 4 + sides * (pw(st.ka) + pw(st.kb)); break;
+            case GH_OP_ADDC: w += 2 + sides * pw(st.ka); break;
+            case GH_OP_NEG: w += sides * pw(st.ka); break;
+            case GH_OP_MULC: w += 2 + sides * pw(st.ka); break;
+            case GH_OP_MUL: w += 7 + sides * (pw(st.ka) + pw(st.kb)); break;
+            default: w += 5 + sides * pw(st.ka); break;
+            }
+        }
+        return w + (int64_t)P.ncin * (2 + sides);
+    }
 
     struct Local {
         int op, a, b;
@@ -165,6 +198,9 @@ struct Builder {
 
     int add_program(int p, const gh_prog &G)
     {
+        prog_words.push_back(0);
+        prog_io.push_back(0);
+        prog_steps.push_back(0);
         const int n = G.n, Vg = G.V;
         if (n < 0 || Vg < n || G.R < 0 || (Vg > 0 && !G.nodes) || (G.R > 0 && !G.out))
             return GH_EINVAL;
@@ -242,6 +278,7 @@ struct Builder {
             J.fwd_off = (int32_t)fwd.size();
             J.pass_off = (int32_t)passes.size();
             int64_t cost = V;
+            int64_t words = 2 * (int64_t)ni;  // the row's inputs into L / U
             for (int k = ni; k < V; ++k) {
                 DagFwd f{};
                 f.op = N[k].op;
@@ -250,9 +287,11 @@ struct Builder {
                 f.rs = rs[k];
                 f.c = N[k].c;
                 f.pass = -1;
+                words += 4 + (is_binary(f.op) ? 2 : 0) + (rs[k] < 0 ? 0 : (f.op == GH_OP_MUL ? 7 : 5));
                 if (needs[N[k].gid] && N[k].op != GH_OP_CONST) {
                     f.pass = add_pass(N, ni, k, 1, abase, &apeak, rs);
                     cost += passes.back().nsteps;
+                    words += pass_words(f.pass) + 2;
                 }
                 fwd.push_back(f);
             }
@@ -261,7 +300,11 @@ struct Builder {
             } else {
                 J.final_pass = add_pass(N, ni, J.out, 0, abase, &apeak, rs);
                 cost += passes.back().nsteps;
+                words += pass_words(J.final_pass);
             }
+            prog_words[p] += words;
+            prog_io[p] += 2 * (int64_t)ni + 1;  // box endpoints in, bound out
+            prog_steps[p] += cost - V;
             J.slots = abase + 2 * apeak;
             if (J.slots > 32767) return GH_EUNSUPPORTED;
             J.cost = cost;
@@ -325,6 +368,11 @@ extern "C" int gh_dag_plan_create(int nprog, const gh_prog *progs, gh_dag_plan *
     P->max_slots = 1;
     for (auto &J : B.jobs) P->max_slots = std::max<int64_t>(P->max_slots, J.slots);
     P->tape_bytes = (int64_t)buf.size();
+    for (int p = 0; p < nprog; ++p) {
+        P->state_bytes.push_back(8 * B.prog_words[p]);
+        P->io_bytes.push_back(8 * B.prog_io[p]);
+        P->visits.push_back(B.prog_steps[p]);
+    }
     *plan = P;
     return GH_OK;
 }
@@ -344,5 +392,17 @@ extern "C" int gh_dag_plan_info(const gh_dag_plan *plan, int64_t *jobs, int64_t 
     if (jobs) *jobs = plan->njobs;
     if (max_slots) *max_slots = plan->max_slots;
     if (tape_bytes) *tape_bytes = plan->tape_bytes;
+    return GH_OK;
+}
+
+extern "C" int gh_dag_plan_stats(const gh_dag_plan *plan, int64_t *state_bytes, int64_t *io_bytes,
+                                 int64_t *steps)
+{
+    if (!plan) return GH_EINVAL;
+    for (int64_t p = 0; p < plan->nprog; ++p) {
+        if (state_bytes) state_bytes[p] = plan->state_bytes[p];
+        if (io_bytes) io_bytes[p] = plan->io_bytes[p];
+        if (steps) steps[p] = plan->visits[p];
+    }
     return GH_OK;
 }

@@ -25,6 +25,8 @@
 
 #include <stdint.h>
 
+#include <vector>
+
 namespace gh {
 
 // push kinds: how an A contribution lands in its receiving node
@@ -89,6 +91,9 @@ struct gh_dag_plan {
     const gh::DagStep *steps;
     const int32_t *cin;         // (node << 16) | A slot
     const int32_t *gin;
+    // per program, per box: bytes of per-lane state the tape reads and
+    // writes (the A propagation and relaxation data), and box / bound I/O
+    std::vector<int64_t> state_bytes, io_bytes, visits;
 };
 
 #endif

@@ -33,6 +33,8 @@ class DagPlan:
     def __init__(self, programs, device="cuda"):
         self.programs = list(programs)
         self.device = torch.device(device)
+        if self.device.type == "cuda" and self.device.index is None:
+            self.device = torch.device("cuda", torch.cuda.current_device())
         self.n = [P.n for P in self.programs]
         self.R = [P.R for P in self.programs]
         arrs = [P.arrays() for P in self.programs]
@@ -48,6 +50,13 @@ class DagPlan:
         _lib.check(_lib.lib().gh_dag_plan_info(handle, ctypes.byref(jobs), ctypes.byref(slots),
                                                ctypes.byref(tape)), "dag plan info")
         self.jobs, self.max_slots, self.tape_bytes = jobs.value, slots.value, tape.value
+        k = max(1, len(arrs))
+        st, io, steps = (ctypes.c_int64 * k)(), (ctypes.c_int64 * k)(), (ctypes.c_int64 * k)()
+        _lib.check(_lib.lib().gh_dag_plan_stats(handle, st, io, steps), "dag plan stats")
+        # per program, per box: state bytes, I/O bytes, backward node visits
+        self.state_bytes = list(st)[:len(arrs)]
+        self.io_bytes = list(io)[:len(arrs)]
+        self.steps = list(steps)[:len(arrs)]
         self._ws = None
         self._ws_waves = None
 
@@ -89,6 +98,35 @@ class DagPlan:
         _lib.check(_lib.lib().gh_dag_crown_f64(self._h, batches.data_ptr(), ws.data_ptr(),
                                                ws.numel(), waves, _lib.stream_ptr(stream)),
                    "dag crown")
+
+    def tune_waves(self, batches, total_boxes, tuner=None):
+        """Resident wavefronts for this plan and batch size, picked by the
+        autotuner among 2, 4, 8 and 12 waves per compute unit."""
+        from .autotune import default_tuner
+        tuner = tuner or default_tuner()
+        props = torch.cuda.get_device_properties(self.device)
+        cus = props.multi_processor_count
+        cands = [k * cus for k in (2, 4, 8, 12)]
+        bucket = 1 << max(0, int(total_boxes).bit_length() - 1)
+        key = ("dag_crown_f64", getattr(props, "gcnArchName", ""), cus, self.jobs,
+               self.max_slots, self.tape_bytes, bucket)
+        waves, _ = tuner.pick(key, cands, lambda w: self.launch(batches, w))
+        return waves
+
+    def graph(self, batches, waves):
+        """A captured replay of the bound over `batches` (a
+        torch.cuda.CUDAGraph).  Counts are read on the device at replay, so
+        the same graph serves changing batch sizes up to each capacity."""
+        self.workspace(waves)
+        g = torch.cuda.CUDAGraph()
+        s = torch.cuda.Stream()
+        s.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(s):
+            self.launch(batches, waves)     # settles lazy library state outside capture
+        torch.cuda.current_stream().wait_stream(s)
+        with torch.cuda.graph(g):
+            self.launch(batches, waves)
+        return g
 
     def bound(self, xLs, xUs, waves=None):
         """lb per program (B_k, R_k), FP64."""

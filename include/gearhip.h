@@ -169,6 +169,12 @@ int gh_dag_plan_destroy(gh_dag_plan *plan);
 /* Rows (jobs) in the plan and per-lane state slots of the largest row. */
 int gh_dag_plan_info(const gh_dag_plan *plan, int64_t *jobs, int64_t *max_slots,
                      int64_t *tape_bytes);
+/* Per program (host arrays of nprog, each may be NULL), per box bound:
+ * bytes of per-box state the bound reads and writes (A propagation and
+ * relaxation data, as the reference's backward pass materializes them),
+ * bytes of box and bound I/O, and backward-pass node visits. */
+int gh_dag_plan_stats(const gh_dag_plan *plan, int64_t *state_bytes, int64_t *io_bytes,
+                      int64_t *steps);
 
 /* One program's batch of boxes (device pointers). */
 typedef struct {

@@ -138,3 +138,48 @@ def test_device_counts_follow_batch_size(cuda):
     got = lb.cpu().numpy()
     _check(got[:70], O.dag_crown(P, L[:70], U[:70]), "first 70")
     assert np.all(got[70:] == 7.0)
+
+
+def test_graph_replay_matches_eager(cuda):
+    """The captured bound replays correctly several times, with eager work
+    and a changed batch count between replays."""
+    from gear_amd.dag import DagPlan
+    progs = [random_nlp(8 + k, 2, seed=30 + k) for k in range(3)]
+    boxes = [random_bisection_boxes(P.xl, P.xu, 150, seed=k) for k, P in enumerate(progs)]
+    plan = DagPlan(progs, device=cuda)
+    xL = [_dev(L, cuda) for L, _ in boxes]
+    xU = [_dev(U, cuda) for _, U in boxes]
+    lb = [torch.zeros(150, P.R, dtype=torch.float64, device=cuda) for P in progs]
+    cnt = torch.tensor([150, 150, 150], dtype=torch.int64, device=cuda)
+    batches = plan.make_batches(xL, xU, lb, [cnt[k:k + 1] for k in range(3)])
+    g = plan.graph(batches, 64)
+    for rep, c in enumerate([150, 97, 150, 1]):
+        for t in lb:
+            t.fill_(-7.0)
+        cnt.fill_(c)
+        g.replay()
+        torch.cuda.synchronize()
+        _ = (xL[0] * 2).sum().item()        # eager work between replays
+        for k, P in enumerate(progs):
+            got = lb[k].cpu().numpy()
+            _check(got[:c], O.dag_crown(P, boxes[k][0][:c], boxes[k][1][:c]), f"replay {rep} prog {k}")
+            assert np.all(got[c:] == -7.0)
+
+
+def test_autotuned_waves_give_identical_bounds(cuda):
+    from gear_amd.autotune import Autotuner
+    from gear_amd.dag import DagPlan
+    P = random_nlp(12, 3, seed=40)
+    L, U = random_bisection_boxes(P.xl, P.xu, 500, seed=4)
+    plan = DagPlan([P], device=cuda)
+    xL, xU = _dev(L, cuda), _dev(U, cuda)
+    lb = torch.empty(500, P.R, dtype=torch.float64, device=cuda)
+    cnt = torch.tensor([500], dtype=torch.int64, device=cuda)
+    batches = plan.make_batches([xL], [xU], [lb], [cnt])
+    tuner = Autotuner()
+    w = plan.tune_waves(batches, 500, tuner=tuner)
+    assert w in [k * torch.cuda.get_device_properties(cuda).multi_processor_count for k in (2, 4, 8, 12)]
+    assert plan.tune_waves(batches, 500, tuner=tuner) == w      # cached
+    plan.launch(batches, w)
+    torch.cuda.synchronize()
+    _check(lb.cpu().numpy(), O.dag_crown(P, L, U), "tuned")
