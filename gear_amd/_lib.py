@@ -33,6 +33,10 @@ SIGNATURES = {
     "gh_filter_workspace_bytes": [_I],
     "gh_filter_append_f64": [_I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _L, _L, _P, _P, _P, _P],
     "gh_min_strided_f64": [_L, _L, _P, _P, _P],
+    "gh_qcqp_large_plan_bytes": [_I, _I],
+    "gh_qcqp_large_prepare": [_I, _I, _P, _P, _P],
+    "gh_qcqp_large_workspace_bytes": [_I, _I, _I],
+    "gh_qcqp_crown_f32s_large": [_I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, ctypes.c_size_t, _P],
     "gh_dag_plan_create": [_I, _P, ctypes.POINTER(_P)],
     "gh_dag_plan_destroy": [_P],
     "gh_dag_plan_info": [_P, _P, _P, _P],
@@ -41,7 +45,9 @@ SIGNATURES = {
     "gh_dag_crown_f64": [_P, _P, _P, ctypes.c_size_t, _I, _P],
 }
 _RESTYPES = {"gh_filter_workspace_bytes": ctypes.c_size_t, "gh_qcqp_pack_bytes": ctypes.c_size_t,
-             "gh_qcqp_pack_f32_bytes": ctypes.c_size_t, "gh_dag_workspace_bytes": ctypes.c_size_t}
+             "gh_qcqp_pack_f32_bytes": ctypes.c_size_t, "gh_dag_workspace_bytes": ctypes.c_size_t,
+             "gh_qcqp_large_plan_bytes": ctypes.c_size_t,
+             "gh_qcqp_large_workspace_bytes": ctypes.c_size_t}
 
 _lib = None
 

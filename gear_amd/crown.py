@@ -47,18 +47,33 @@ class BoundedQCQP:
     output_name = "/out"
     input_name = "/x"
 
-    def __init__(self, problem: QCQP, device="cuda", precision="fp64"):
+    # workspace cap of the large-n path (boxes are bounded in chunks that fit)
+    LARGE_WS_BYTES = 8 << 30
+
+    def __init__(self, problem: QCQP, device="cuda", precision="fp64", large=None):
         """precision: "fp64" (bit-identical to the FP64 oracle) or "fp32s"
-        (FP32 with a rigorous outward margin; never above the FP64 bound)."""
-        if problem.n > 64:
-            raise _lib.GearHipError("the fused QCQP kernels cover n <= 64")
+        (FP32 with a rigorous outward margin; never above the FP64 bound).
+        n > 64 (or large=True) takes the large-n FP32-sound path, whose dense
+        products are batched FP32 GEMMs; FP64 covers n <= 64."""
         if precision not in ("fp64", "fp32s"):
             raise ValueError(f"precision {precision!r}")
+        self.large = bool(large) or problem.n > 64
+        if self.large and precision != "fp32s":
+            raise _lib.GearHipError("the FP64 kernels cover n <= 64; use precision='fp32s'")
         self.problem = problem
         self.precision = precision
         self.dev = DeviceQCQP(problem, device)
         self.device = self.dev.device
         self.n, self.S = problem.n, problem.S
+        self.last_flagged = 0
+        self.packed = self.packed32 = None
+        self._ws = None
+        if self.large:
+            nb = int(lib().gh_qcqp_large_plan_bytes(self.n, self.S))
+            self.plan = torch.empty(nb, dtype=torch.uint8, device=self.device)
+            check(lib().gh_qcqp_large_prepare(self.n, self.S, ptr(self.dev.Q), ptr(self.plan),
+                                              stream_ptr()), "gh_qcqp_large_prepare")
+            return
         nbytes = int(lib().gh_qcqp_pack_bytes(self.n, self.S))
         self.packed = torch.empty(nbytes, dtype=torch.uint8, device=self.device)
         check(lib().gh_qcqp_pack_f64(self.n, self.S, ptr(self.dev.Q), ptr(self.packed),
@@ -103,12 +118,24 @@ class BoundedQCQP:
         B = x_L.shape[0]
         if x_L.shape != (B, self.n) or x_U.shape != (B, self.n):
             raise ValueError(f"boxes must be (B, {self.n})")
-        if self.packed32 is None:
-            raise _lib.GearHipError("constructed without precision='fp32s'")
         lb = out if out is not None else torch.empty(B, self.S, dtype=torch.float64,
                                                      device=self.device)
         flag = torch.empty(B, dtype=torch.int32, device=self.device)
         d = self.dev
+        if self.large:
+            per_box = int(lib().gh_qcqp_large_workspace_bytes(self.n, self.S, 1))
+            chunk = max(1, min(B, self.LARGE_WS_BYTES // per_box))
+            nb = int(lib().gh_qcqp_large_workspace_bytes(self.n, self.S, chunk))
+            if self._ws is None or self._ws.numel() < nb:
+                self._ws = torch.empty(nb, dtype=torch.uint8, device=self.device)
+            check(lib().gh_qcqp_crown_f32s_large(B, self.n, self.S, ptr(self.plan), ptr(d.Q),
+                                                 ptr(d.c), ptr(d.d), ptr(x_L), ptr(x_U), ptr(lb),
+                                                 ptr(flag), ptr(self._ws), self._ws.numel(),
+                                                 stream_ptr(stream)), "gh_qcqp_crown_f32s_large")
+            self.last_flag = flag
+            return lb
+        if self.packed32 is None:
+            raise _lib.GearHipError("constructed without precision='fp32s'")
         check(lib().gh_qcqp_crown_f32s(B, self.n, self.S, ptr(self.packed32), ptr(d.Q), ptr(d.c),
                                        ptr(d.d), ptr(x_L), ptr(x_U), ptr(lb), ptr(flag),
                                        stream_ptr(stream)), "gh_qcqp_crown_f32s")

@@ -29,6 +29,7 @@
 // agree.
 #include "common.h"
 #include "gearhip_internal.h"
+#include "qcqp_choice.h"
 
 namespace {
 
@@ -77,38 +78,6 @@ struct F32Consts {
     float f64slack;  // covers the FP64 path's own rounding (relative)
     double kn64;   // FP64 y-bound margin factor (n + 16) * 4 * eps
 };
-
-// The FP64 kernel's plane choice for element j of row s of box b: the same
-// FMA chains over i (qcqp_crown.hip phase 1) and the same selection
-// arithmetic, so the result is the FP64 path's r bit for bit.
-__device__ __forceinline__ bool fp64_choice(int n, const double* __restrict__ Qrow,
-                                         const double* __restrict__ lo,
-                                         const double* __restrict__ hi, int j, double kn)
-{
-    double c = 0.0, dv = 0.0, mg = 0.0;
-    for (int i = 0; i < n; ++i) {
-        double l = lo[i], h = hi[i];
-        double x0 = (h + l) / 2.0, ep = (h - l) / 2.0;
-        double mx = gh::finite_abs(fmax(-l, h));
-        double q = Qrow[i], qa = fabs(q);
-        c = fma(q, x0, c);
-        dv = fma(qa, ep, dv);
-        mg = fma(qa, mx, mg);
-    }
-    double e = mg * kn;
-    double yl = gh::widen((c - dv) + 0.0, e, -1.0);
-    double yu = gh::widen((c + dv) + 0.0, e, 1.0);
-    double xl = lo[j], xu = hi[j];
-    double a1 = yl, b1 = xl, c1 = (-xl) * yl;
-    double a2 = yu, b2 = xu, c2 = (-xu) * yu;
-    double m1 = a1 * (a1 >= 0 ? xl : xu);
-    m1 = m1 + b1 * (b1 >= 0 ? yl : yu);
-    m1 = m1 + c1;
-    double m2 = a2 * (a2 >= 0 ? xl : xu);
-    m2 = m2 + b2 * (b2 >= 0 ? yl : yu);
-    m2 = m2 + c2;
-    return m1 >= m2;
-}
 
 __device__ __forceinline__ float down_f32(double v)
 {
@@ -219,7 +188,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void qc
                             if (fabsf(m1 - m2) <= tau) {
                                 // near a tie: take the FP64 path's choice
                                 ambiguous = 1;
-                                r1 = fp64_choice(n, Q64 + ((size_t)s * n + j) * n,
+                                r1 = gh::fp64_choice(n, Q64 + ((size_t)s * n + j) * n,
                                                  xL + (size_t)bo * n, xU + (size_t)bo * n, j,
                                                  k.kn64);
                             }

@@ -93,6 +93,31 @@ int gh_qcqp_crown_f32s(int B, int n, int S, const void *packed,
                        int32_t *flag, hipStream_t stream);
 
 /*
+ * FP32-sound CROWN lower bound of a QCQP of any size.  Same contract and
+ * the same guarantee as gh_qcqp_crown_f32s (a valid lower bound, not above
+ * the FP64 bound; near-tie McCormick choices settled with the FP64 path's
+ * arithmetic, flag[b] = 1 when that happened), arranged for large n: the
+ * dense products y = Q_s x (centre and radius) and Q_s^T beta run as
+ * strided-batched FP32 GEMMs over all rows and boxes, with error terms that
+ * hold for any summation order; the relaxation, error terms and
+ * concretization run in FP64 from the FP32 products.
+ *   plan: gh_qcqp_large_plan_bytes(n, S) bytes, filled once by
+ *   gh_qcqp_large_prepare from Q (S, n, n) FP64;
+ *   Q64: the same Q (FP64, for the near-tie cases); Cm, d, xL, xU, lb, flag
+ *   as gh_qcqp_crown_f32s (lb holds FP64 values);
+ *   ws: workspace; gh_qcqp_large_workspace_bytes(n, S, k) bytes hold k
+ *   boxes, and B boxes are bounded in chunks of as many as ws holds.
+ * New surface (the reference has FP64 only, CV/gear/config.py:219-221).
+ */
+size_t gh_qcqp_large_plan_bytes(int n, int S);
+int gh_qcqp_large_prepare(int n, int S, const double *Q, void *plan, hipStream_t stream);
+size_t gh_qcqp_large_workspace_bytes(int n, int S, int B);
+int gh_qcqp_crown_f32s_large(int B, int n, int S, const void *plan, const double *Q64,
+                             const double *Cm, const double *d, const double *xL,
+                             const double *xU, double *lb, int32_t *flag, void *ws,
+                             size_t ws_bytes, hipStream_t stream);
+
+/*
  * Widest-variable branching: idx[b] = argmax_i (xU[b,i] - xL[b,i]), the
  * lowest index on ties.  Replaces input_split_branching(...,
  * branching_method='naive') (complete_verifier/input_split/
