@@ -22,6 +22,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
 
 FP64_PEAK_TFLOPS = 78.6
+FP32_PEAK_TFLOPS = 157.3
 # HBM bytes per bound launch of 32,768 boxes, measured with rocprofv3
 # FETCH_SIZE (x2.00) + WRITE_SIZE (x1.00), scale factors from a 256 MiB copy
 # (profiles/qcqp_crown_f64_n50_m20.md); scaled to the launch size below.
@@ -45,6 +46,10 @@ def parse():
     ap.add_argument("--no-dag", action="store_true", help="skip the expression-program section")
     ap.add_argument("--dag-only", action="store_true",
                     help="run only the expression-program section (256 uneven instances)")
+    ap.add_argument("--no-large", action="store_true", help="skip the n=500 FP32-sound section")
+    ap.add_argument("--large-only", action="store_true",
+                    help="run only the n=500, m=300 FP32-sound section")
+    ap.add_argument("--large-boxes", type=int, default=2048)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device 5 times (a known-size "
                          "reference for the HBM byte counters) and exit")
@@ -149,6 +154,71 @@ def bench_uneven_dag(reps=20, cpu=True):
     return out
 
 
+def bench_precision_large(boxes=2048, reps=3, cpu=True, sample=16):
+    """Dense QCQP n = 500, m = 300 (config-4 generator), FP32-sound bound:
+    batched FP32 GEMMs for the dense products, FP64 relaxation; looseness
+    and violations against the FP64 oracle on a sample of the boxes."""
+    import numpy as np
+    import torch
+
+    from gear_amd.crown import BoundedQCQP
+    from gear_amd.problems import config4_problem, random_bisection_boxes
+
+    dev = torch.device("cuda:0")
+    p = config4_problem()
+    n, S = p.n, p.S
+    L, U = random_bisection_boxes(p.xl, p.xu, boxes, seed=7)
+    net = BoundedQCQP(p, dev, precision="fp32s")
+    xL = torch.as_tensor(L, device=dev)
+    xU = torch.as_tensor(U, device=dev)
+    lb = net.bound(xL, xU)
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        net.bound(xL, xU, out=lb)
+    e1.record(stream)
+    e1.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    from gear_amd.problems import flops_per_box
+    fpb = flops_per_box(n, S)
+    achieved = fpb * boxes / (ms * 1e-3) / 1e12
+    out = {
+        "value": round(boxes / (ms * 1e-3), 1),
+        "unit": "subdomain bounds/s",
+        "ms_per_launch": round(ms, 3),
+        "dtype": "f32 (dense products) + f64 (relaxation, error terms)",
+        "config": {"workload": "dense QCQP n=500, m=300, coefficients N(0,1)/sqrt(n), "
+                               "random bisection boxes of [-1, 1]^500",
+                   "boxes": boxes, "n": n, "m": S - 1},
+        "roofline": {"bound": "mfma",
+                     "kernel": "FP32 GEMMs (Q x0, |Q| [ep | |x0|], Q^T beta) + relax + concretize",
+                     "achieved": round(achieved, 2), "peak": FP32_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
+                     "flops_per_box": fpb,
+                     "note": "whole bound (all kernels) per launch; the three GEMMs are "
+                             "81% of the time (profiles/qcqp_large_f32s_n500_m300.md)"},
+        "near_tie_boxes": int(net.last_flag.sum()),
+    }
+    if cpu:
+        from oracle import oracle as O
+        cores = min(16, os.cpu_count() or 1)
+        k = min(sample, boxes)
+        t = time.perf_counter()
+        lb64 = O.qcqp_crown(p.Q, p.c, p.d, L[:k], U[:k], threads=cores)
+        t = time.perf_counter() - t
+        lb32 = lb[:k].cpu().numpy()
+        loose = (lb64 - lb32) / (np.abs(lb64) + 1.0)
+        out["soundness"] = {"sample_boxes": k, "violations_vs_fp64_oracle": int(np.sum(lb32 > lb64)),
+                            "looseness_max": float(loose.max()),
+                            "looseness_mean": float(loose.mean())}
+        out["cpu_baseline"] = {"value": round(k / t, 2), "unit": "subdomain bounds/s",
+                               "cores": cores, "kind": "port",
+                               "sample": f"{k} of the boxes, FP64 oracle ({t:.2f} s wall)"}
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -159,6 +229,9 @@ def main():
         return
     if args.dag_only:
         print(json.dumps(bench_uneven_dag(cpu=not args.no_cpu)), flush=True)
+        return
+    if args.large_only:
+        print(json.dumps(bench_precision_large(args.large_boxes, cpu=not args.no_cpu)), flush=True)
         return
     import numpy as np
     import torch
@@ -329,6 +402,8 @@ def main():
                                             f"({tc:.2f} s wall)"}
     if not args.no_dag:
         result["uneven_dag"] = bench_uneven_dag(cpu=not args.no_cpu)
+    if not args.no_large:
+        result["precision_large"] = bench_precision_large(args.large_boxes, cpu=not args.no_cpu)
     print(json.dumps(result), flush=True)
 
 

@@ -8,16 +8,17 @@
 // large n:
 //   1. boxes are rounded outward to FP32 and written as centre / radius
 //      columns (x0, ep) with box ⊆ [x0 - ep, x0 + ep];
-//   2. C_s = Q32_s x0 and DV_s = |Q32_s| ep for every row s and box: two
-//      strided-batched FP32 GEMMs (MFMA; rocBLAS with its default IEEE math
-//      mode), S batches of (n x n) (n x B);
+//   2. C_s = Q32_s x0, DV_s = |Q32_s| ep and CA_s = |Q32_s| |x0| for every
+//      row s and box: two strided-batched FP32 GEMMs (MFMA; rocBLAS with its
+//      default IEEE math mode), S batches of (n x n) (n x B) and
+//      (n x n) (n x 2B);
 //   3. per (s, b, i), in FP64 from the FP32 products: y bounds
 //      c -/+ dv -/+ E, with E covering the GEMM rounding for any summation
 //      order (gamma_n = n u / (1 - n u)), the rounding of Q to FP32 and the
 //      FP64 path's own margin; the plane choice of the FP64 path (elements
-//      within TAU of a tie take the FP64 path's exact choice); beta rounded
-//      to FP32, Ainit = fl32(alpha + Cm); per (s, b) sums of gamma (rounded
-//      down), |beta| mg and |Ainit| mx;
+//      within a rigorous distance of a tie take the FP64 path's exact
+//      choice); beta rounded to FP32, Ainit = fl32(alpha + Cm); per (s, b)
+//      sums of gamma (rounded down), |beta| mg and |Ainit| mx;
 //   4. A_s = Q32_s^T beta_s + Ainit_s: a third strided-batched FP32 GEMM;
 //   5. per (s, b): lb = d + sum gamma + min_box(A x) - E_c - slack, with
 //      E_c = (gamma_{n+1} + 2.05 u) (sum |beta| mg + sum |Ainit| mx) (the
@@ -91,7 +92,8 @@ large_prepare_kernel(int n, int S, const double *__restrict__ Q, float *__restri
 // largest magnitude of the outward FP32 box.
 __global__ void __launch_bounds__(64)
 large_boxes_kernel(int B, int n, const double *__restrict__ xL, const double *__restrict__ xU,
-                   float *__restrict__ X0, float *__restrict__ EP, double *__restrict__ mxmax)
+                   float *__restrict__ X0, float *__restrict__ EP, float *__restrict__ XA,
+                   double *__restrict__ mxmax)
 {
     const int b = blockIdx.x;
     if (b >= B) return;
@@ -104,6 +106,7 @@ large_boxes_kernel(int B, int n, const double *__restrict__ xL, const double *__
         float ep = up_f32(fmax((double)u32 - (double)x0, (double)x0 - (double)l32));
         X0[(size_t)b * n + j] = x0;
         EP[(size_t)b * n + j] = ep;
+        XA[(size_t)b * n + j] = fabsf(x0);
         m = fmax(m, fmax(fabs((double)l32), fabs((double)u32)));
     }
     m = wave_max(m);
@@ -119,41 +122,69 @@ struct LargeConsts {
 
 // Step 3: y bounds, plane choice, beta / Ainit and the (s, b) sums.
 // One wavefront per (s, b); sums: [gamma, |gamma|, |beta| mg, |Ainit| mx].
+//
+// Errors of the FP32 products (any summation order), with ca = |Q32| |x0|:
+//   |c - Q x0|   <= e_c  = (gamma_n + uq) (1 + gamma_n) ca
+//   |dv - |Q| ep| <= e_dv = (gamma_n + uq) (1 + 2 gamma_n) dv
+// (uq covers Q rounded to FP32).  The planes use y bounds widened by
+// E = e_c + e_dv + the FP64 path's margin + an FP64 rounding slack.  The
+// choice is decided on the unwidened estimate (c -/+ dv), whose m1 - m2
+// differs from the FP64 path's by at most
+//   |D_l + D_u| e_c + |D_u - D_l| e_d + FP64 rounding of m1, m2,
+// D_l, D_u being the derivatives of m1 - m2 in yl and yu; within that
+// distance of a tie the element takes the FP64 path's exact choice.
 __global__ void __launch_bounds__(64)
-large_relax_kernel(int B, int n, int S, const float *__restrict__ C, const float *__restrict__ DV,
-                   const float *__restrict__ rowabs, const double *__restrict__ mxmax,
-                   const double *__restrict__ Q64, const double *__restrict__ Cm,
-                   const double *__restrict__ xL, const double *__restrict__ xU,
-                   float *__restrict__ BETA, float *__restrict__ AOUT, double *__restrict__ sums,
-                   int *__restrict__ flag, LargeConsts k)
+large_relax_kernel(int B, int n, int S, const float *__restrict__ C, const float *__restrict__ DVCA,
+                   const double *__restrict__ mxmax, const double *__restrict__ Q64,
+                   const double *__restrict__ Cm, const double *__restrict__ xL,
+                   const double *__restrict__ xU, float *__restrict__ BETA, float *__restrict__ AOUT,
+                   double *__restrict__ sums, int *__restrict__ flag, LargeConsts k)
 {
     const size_t sb = blockIdx.x;  // s * B + b
     if (sb >= (size_t)S * B) return;
     const int s = (int)(sb / B), b = (int)(sb - (size_t)s * B);
-    const double mxb = mxmax[b];
+    (void)mxmax;
     const double *lo = xL + (size_t)b * n, *hi = xU + (size_t)b * n;
     double gs = 0.0, ga = 0.0, bm = 0.0, am = 0.0;
     int tie = 0;
     const size_t base = sb * n;
+    // DVCA per row s: columns [0, B) are |Q| ep, [B, 2B) are |Q| |x0|
+    const float *DV = DVCA + (size_t)s * 2 * B * n + (size_t)b * n;
+    const float *CA = DV + (size_t)B * n;
+    const double f = k.gn + k.uq;
     for (int i = threadIdx.x; i < n; i += 64) {
-        const double c = C[base + i], dv = DV[base + i];
-        const double mg = (double)rowabs[(size_t)s * n + i] * mxb * (1.0 + 4.0 * EPS64);
-        double E = (k.gn * (mg + dv * (1.0 + 2.0 * k.gn)) + k.uq * mg) * (1.0 + 1e-9)
-                   + 1e-12 * (fabs(c) + dv + mg) + 2.0 * k.kn64 * mg;
+        const double c = C[base + i], dv = DV[i], ca = CA[i];
+        const double e_c = f * (1.0 + k.gn) * ca;
+        const double e_dv = f * (1.0 + 2.0 * k.gn) * dv;
+        // |Q| mx <= |Q| (|x0| + ep) on the FP32 box, computed values rounded up
+        const double mg = (ca + dv) * (1.0 + 2.0 * k.gn) * (1.0 + 4.0 * EPS64);
+        const double m64 = 2.0 * k.kn64 * mg;
+        const double E = (e_c + e_dv) * (1.0 + 1e-9) + m64 + 1e-12 * (fabs(c) + dv + ca);
         const double yl = (c - dv) - E, yu = (c + dv) + E;
         const double xl = lo[i], xu = hi[i];
-        // the FP64 path's selection arithmetic (bivariate.py:205-261)
-        double a1 = yl, b1 = xl, c1 = (-xl) * yl;
-        double a2 = yu, b2 = xu, c2 = (-xu) * yu;
-        double m1 = a1 * (a1 >= 0 ? xl : xu);
-        m1 = m1 + b1 * (b1 >= 0 ? yl : yu);
-        m1 = m1 + c1;
-        double m2 = a2 * (a2 >= 0 ? xl : xu);
-        m2 = m2 + b2 * (b2 >= 0 ? yl : yu);
-        m2 = m2 + c2;
+        // choice on the unwidened estimate, the FP64 path's selection
+        // arithmetic (bivariate.py:205-261)
+        const double el = c - dv, eu = c + dv;
+        const double X1 = el >= 0 ? xl : xu, Y1 = xl >= 0 ? el : eu;
+        const double X2 = eu >= 0 ? xl : xu, Y2 = xu >= 0 ? el : eu;
+        double m1 = el * X1;
+        m1 = m1 + xl * Y1;
+        m1 = m1 + (-xl) * el;
+        double m2 = eu * X2;
+        m2 = m2 + xu * Y2;
+        m2 = m2 + (-xu) * eu;
+        const double d1l = X1 - xl + (xl >= 0 ? xl : 0.0), d1u = xl >= 0 ? 0.0 : xl;
+        const double d2l = xu >= 0 ? xu : 0.0, d2u = X2 - xu + (xu >= 0 ? 0.0 : xu);
+        const double Dl = d1l - d2l, Du = d1u - d2u;
+        const double ed = e_dv + m64;
+        double tau = fabs(Dl + Du) * e_c + fabs(Du - Dl) * ed;
+        const double xm = fmax(fabs(xl), fabs(xu));
+        // a sign of yl / yu may differ: |D_l|, |D_u| <= 4 max|x| in every region
+        if (fabs(el) <= e_c + ed || fabs(eu) <= e_c + ed)
+            tau = fmax(tau, 8.0 * xm * (e_c + ed));
+        const double T = 3.0 * xm * (fabs(el) + fabs(eu));
+        tau = (tau + 16.0 * EPS64 * T) * (1.0 + 1e-6) + 1e-300;
         bool r1 = m1 >= m2;
-        const double mxi = fmax(fabs(xl), fabs(xu));
-        const double tau = 16.0 * mxi * E + 1e-300;
         if (fabs(m1 - m2) <= tau || !(fabs(m1 - m2) < INFINITY)) {
             r1 = gh::fp64_choice(n, Q64 + ((size_t)s * n + i) * n, lo, hi, i, k.kn64);
             tie = 1;
@@ -262,11 +293,11 @@ LargePlanLayout plan_layout(int n, int S)
     return L;
 }
 
-// per-box workspace: X0, EP (n floats each), C, DV, BETA, A (S n floats
-// each), sums (4 S doubles), mxmax (1 double)
+// per-box workspace: X0, EP, |x0| (n floats each), C, BETA, A (S n floats
+// each), DV and CA (2 S n floats), sums (4 S doubles), mxmax (1 double)
 size_t per_box_bytes(int n, int S)
 {
-    return (size_t)2 * n * 4 + (size_t)4 * S * n * 4 + (size_t)4 * S * 8 + 8;
+    return (size_t)3 * n * 4 + (size_t)5 * S * n * 4 + (size_t)4 * S * 8 + 8;
 }
 
 }  // namespace
@@ -331,30 +362,32 @@ extern "C" int gh_qcqp_crown_f32s_large(int B, int n, int S, const void *plan, c
             return r;
         };
         float *X0 = reinterpret_cast<float *>(take((size_t)Bc * n * 4));
-        float *EP = reinterpret_cast<float *>(take((size_t)Bc * n * 4));
+        // EP and |x0| adjacent: one GEMM with 2 Bc columns gives DV and CA
+        float *EP = reinterpret_cast<float *>(take((size_t)2 * Bc * n * 4));
+        float *XA = EP + (size_t)Bc * n;
         float *Cb = reinterpret_cast<float *>(take((size_t)S * Bc * n * 4));
-        float *DVb = reinterpret_cast<float *>(take((size_t)S * Bc * n * 4));
+        float *DVb = reinterpret_cast<float *>(take((size_t)2 * S * Bc * n * 4));
         float *BETA = reinterpret_cast<float *>(take((size_t)S * Bc * n * 4));
         float *AOUT = reinterpret_cast<float *>(take((size_t)S * Bc * n * 4));
         double *sums = reinterpret_cast<double *>(take((size_t)S * Bc * 4 * 8));
         double *mxm = reinterpret_cast<double *>(take((size_t)Bc * 8));
         const double *cxL = xL + (size_t)b0 * n, *cxU = xU + (size_t)b0 * n;
         hipLaunchKernelGGL(large_boxes_kernel, dim3(Bc), dim3(64), 0, stream, Bc, n, cxL, cxU, X0,
-                           EP, mxm);
+                           EP, XA, mxm);
         const float one = 1.0f, zero = 0.0f;
         const rocblas_stride sq = (rocblas_stride)n * n, sc = (rocblas_stride)n * Bc;
         // C_s (n x Bc, column-major) = Q32_s x0; Q32_s row-major is Q32_s^T column-major
         rocblas_status st = rocblas_sgemm_strided_batched(
             h, rocblas_operation_transpose, rocblas_operation_none, n, Bc, n, &one, Q32, n, sq, X0,
             n, 0, &zero, Cb, n, sc, S);
-        if (st == rocblas_status_success)
+        if (st == rocblas_status_success)  // [DV | CA]_s = |Q32_s| [ep | |x0|]
             st = rocblas_sgemm_strided_batched(h, rocblas_operation_transpose,
-                                               rocblas_operation_none, n, Bc, n, &one, absQ, n, sq,
-                                               EP, n, 0, &zero, DVb, n, sc, S);
+                                               rocblas_operation_none, n, 2 * Bc, n, &one, absQ, n,
+                                               sq, EP, n, 0, &zero, DVb, n, 2 * sc, S);
         if (st != rocblas_status_success) return GH_ELAUNCH;
         hipLaunchKernelGGL(large_relax_kernel, dim3((unsigned)((size_t)S * Bc)), dim3(64), 0, stream,
-                           Bc, n, S, Cb, DVb, rowabs, mxm, Q64, Cm, cxL, cxU, BETA, AOUT, sums,
-                           flag + b0, k);
+                           Bc, n, S, Cb, DVb, mxm, Q64, Cm, cxL, cxU, BETA, AOUT, sums, flag + b0,
+                           k);
         // A_s = Q32_s^T beta_s + Ainit_s
         st = rocblas_sgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_none, n, Bc,
                                            n, &one, Q32, n, sq, BETA, n, sc, &one, AOUT, n, sc, S);
