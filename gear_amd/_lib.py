@@ -37,6 +37,9 @@ SIGNATURES = {
     "gh_qcqp_large_prepare": [_I, _I, _P, _P, _P],
     "gh_qcqp_large_workspace_bytes": [_I, _I, _I],
     "gh_qcqp_crown_f32s_large": [_I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, ctypes.c_size_t, _P],
+    "gh_alm_plan_bytes": [_I, _I],
+    "gh_alm_prepare": [_I, _I, _P, _P, _P],
+    "gh_alm_pgd_f64": [_I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P],
     "gh_dag_plan_create": [_I, _P, ctypes.POINTER(_P)],
     "gh_dag_plan_destroy": [_P],
     "gh_dag_plan_info": [_P, _P, _P, _P],
@@ -47,6 +50,7 @@ SIGNATURES = {
 _RESTYPES = {"gh_filter_workspace_bytes": ctypes.c_size_t, "gh_qcqp_pack_bytes": ctypes.c_size_t,
              "gh_qcqp_pack_f32_bytes": ctypes.c_size_t, "gh_dag_workspace_bytes": ctypes.c_size_t,
              "gh_qcqp_large_plan_bytes": ctypes.c_size_t,
+             "gh_alm_plan_bytes": ctypes.c_size_t,
              "gh_qcqp_large_workspace_bytes": ctypes.c_size_t}
 
 _lib = None

@@ -16,6 +16,8 @@ feas_tol; its objective is replayed in FP64 (gear/report.py:114-141).
 """
 from dataclasses import dataclass
 
+import ctypes
+
 import torch
 
 
@@ -50,9 +52,53 @@ def qcqp_values(dev, x):
     return (x.unsqueeze(1) * y.view(-1, S, n)).sum(-1) + x @ dev.c.t() + dev.d
 
 
+class _GhAlmConfig(ctypes.Structure):
+    """gh_alm_config of include/gearhip.h."""
+    _fields_ = [(k, ctypes.c_double) for k in ("rho0", "kappa", "max_rho", "inner_lr", "feas_tol",
+                                               "beta1", "beta2", "adam_eps", "lr0")] + \
+               [(k, ctypes.c_int32) for k in ("outer_max_iter", "inner_steps",
+                                              "scale_eps_with_bias", "pad")]
+
+
 class ALMPGDSolver:
-    def __init__(self, cfg: ALMConfig = None):
+    """backend "hip": the fused kernel gh_alm_pgd_f64 (one wavefront per
+    restart); "torch": the same iteration as device tensor operations with
+    autograd (kept as a second implementation the kernel is tested
+    against)."""
+
+    def __init__(self, cfg: ALMConfig = None, backend="hip"):
         self.cfg = cfg or ALMConfig()
+        if backend not in ("hip", "torch"):
+            raise ValueError(backend)
+        self.backend = backend
+        self._plans = {}
+
+    def _plan(self, dev):
+        key = id(dev)
+        if key not in self._plans:
+            from . import _lib
+            nb = int(_lib.lib().gh_alm_plan_bytes(dev.n, dev.S))
+            plan = torch.empty(nb // 8, dtype=torch.float64, device=dev.Q.device)
+            _lib.check(_lib.lib().gh_alm_prepare(dev.n, dev.S, dev.Q.data_ptr(), plan.data_ptr(),
+                                                 _lib.stream_ptr()), "gh_alm_prepare")
+            self._plans[key] = (dev, plan)
+        return self._plans[key][1]
+
+    def _run_hip(self, dev, lo, hi, x, lr0):
+        from . import _lib
+        cfg = self.cfg
+        N, n = x.shape
+        c = _GhAlmConfig(cfg.rho0, cfg.kappa, cfg.max_rho, cfg.inner_lr, cfg.feas_tol, cfg.beta1,
+                         cfg.beta2, cfg.adam_eps, lr0, cfg.outer_max_iter, cfg.inner_steps,
+                         int(cfg.scale_eps_with_bias), 0)
+        best_x = torch.empty_like(x)
+        best_obj = torch.empty(N, dtype=torch.float64, device=x.device)
+        _lib.check(_lib.lib().gh_alm_pgd_f64(N, n, dev.S, self._plan(dev).data_ptr(),
+                                             dev.c.data_ptr(), dev.d.data_ptr(),
+                                             dev.rhs.data_ptr(), lo.data_ptr(), hi.data_ptr(),
+                                             x.data_ptr(), best_x.data_ptr(), best_obj.data_ptr(),
+                                             ctypes.byref(c), _lib.stream_ptr()), "gh_alm_pgd_f64")
+        return best_x, best_obj
 
     def solve_with_restarts(self, dev, x_L, x_U, num_restarts=None, x_init=None):
         """Search every box x_L/x_U (B, n) with R restarts each; returns the
@@ -69,6 +115,17 @@ class ALMPGDSolver:
         x = lo + u * width
         x.view(B, R, n)[:, 0] = (x_L + x_U) / 2 if x_init is None else x_init
         N = x.shape[0]
+        if self.backend == "hip":
+            lr0 = cfg.inner_lr * float(width.max()) if N else 0.0
+            best_x, best_obj = self._run_hip(dev, lo.contiguous(), hi.contiguous(),
+                                             x.contiguous(), lr0)
+            if N == 0:
+                return ALMResult(None, float("inf"), float("inf"))
+            i = int(torch.argmin(best_obj))
+            obj = float(best_obj[i])
+            if obj == float("inf"):
+                return ALMResult(None, float("inf"), float("inf"))
+            return ALMResult(best_x[i].detach().clone(), obj, 0.0)
         m = dev.S - 1
         rhs = dev.rhs[1:]
         mu = torch.zeros(N, m, dtype=x.dtype, device=x.device)

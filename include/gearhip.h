@@ -118,6 +118,33 @@ int gh_qcqp_crown_f32s_large(int B, int n, int S, const void *plan, const double
                              size_t ws_bytes, hipStream_t stream);
 
 /*
+ * ALM-PGD primal search, fused: N independent restarts of the reference's
+ * ALMPGDSolver.solve_with_restarts (complete_verifier/nlp_solver/
+ * alm.py:1268-2065; inner Adam :551-788, AdamClipping_NLP.step :134-160,
+ * ALMConfig :163-268) on a QCQP, one wavefront per restart, every outer and
+ * inner iteration in one launch.
+ *   plan: gh_alm_plan_bytes(n, S) bytes, filled once by gh_alm_prepare from
+ *   Q (S, n, n) (the symmetric parts (Q + Q^T) / 2);
+ *   Cm (S, n), d (S), rhs (S): objective / constraint data (rhs[0] unused);
+ *   lo, hi (N, n): each restart's box; x (N, n): starting points in, last
+ *   iterates out; best_x (N, n), best_obj (N): each restart's best feasible
+ *   point (violation <= feas_tol) and objective (+inf if none);
+ *   cfg: HOST pointer; lr0 = inner_lr * the largest box width over all N.
+ * Supported: n <= 1024, S <= 1024.
+ */
+typedef struct {
+    double rho0, kappa, max_rho, inner_lr, feas_tol, beta1, beta2, adam_eps;
+    double lr0;
+    int32_t outer_max_iter, inner_steps, scale_eps_with_bias, pad;
+} gh_alm_config;
+size_t gh_alm_plan_bytes(int n, int S);
+int gh_alm_prepare(int n, int S, const double *Q, void *plan, hipStream_t stream);
+int gh_alm_pgd_f64(int N, int n, int S, const void *plan, const double *Cm, const double *d,
+                   const double *rhs, const double *lo, const double *hi, double *x,
+                   double *best_x, double *best_obj, const gh_alm_config *cfg,
+                   hipStream_t stream);
+
+/*
  * Widest-variable branching: idx[b] = argmax_i (xU[b,i] - xL[b,i]), the
  * lowest index on ties.  Replaces input_split_branching(...,
  * branching_method='naive') (complete_verifier/input_split/
