@@ -50,6 +50,13 @@ def parse():
     ap.add_argument("--large-only", action="store_true",
                     help="run only the n=500, m=300 FP32-sound section")
     ap.add_argument("--large-boxes", type=int, default=2048)
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving section")
+    ap.add_argument("--serve-only", action="store_true",
+                    help="run only the serving section (Poisson stream of QCQP instances)")
+    ap.add_argument("--serve-requests", type=int, default=24)
+    ap.add_argument("--serve-rate", type=float, default=8.0, help="arrivals per second")
+    ap.add_argument("--serve-plain-budget", type=float, default=60.0,
+                    help="seconds the plain sequential form may run")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device 5 times (a known-size "
                          "reference for the HBM byte counters) and exit")
@@ -219,6 +226,53 @@ def bench_precision_large(boxes=2048, reps=3, cpu=True, sample=16):
     return out
 
 
+def bench_serving(requests=24, rate=8.0, plain_budget=60.0, plain=True):
+    """Serving: a Poisson stream of QCQP instances from the config-2 generator
+    (n lognormal(ln 8, 0.6) clipped to [5, 1000], m = 0.4 n), admitted by HBM
+    budget and solved to a 1e-4 relative gap with continuous batching,
+    FP32-sound bounds and FP64-replayed incumbents; beside it the plain FP64
+    form (op-by-op bound, autograd ALM) one instance at a time."""
+    import torch
+
+    from gear_amd.serve import ServeConfig, poisson_stream, serve, serve_plain_sequential
+
+    dev = torch.device("cuda:0")
+    reqs = poisson_stream(requests, rate, seed=11, n_median=8.0, n_sigma=0.6)
+    cfg = ServeConfig()
+    # warm the kernels once (not timed)
+    serve(poisson_stream(2, 100.0, seed=99, n_median=6, n_sigma=0.2), cfg, device=dev,
+          realtime=False)
+    res, summ = serve(reqs, cfg, device=dev, realtime=True)
+    out = {"value": round(summ["subdomain_bounds_per_s"], 1),
+           "unit": "subdomain bounds/s",
+           "p50_time_to_gap_s": summ["p50_time_to_gap_s"],
+           "p99_time_to_gap_s": summ["p99_time_to_gap_s"],
+           "converged": summ["converged"], "timeout": summ["timeout"],
+           "infeasible": summ["infeasible"], "censored": summ["censored"],
+           "wall_s": round(summ["wall_s"], 3),
+           "config": {"workload": f"{requests} QCQP instances, Poisson arrivals at {rate}/s, "
+                                  "n lognormal(ln 8, 0.6) in [5, 1000], m = 0.4 n, "
+                                  "rel gap 1e-4, time limit 10 s per instance",
+                      "n": [r["n"] for r in res], "precision": cfg.precision,
+                      "hbm_budget_bytes": cfg.hbm_budget},
+           "time_to_gap_s": [round(r["time_to_gap"], 4) if r["time_to_gap"] < float("inf")
+                             else None for r in res]}
+    if plain:
+        pres, psumm = serve_plain_sequential(reqs, cfg, device=dev, budget_s=plain_budget)
+        out["plain_fp64_sequential"] = {
+            "value": round(psumm["subdomain_bounds_per_s"], 1), "unit": "subdomain bounds/s",
+            "p50_time_to_gap_s": psumm["p50_time_to_gap_s"],
+            "p99_time_to_gap_s": psumm["p99_time_to_gap_s"],
+            "converged": psumm["converged"], "timeout": psumm["timeout"],
+            "censored": psumm["censored"],
+            "not_started": sum(r["status"] == "not_started" for r in pres),
+            "wall_s": round(psumm["wall_s"], 3), "budget_s": plain_budget}
+        same = [abs(a["D"] - b["D"]) <= 3e-4 * max(1.0, abs(a["P"]))
+                for a, b in zip(res, pres) if a["status"] == b["status"] == "converged"]
+        out["plain_fp64_sequential"]["certificates_agree"] = f"{sum(same)}/{len(same)}"
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -232,6 +286,11 @@ def main():
         return
     if args.large_only:
         print(json.dumps(bench_precision_large(args.large_boxes, cpu=not args.no_cpu)), flush=True)
+        return
+    if args.serve_only:
+        print(json.dumps(bench_serving(args.serve_requests, args.serve_rate,
+                                       args.serve_plain_budget, plain=not args.no_plain)),
+              flush=True)
         return
     import numpy as np
     import torch
@@ -404,6 +463,9 @@ def main():
         result["uneven_dag"] = bench_uneven_dag(cpu=not args.no_cpu)
     if not args.no_large:
         result["precision_large"] = bench_precision_large(args.large_boxes, cpu=not args.no_cpu)
+    if not args.no_serve:
+        result["serving"] = bench_serving(args.serve_requests, args.serve_rate,
+                                          args.serve_plain_budget, plain=not args.no_plain)
     print(json.dumps(result), flush=True)
 
 

@@ -1,8 +1,9 @@
 """Batched branch and bound over input boxes on one device.
 
 One iteration follows the reference's regular order
-(complete_verifier/input_split/batch_branch_and_bound.py:275-652):
-pick out the last `batch` domains -> branch on the widest variable -> bisect
+(complete_verifier/input_split/batch_branch_and_bound.py:275-652, 2660-2663):
+sort the frontier (least verified last) -> pick out the last `batch` domains
+-> branch on the widest variable -> bisect
 -> bound the children (CROWN) -> prune against the thresholds -> append the
 survivors; then the global lower bound is the frontier minimum capped at the
 discarded-threshold minimum (:620-634), and the gap test is
@@ -41,6 +42,7 @@ class BaBConfig:
     max_iters: int = 1000
     time_limit: float = 60.0
     alm_interval: int = 0          # 0: ALM only at the root
+    sort_interval: int = 1         # sort the frontier every k iterations (0: never)
     alm: ALMConfig = field(default_factory=ALMConfig)
 
 
@@ -124,6 +126,9 @@ class QCQPBranchAndBound:
     def step(self, batch=None, sync=True):
         """One iteration; returns the number of children bounded."""
         f = self.frontier
+        si = self.cfg.sort_interval
+        if si > 0 and self.stats.iterations % si == 0:
+            f.sort(self.thr)
         # children (2 nb) go where the nb parents were plus free rows
         nb = min(batch or self.cfg.batch, len(f), f.cap - len(f))
         if nb <= 0:

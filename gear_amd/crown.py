@@ -216,6 +216,24 @@ class DeviceFrontier:
         self.lb[self.count:self.count + B] = lb
         self.count += B
 
+    def sort(self, thr):
+        """Order the rows by their largest margin max_s (lb_s - thr_s),
+        descending, so picking from the end takes the least verified domains
+        first: UnsortedInputDomainList.sort with sort_heuristic 'lb_rhs',
+        sort_index None and sort_descending, as the GEAR presets set it
+        (complete_verifier/input_split/branching_domains.py:1171-1241,
+        976-981; gear/presets/gear-f0.yaml:409, 550-552).  Ties keep their
+        current order (a stable sort; the reference's argsort leaves tie
+        order unspecified)."""
+        c = self.count
+        if c <= 1:
+            return
+        keys = (self.lb[:c] - thr).amax(1)
+        order = torch.argsort(keys, descending=True, stable=True)
+        self.L[:c] = self.L[:c].index_select(0, order)
+        self.U[:c] = self.U[:c].index_select(0, order)
+        self.lb[:c] = self.lb[:c].index_select(0, order)
+
     def pick_out(self, batch):
         """Views of the last `batch` rows (the parents of the next split);
         the rows stay valid until the next append overwrites them."""

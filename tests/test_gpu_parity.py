@@ -121,8 +121,9 @@ def test_filter_append_matches_oracle(cuda, B):
 
 
 def test_bab_iterations_match_oracle_loop(cuda):
-    """Three B&B iterations on the device reproduce the oracle-driven loop
-    (pick from the end, widest branch, bisect, bound, prune, append)."""
+    """Six B&B iterations on the device reproduce the oracle-driven loop
+    (sort by the largest margin lb - thr, descending and stable; pick from
+    the end, widest branch, bisect, bound, prune, append)."""
     from gear_amd.bab import BaBConfig, QCQPBranchAndBound
     p = qcqp_dense(10, 3, seed=11)
     bb = QCQPBranchAndBound(p, BaBConfig(batch=64, cap=4096), device=cuda)
@@ -133,6 +134,9 @@ def test_bab_iterations_match_oracle_loop(cuda):
     Flb = O.qcqp_crown(p.Q, p.c, p.d, FL, FU)
     thr = np.concatenate([[-1.0], p.rhs[1:] + 1e-6])
     for it in range(6):
+        keys = (Flb - thr).max(axis=1)
+        order = np.argsort(-keys, kind="stable")
+        FL, FU, Flb = FL[order], FU[order], Flb[order]
         nb = min(64, len(FL))
         pL, pU = FL[-nb:], FU[-nb:]
         FL, FU, Flb = FL[:-nb], FU[:-nb], Flb[:-nb]
