@@ -1,4 +1,6 @@
-"""Benchmark: subdomain bounds per second of the batched B&B loop on one GPU.
+"""Benchmark: subdomain bounds per second of the batched B&B loop on one GPU,
+with the expression-program, large-n FP32-sound and serving sections beside
+it (each its own JSON object on the same line).
 
 Workload: seeded nonconvex QCQP, n = 50, m = 20, coefficients N(0,1)/sqrt(n),
 x in [-1, 1]^50; the loop bounds (CROWN, FP64), branches on the widest
@@ -362,8 +364,28 @@ def main():
     achieved_tf = fpb * avg_boxes / avg_launch / 1e12
     value = boxes / wall
 
+    # the same loop with FP32-sound bounds (fresh tree, same start)
+    bb32 = QCQPBranchAndBound(p, BaBConfig(batch=args.batch, cap=cap), device=dev,
+                              precision="fp32s")
+    if bb.x_best is not None:
+        bb32.set_incumbent(bb.P, bb.x_best)
+    bb32.presplit(args.batch, alm=False)
+    for _ in range(args.warmup):
+        bb32.step()
+    torch.cuda.synchronize()
+    gc.collect()
+    gc.disable()
+    boxes32 = 0
+    t32 = time.perf_counter()
+    for _ in range(args.steps):
+        boxes32 += bb32.step()
+    torch.cuda.synchronize()
+    t32 = time.perf_counter() - t32
+    gc.enable()
+    bb32.update_dual()
+
     result = {
-        "metric": "subdomain bounds/sec (full batched B&B loop, FP64 CROWN)",
+        "metric": "subdomain bounds/sec + p50/p99 time-to-1e-4 gap, served FP32-sound vs plain FP64",
         "value": round(value, 1),
         "unit": "subdomain bounds/s",
         "n_gpus": 1,
@@ -396,6 +418,10 @@ def main():
         "step_ms": step_ms,
         "bab": {"frontier": len(bb.frontier), "P": bb.P, "D": bb.D,
                 "root_alm_s": round(t_alm, 3)},
+        "bab_fp32_sound": {"value": round(boxes32 / t32, 1), "unit": "subdomain bounds/s",
+                           "ms_per_step": round(t32 / args.steps * 1e3, 4),
+                           "frontier": len(bb32.frontier), "D": bb32.D,
+                           "note": "the same loop and start with FP32-sound bounds"},
     }
 
     if sample_parents is not None and not args.no_plain:
@@ -466,6 +492,18 @@ def main():
     if not args.no_serve:
         result["serving"] = bench_serving(args.serve_requests, args.serve_rate,
                                           args.serve_plain_budget, plain=not args.no_plain)
+        sv = result["serving"]
+        head = {"served_fp32s_p50_time_to_gap_s": sv["p50_time_to_gap_s"],
+                "served_fp32s_p99_time_to_gap_s": sv["p99_time_to_gap_s"],
+                "served_fp32s_subdomain_bounds_per_s": sv["value"]}
+        if "plain_fp64_sequential" in sv:
+            pl = sv["plain_fp64_sequential"]
+            head.update({"plain_fp64_p50_time_to_gap_s": pl["p50_time_to_gap_s"],
+                         "plain_fp64_p99_time_to_gap_s": pl["p99_time_to_gap_s"],
+                         "plain_fp64_subdomain_bounds_per_s": pl["value"],
+                         "served_vs_plain_bounds_per_s": round(sv["value"] / pl["value"], 2)
+                         if pl["value"] else None})
+        result["headline"] = head
     print(json.dumps(result), flush=True)
 
 

@@ -58,10 +58,10 @@ class BaBStats:
 class QCQPBranchAndBound:
     """Device-resident B&B over one QCQP instance."""
 
-    def __init__(self, problem, cfg: BaBConfig = None, device="cuda"):
+    def __init__(self, problem, cfg: BaBConfig = None, device="cuda", precision="fp64"):
         self.p = problem
         self.cfg = cfg or BaBConfig()
-        self.net = BoundedQCQP(problem, device)
+        self.net = BoundedQCQP(problem, device, precision=precision)
         self.device = self.net.device
         self.frontier = DeviceFrontier(problem.n, problem.S, self.cfg.cap, self.device)
         self.alm = ALMPGDSolver(self.cfg.alm)
