@@ -22,11 +22,15 @@
 // Since yl32 <= yl64 and yu32 >= yu64, each FP32 plane lies below the FP64
 // plane of the same choice on the box, so the FP32 bound is not above the
 // FP64 one when both pick the same planes (a relative slack of 1e-12 on the
-// magnitudes covers the FP64 path's own rounding).  Where |m1 - m2| is within
-// TAU of a tie (TAU bounds the difference between the FP32 and FP64 values of
-// m1 - m2) the lane recomputes that element's y interval with the FP64
-// path's exact FMA chains and takes the FP64 choice, so the choices always
-// agree.
+// magnitudes covers the FP64 path's own rounding).  The magnitude term is
+// mg = ca + dv with ca = |Q32| |x0| (a third MFMA chain), which bounds
+// |Q32| mx on the FP32 box.  The plane choice is decided on the unwidened
+// estimate c -/+ dv; where its m1 - m2 is within the rigorous distance
+// |D_l + D_u| e_c + |D_u - D_l| e_d (+ rounding) of a tie, D_l, D_u being
+// the derivatives of m1 - m2 in yl and yu, e_c and e_d the errors of the
+// centre and radius, the lane recomputes that element's y interval with the
+// FP64 path's exact FMA chains and takes the FP64 choice, so the choices
+// always agree.
 #include "common.h"
 #include "gearhip_internal.h"
 #include "qcqp_choice.h"
@@ -150,7 +154,6 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void qc
                 float x0 = (xu[t] + xl[t]) * 0.5f;
                 float ep = (xu[t] - xl[t]) * 0.5f;
                 ep = ep + (ep + fabsf(x0)) * (4.0f * k.u);   // covers both roundings
-                float mx = fmaxf(-xl[t], xu[t]);
 #pragma unroll
                 for (int g = 0; g < G; ++g) {
                     if (c0 + g < NCH) {
@@ -158,7 +161,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void qc
                         float qa_ = fabsf(q);
                         Dc[g] = mfma16f(q, x0, Dc[g]);
                         Dd[g] = mfma16f(qa_, ep, Dd[g]);
-                        Dm[g] = mfma16f(qa_, mx, Dm[g]);
+                        Dm[g] = mfma16f(qa_, fabsf(x0), Dm[g]);
                     }
                 }
                 if (t % 4 == 3) __builtin_amdgcn_sched_barrier(0);
@@ -173,19 +176,43 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void qc
                         acm[m] = 0.0f;
                         beta[m] = 0.0f;
                         if (j < n) {
-                            float cc = Dc[g][v], dv = Dd[g][v], mg = Dm[g][v];
+                            // ca = |Q32| |x0|; |Q32| mx <= |Q32| (|x0| + ep) = ca + dv
+                            // on the FP32 box, the computed sums rounded up
+                            float cc = Dc[g][v], dv = Dd[g][v], ca = Dm[g][v];
+                            float mg = (ca + dv) * (1.0f + 2.0f * k.g + 2.0f * k.u);
                             float ey = k.ky * (mg + dv) + k.tiny;
                             float yl = (cc - dv) - ey, yu = (cc + dv) + ey;
                             float xlj = xl[m], xuj = xu[m];
-                            float a1 = yl, b1 = xlj, c1 = (-xlj) * yl;
-                            float a2 = yu, b2 = xuj, c2 = (-xuj) * yu;
-                            float m1 = a1 * (a1 >= 0 ? xlj : xuj) + b1 * (b1 >= 0 ? yl : yu) + c1;
-                            float m2 = a2 * (a2 >= 0 ? xlj : xuj) + b2 * (b2 >= 0 ? yl : yu) + c2;
+                            // the choice is decided on the unwidened estimate, with
+                            // the FP64 path's selection arithmetic (bivariate.py:205-261)
+                            float el = cc - dv, eu = cc + dv;
+                            float X1 = el >= 0 ? xlj : xuj, Y1 = xlj >= 0 ? el : eu;
+                            float X2 = eu >= 0 ? xlj : xuj, Y2 = xuj >= 0 ? el : eu;
+                            float m1 = el * X1 + xlj * Y1 + (-xlj) * el;
+                            float m2 = eu * X2 + xuj * Y2 + (-xuj) * eu;
+                            // distance of the estimate's m1 - m2 from the FP64 path's:
+                            // errors of the centre (e_c, from the chain and the box
+                            // rounding, proportional to ca) and of the radius (e_d,
+                            // proportional to dv, with the FP64 margin), through the
+                            // derivatives D_l, D_u of m1 - m2 in yl, yu
                             float mxj = fmaxf(-xlj, xuj);
-                            float ymax = fmaxf(fabsf(yl), fabsf(yu));
-                            float tau = 6.0f * mxj * (ey + 8.0f * k.u * ymax) + k.tiny;
+                            float e_c = ((k.g + 4.0f * k.u) * (1.0f + 2.0f * k.g) * ca
+                                         + 4.0f * k.u * dv) * 1.01f;
+                            float e_d = ((k.g + 8.0f * k.u) * (1.0f + 2.0f * k.g) * dv
+                                         + 8.0f * k.u * ca + 2.0f * (float)k.kn64 * mg) * 1.01f;
+                            float d1l = X1 - xlj + (xlj >= 0 ? xlj : 0.0f);
+                            float d1u = xlj >= 0 ? 0.0f : xlj;
+                            float d2l = xuj >= 0 ? xuj : 0.0f;
+                            float d2u = X2 - xuj + (xuj >= 0 ? 0.0f : xuj);
+                            float Dl = d1l - d2l, Du = d1u - d2u;
+                            float tau = fabsf(Dl + Du) * e_c + fabsf(Du - Dl) * e_d;
+                            float xm = fmaxf(fabsf(xlj), fabsf(xuj));
+                            if (fabsf(el) <= e_c + e_d || fabsf(eu) <= e_c + e_d)
+                                tau = fmaxf(tau, 8.0f * xm * (e_c + e_d));
+                            float T = 3.0f * xm * (fabsf(el) + fabsf(eu));
+                            tau = (tau + 16.0f * k.u * T) * 1.01f + k.tiny;
                             bool r1 = m1 >= m2;
-                            if (fabsf(m1 - m2) <= tau) {
+                            if (!(fabsf(m1 - m2) > tau)) {
                                 // near a tie: take the FP64 path's choice
                                 ambiguous = 1;
                                 r1 = gh::fp64_choice(n, Q64 + ((size_t)s * n + j) * n,
