@@ -454,6 +454,23 @@ def main():
             net32.bound(cL, cU, out=lb32)
         torch.cuda.synchronize()
         t3 = (time.perf_counter() - t3) / reps
+        # LP dual certificate (FP64, 500 PDHG iterations) on the same children
+        from gear_amd.lp import qcqp_lp_bound
+        _, lA_c, lbias_c = orig_bound(cL, cU, return_A=True)
+        thr_c = bb.thr
+        qcqp_lp_bound(lA_c, lbias_c, thr_c, cL, cU, iters=500)
+        torch.cuda.synchronize()
+        tl = time.perf_counter()
+        lp_c = qcqp_lp_bound(lA_c, lbias_c, thr_c, cL, cU, iters=500)
+        torch.cuda.synchronize()
+        tl = time.perf_counter() - tl
+        raised = int((lp_c > fused[:, 0] + 1e-9 * (1 + fused[:, 0].abs())).sum())
+        result["lp_certificate"] = {"value": round(cL.shape[0] / tl, 1),
+                                    "unit": "subdomain LP certificates/s",
+                                    "pdhg_iters": 500, "ms": round(tl * 1e3, 3),
+                                    "boxes_raised_above_crown": raised,
+                                    "flops_per_box": 500 * 4 * (S - 1) * n}
+        del lA_c, lbias_c
         viol = int((lb32 > fused).sum())
         loose = ((fused - lb32) / (fused.abs() + 1.0))
         result["plain_fp64_gpu"] = {"value": round(cL.shape[0] / tp, 1),

@@ -43,6 +43,9 @@ class BaBConfig:
     time_limit: float = 60.0
     alm_interval: int = 0          # 0: ALM only at the root
     sort_interval: int = 1         # sort the frontier every k iterations (0: never)
+    # LP dual certificate on the children's CROWN planes (FP64 bounds only):
+    # PDHG iterations, 0 to skip (bounding.py:489-520, dual_lp_pdhg_iters)
+    dual_lp_iters: int = 0
     alm: ALMConfig = field(default_factory=ALMConfig)
 
 
@@ -136,11 +139,29 @@ class QCQPBranchAndBound:
         pL, pU, _ = f.pick_out(nb)
         idx = input_split_branching(self.net, None, pL, pU, None, None, "naive")
         cL, cU, _ = input_split_parallel(pL, pU, i_idx=idx)
-        lb = self.net.bound(cL, cU)
+        if self.cfg.dual_lp_iters > 0 and self.net.precision == "fp64" and \
+                self.P < float("inf"):
+            lb, lA, lbias = self.net.bound(cL, cU, return_A=True)
+            self.raise_by_lp(lb, lA, lbias, cL, cU)
+        else:
+            lb = self.net.bound(cL, cU)
         f.filter_append(lb, self.thr, cL, cU, sync=sync)
         self.stats.iterations += 1
         self.stats.boxes_bounded += 2 * nb
         return 2 * nb
+
+    def raise_by_lp(self, lb, lA, lbias, cL, cU):
+        """Solving-mode merge of the LP certificate into the objective row
+        (bounding.py:698-723 _merge_dual_ascent_bound): a child whose
+        certified LP value reaches P - near_tol takes it as its bound, so
+        the prune drops it; other children keep their CROWN bound."""
+        from .lp import qcqp_lp_bound
+        cfg = self.cfg
+        lp = qcqp_lp_bound(lA, lbias, self.thr, cL, cU, iters=cfg.dual_lp_iters)
+        near = 0.5 * max(cfg.abs_tol, cfg.rel_tol * min(max(abs(self.P), 1.0), 1e300))
+        hit = lp >= (self.P - near)
+        lb[:, 0] = torch.where(hit, torch.maximum(lb[:, 0], lp), lb[:, 0])
+        return lb
 
     def solve(self):
         cfg = self.cfg

@@ -145,6 +145,28 @@ int gh_alm_pgd_f64(int N, int n, int S, const void *plan, const double *Cm, cons
                    hipStream_t stream);
 
 /*
+ * FP64 LP dual certificate per box: a rigorous lower bound of
+ *   min c^T x + c0  s.t.  G x <= h,  xL <= x <= xU
+ * with c, c0 the objective row's CROWN plane (lA[b, 0, :], lbias[b, 0]) and
+ * G, h the constraint rows' planes against their thresholds
+ * (G = lA[b, 1:, :], h = thr[1:] - lbias[b, 1:]), as bounding.py forms it
+ * (complete_verifier/input_split/bounding.py:489-520) for lp_bound
+ * (complete_verifier/nlp_solver/dual_ascent.py:459-561).  Multipliers
+ * y >= 0 come from `iters` iterations of diagonally preconditioned PDHG
+ * (dual_ascent.py:312-432); the bound is the weak-duality value at the best
+ * checkpoint, evaluated with the error model of rigorous_dual.py:1-54 from
+ * FP64 operations only, so it is a certificate whatever the iteration
+ * count.  Rows with a non-finite threshold or plane are inactive (y = 0).
+ *   lA (B, S, n), lbias (B, S) from gh_qcqp_crown_f64; thr (S); xL, xU (B, n);
+ *   lb (B): the certified bound (-inf if none); y (B, S - 1): its
+ *   multipliers (may be NULL).
+ * Supported: n <= 512, S - 1 <= 512.
+ */
+int gh_lp_dual_bound_f64(int B, int n, int S, const double *lA, const double *lbias,
+                         const double *thr, const double *xL, const double *xU, int iters,
+                         double *lb, double *y, hipStream_t stream);
+
+/*
  * Widest-variable branching: idx[b] = argmax_i (xU[b,i] - xL[b,i]), the
  * lowest index on ties.  Replaces input_split_branching(...,
  * branching_method='naive') (complete_verifier/input_split/

@@ -162,3 +162,43 @@ def math_fn(name, x):
     h.gho_math.restype = ctypes.c_int
     h.gho_math(code, x.size, x, y)
     return y
+
+
+# ------------------------------------------------------------- LP certificate
+U64 = 2.0 ** -53
+ETA64 = 2.0 ** -1074
+
+
+def _gamma(k):
+    return k * U64 / (1.0 - k * U64)
+
+
+def lp_dual_value(c, c0, G, h, l, u, y):
+    """Rigorous lower bound of the weak-duality value
+        D(y) = c0 - h^T y + sum_j min(r_j l_j, r_j u_j),  r = c + G^T y,
+    of  min c^T x + c0  s.t.  G x <= h,  l <= x <= u  (finite boxes), from
+    FP64 operations: the single-pass error bound of
+    complete_verifier/nlp_solver/rigorous_dual.py:1-54, 118-168 (Higham,
+    Lemma 3.1).  c, l, u (n,), G (m, n), h, y (m,); returns lb <= D(y)."""
+    c, G, h, l, u = (np.asarray(a, dtype=np.float64) for a in (c, G, h, l, u))
+    y = np.maximum(np.asarray(y, dtype=np.float64), 0.0)
+    m, n = G.shape
+    infl = 1.0 + _gamma(2 * (m + n) + 16)
+    act = (y[:, None] != 0) & (G != 0)
+    P = np.where(act, G * y[:, None], 0.0)
+    r = P.sum(0) + c
+    rho = np.abs(P).sum(0) + np.abs(c)
+    k = act.sum(0).astype(np.float64)
+    kk = np.where(k > 0, _gamma(k + 1.0), 0.0)
+    eps = kk * rho + 4 * ETA64 * (k + 2)
+    t = np.minimum(r * l, r * u)
+    M = np.maximum(np.abs(l), np.abs(u))
+    hy = np.where(y > 0, h * y, 0.0)
+    val = c0 - hy.sum() + t.sum()
+    N = 1.0 + (y > 0).sum() + (t != 0).sum()
+    E = _gamma(N + 2) * (abs(c0) + np.abs(hy).sum() + np.abs(t).sum()) + (M * eps).sum() \
+        + (m + n + 3) * ETA64
+    E = E * infl
+    E = E + 4 * U64 * (abs(val) + E)
+    lb = val - E
+    return lb if np.isfinite(lb) else -np.inf
