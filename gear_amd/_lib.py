@@ -27,7 +27,7 @@ SIGNATURES = {
     "gh_qcqp_crown_f64": [_I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P],
     "gh_qcqp_pack_f32_bytes": [_I, _I],
     "gh_qcqp_pack_f32": [_I, _I, _P, _P, _P],
-    "gh_qcqp_crown_f32s": [_I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P],
+    "gh_qcqp_crown_f32s": [_I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P],
     "gh_branch_widest_f64": [_I, _I, _P, _P, _P, _P],
     "gh_split_f64": [_I, _I, _P, _P, _P, _P, _P, _P],
     "gh_filter_workspace_bytes": [_I],

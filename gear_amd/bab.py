@@ -139,8 +139,7 @@ class QCQPBranchAndBound:
         pL, pU, _ = f.pick_out(nb)
         idx = input_split_branching(self.net, None, pL, pU, None, None, "naive")
         cL, cU, _ = input_split_parallel(pL, pU, i_idx=idx)
-        if self.cfg.dual_lp_iters > 0 and self.net.precision == "fp64" and \
-                self.P < float("inf"):
+        if self.cfg.dual_lp_iters > 0 and not self.net.large and self.P < float("inf"):
             lb, lA, lbias = self.net.bound(cL, cU, return_A=True)
             self.raise_by_lp(lb, lA, lbias, cL, cU)
         else:

@@ -87,9 +87,11 @@ class BoundedQCQP:
         self.last_flagged = 0
 
     def bound(self, x_L, x_U, return_A=False, out=None, stream=None):
-        """lb (B, S) [, lA (B, S, n), lbias (B, S)] for boxes x_L, x_U (B, n)."""
-        if self.precision == "fp32s" and not return_A:
-            return self.bound_fp32s(x_L, x_U, out=out, stream=stream)
+        """lb (B, S) [, lA (B, S, n), lbias (B, S)] for boxes x_L, x_U (B, n).
+        In the FP32-sound form the planes are lA x + lbias <= f_s(x) on the
+        box with the planes' rigorous error folded into lbias (n <= 64)."""
+        if self.precision == "fp32s":
+            return self.bound_fp32s(x_L, x_U, out=out, stream=stream, return_A=return_A)
         x_L = x_L.contiguous()
         x_U = x_U.contiguous()
         if x_L.dtype != torch.float64 or x_U.dtype != torch.float64:
@@ -109,7 +111,7 @@ class BoundedQCQP:
                                       stream_ptr(stream)), "gh_qcqp_crown_f64")
         return (lb, lA, lbias) if return_A else lb
 
-    def bound_fp32s(self, x_L, x_U, out=None, stream=None):
+    def bound_fp32s(self, x_L, x_U, out=None, stream=None, return_A=False):
         """FP32-sound lower bounds (B, S) as float64 values, each <= the FP64
         bound of the same box.  `last_flag` marks boxes where a near-tie
         McCormick choice was settled in FP64 (diagnostic)."""
@@ -134,13 +136,19 @@ class BoundedQCQP:
                                                  stream_ptr(stream)), "gh_qcqp_crown_f32s_large")
             self.last_flag = flag
             return lb
+        if return_A and self.large:
+            raise _lib.GearHipError("the large-n FP32-sound path has no plane output")
         if self.packed32 is None:
             raise _lib.GearHipError("constructed without precision='fp32s'")
+        lA = lbias = None
+        if return_A:
+            lA = torch.empty(B, self.S, self.n, dtype=torch.float64, device=self.device)
+            lbias = torch.empty(B, self.S, dtype=torch.float64, device=self.device)
         check(lib().gh_qcqp_crown_f32s(B, self.n, self.S, ptr(self.packed32), ptr(d.Q), ptr(d.c),
-                                       ptr(d.d), ptr(x_L), ptr(x_U), ptr(lb), ptr(flag),
-                                       stream_ptr(stream)), "gh_qcqp_crown_f32s")
+                                       ptr(d.d), ptr(x_L), ptr(x_U), ptr(lb), ptr(flag), ptr(lA),
+                                       ptr(lbias), stream_ptr(stream)), "gh_qcqp_crown_f32s")
         self.last_flag = flag
-        return lb
+        return (lb, lA, lbias) if return_A else lb
 
     def compute_bounds(self, x=None, C=None, method="CROWN", bound_lower=True,
                        bound_upper=False, return_A=False, needed_A_dict=None, **kwargs):

@@ -81,6 +81,9 @@ class ServeConfig:
     alm_interval_noinc: int = 5
     alm_boxes: int = 8
     alm_box_restarts: int = 4
+    # FP64 LP dual certificate on the children's FP32-sound planes, merged in
+    # solving mode (bounding.py:489-520, 698-723); PDHG iterations, 0 to skip
+    dual_lp_iters: int = 300
 
 
 def batch_size(n, S, cfg):
@@ -138,9 +141,11 @@ class Session:
         cfg, p = self.cfg, self.p
         self.t_admit = now
         with torch.cuda.stream(self.stream):
+            self.planes = False
             if self.bound_fn is None:
                 net = BoundedQCQP(p, self.device, precision=cfg.precision)
                 self.bound, self.dev = net.bound, net.dev
+                self.planes = cfg.dual_lp_iters > 0 and not net.large
             else:
                 self.dev = p.to_device(self.device)
                 self.bound = self.bound_fn(self.dev)
@@ -185,6 +190,16 @@ class Session:
                 self.P, self.x_best = obj, res.x
                 self.thr[0] = obj
 
+    def _raise_by_lp(self, lb, lA, lbias, cL, cU):
+        """Solving-mode merge of the LP certificate (bounding.py:698-723):
+        children whose certified LP value reaches P - near_tol take it."""
+        from .lp import qcqp_lp_bound
+        cfg = self.cfg
+        lp = qcqp_lp_bound(lA, lbias, self.thr, cL, cU, iters=cfg.dual_lp_iters)
+        near = 0.5 * max(cfg.abs_tol, cfg.rel_tol * min(max(abs(self.P), 1.0), 1e300))
+        hit = lp >= (self.P - near)
+        lb[:, 0] = torch.where(hit, torch.maximum(lb[:, 0], lp), lb[:, 0])
+
     def launch(self):
         """Enqueue the global bound of the current frontier, the frontier
         sort and one step."""
@@ -200,7 +215,11 @@ class Session:
                 pL, pU, _ = f.pick_out(nb)
                 idx = input_split_branching(None, None, pL, pU, None, None, "naive")
                 cL, cU, _ = input_split_parallel(pL, pU, i_idx=idx)
-                lb = self.bound(cL, cU)
+                if self.planes and self.P < math.inf:
+                    lb, lA, lbias = self.bound(cL, cU, return_A=True)
+                    self._raise_by_lp(lb, lA, lbias, cL, cU)
+                else:
+                    lb = self.bound(cL, cU)
                 f.filter_append(lb, self.thr, cL, cU, sync=False)
                 self._total.copy_(f.total)
                 self._host_total.copy_(self._total, non_blocking=True)

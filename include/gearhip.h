@@ -81,7 +81,10 @@ int gh_qcqp_crown_f64(int B, int n, int S, const void *packed,
  *   packed: gh_qcqp_pack_f32 of Q (gh_qcqp_pack_f32_bytes(n, S) bytes);
  *   Q (S, n, n) FP64 (for the tie cases), Cm, d, xL, xU as
  *   gh_qcqp_crown_f64 (FP64; boxes are rounded outward to FP32);
- *   lb (B, S) FP64 (FP32 values); flag (B) int32.
+ *   lb (B, S) FP64 (FP32 values); flag (B) int32;
+ *   lA (B, S, n), lbias (B, S) (either may be NULL): the lower planes,
+ *   lA x + lbias <= f_s(x) on the box with lbias lowered by the planes'
+ *   rigorous error (for gh_lp_dual_bound_f64).
  * Supported: 1 <= n <= 64.
  */
 size_t gh_qcqp_pack_f32_bytes(int n, int S);
@@ -90,7 +93,7 @@ int gh_qcqp_pack_f32(int n, int S, const double *Q, void *packed,
 int gh_qcqp_crown_f32s(int B, int n, int S, const void *packed,
                        const double *Q, const double *Cm, const double *d,
                        const double *xL, const double *xU, double *lb,
-                       int32_t *flag, hipStream_t stream);
+                       int32_t *flag, double *lA, double *lbias, hipStream_t stream);
 
 /*
  * FP32-sound CROWN lower bound of a QCQP of any size.  Same contract and

@@ -63,3 +63,26 @@ def test_fp32s_exact_ties(cuda):
     assert flag.any()
     assert np.all(got <= ref)
     assert np.all((ref - got) / (np.abs(ref) + 1) < 1e-3)
+
+
+def test_fp32s_planes_are_valid_lower_planes(cuda):
+    """The FP32-sound planes lA x + lbias (lbias lowered by the planes'
+    error) are below every row at sampled points and vertices of the box."""
+    from gear_amd.crown import BoundedQCQP
+    from gear_amd.problems import qcqp_dense, random_bisection_boxes
+    p = qcqp_dense(24, 6, seed=3)
+    L, U = random_bisection_boxes(p.xl, p.xu, 64, seed=9)
+    net = BoundedQCQP(p, cuda, precision="fp32s")
+    lb, lA, lbias = net.bound(torch.as_tensor(L, device=cuda), torch.as_tensor(U, device=cuda),
+                              return_A=True)
+    lA, lbias, lb = lA.cpu().numpy(), lbias.cpu().numpy(), lb.cpu().numpy()
+    rng = np.random.default_rng(1)
+    for b in range(len(L)):
+        X = L[b] + rng.random((500, p.n)) * (U[b] - L[b])
+        X = np.vstack([X, np.where(rng.random((100, p.n)) < 0.5, L[b], U[b])])
+        F = p.values(X)
+        P = X @ lA[b].T + lbias[b]
+        assert np.all(P <= F + 1e-12 * (1 + np.abs(F))), b
+        # the plane's box minimum is at least the concretized bound's value
+        pm = np.minimum(lA[b] * L[b], lA[b] * U[b]).sum(1) + lbias[b]
+        assert np.all(pm >= lb[b] - 1e-6 * (1 + np.abs(lb[b])))
