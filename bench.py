@@ -41,7 +41,8 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--batch", type=int, default=16384, help="parents per step")
+    ap.add_argument("--batch", type=int, default=16384,
+                    help="half the domains bounded per step (the reference's batch_size)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-alm", action="store_true")
@@ -305,14 +306,15 @@ def main():
     torch.cuda.set_device(dev)
     p = config2_problem()
     n, S = p.n, p.S
-    cap = 1 << 18
-    need = args.batch * (args.warmup + args.steps + 1)
+    cap = 1 << 20
+    # a reordered step bounds 2 batch domains and appends at most 4 batch children
+    need = 2 * args.batch * (args.warmup + args.steps + 1)
     if need > cap:
-        raise SystemExit(f"batch * (warmup + steps) = {need} exceeds the frontier cap {cap}")
+        raise SystemExit(f"2 batch * (warmup + steps + 1) = {need} exceeds the frontier cap {cap}")
     bb = QCQPBranchAndBound(p, BaBConfig(batch=args.batch, cap=cap), device=dev)
     t_alm = time.time()
-    # presplit start: `batch` leaves so every step runs a full batch
-    bb.presplit(args.batch, alm=not args.no_alm)
+    # presplit start: 2 batch leaves so every step bounds a full 2 batch
+    bb.presplit(2 * args.batch, alm=not args.no_alm)
     torch.cuda.synchronize()
     t_alm = time.time() - t_alm
     stream = torch.cuda.current_stream()
@@ -369,7 +371,7 @@ def main():
                               precision="fp32s")
     if bb.x_best is not None:
         bb32.set_incumbent(bb.P, bb.x_best)
-    bb32.presplit(args.batch, alm=False)
+    bb32.presplit(2 * args.batch, alm=False)
     for _ in range(args.warmup):
         bb32.step()
     torch.cuda.synchronize()
@@ -397,10 +399,11 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (seeded QCQP generator, default_rng(3))",
-        "config": {"workload": "qcqp_n50_m20 batched B&B: bound (CROWN) -> widest branch -> "
-                               "prune -> append, frontier cap 2^18",
-                   "n": n, "m": S - 1, "parents_per_step": args.batch,
-                   "children_per_step": 2 * args.batch},
+        "config": {"workload": "qcqp_n50_m20 batched B&B, reordered step: sort -> bound the "
+                               "picked domains (CROWN + planes) -> widest branch -> bisect -> "
+                               "child keys -> prune -> append, frontier cap 2^20",
+                   "n": n, "m": S - 1, "domains_bounded_per_step": 2 * args.batch,
+                   "children_per_step": 4 * args.batch},
         "roofline": {
             "bound": "mfma",
             "kernel": "qcqp_crown_f64_kernel",

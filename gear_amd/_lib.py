@@ -41,6 +41,7 @@ SIGNATURES = {
     "gh_alm_prepare": [_I, _I, _P, _P, _P],
     "gh_alm_pgd_f64": [_I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P],
     "gh_lp_dual_bound_f64": [_I, _I, _I, _P, _P, _P, _P, _P, _I, _P, _P, _P],
+    "gh_child_keys_f64": [_I, _I, _I, _P, _P, _P, _P, _P, _P, _P],
     "gh_dag_plan_create": [_I, _P, ctypes.POINTER(_P)],
     "gh_dag_plan_destroy": [_P],
     "gh_dag_plan_info": [_P, _P, _P, _P],

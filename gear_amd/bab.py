@@ -1,14 +1,17 @@
 """Batched branch and bound over input boxes on one device.
 
-One iteration follows the reference's regular order
-(complete_verifier/input_split/batch_branch_and_bound.py:275-652, 2660-2663):
-sort the frontier (least verified last) -> pick out the last `batch` domains
--> branch on the widest variable -> bisect
--> bound the children (CROWN) -> prune against the thresholds -> append the
-survivors; then the global lower bound is the frontier minimum capped at the
-discarded-threshold minimum (:620-634), and the gap test is
-is_primal_dual_converged with rel_tol (complete_verifier/solving_utils.py:
-17-49).  ALM-PGD refreshes the incumbent every `alm_interval` iterations
+One iteration follows the reference's reordered order, the one its GEAR
+presets select (reorder_bab: true, compare_with_old_bounds: true;
+complete_verifier/input_split/batch_branch_and_bound.py:275-652): sort the
+frontier (least verified last) -> pick out the last 2 `batch` domains ->
+bound them (CROWN, never below the key they were stored with) -> branch on
+the widest variable -> bisect -> key each child by its parent's bound and
+the parent's objective plane over the child box (:189-225) -> prune against
+the thresholds -> append the survivors.  The regular order (bisect first,
+then bound the children) is kept behind BaBConfig.reorder = False.  Then
+the global lower bound is the frontier minimum capped at the discarded-
+threshold minimum (:620-634), and the gap test is is_primal_dual_converged
+with rel_tol (complete_verifier/solving_utils.py:17-49).  ALM-PGD refreshes the incumbent every `alm_interval` iterations
 (nlp_in_bab.py:1078-1231).
 """
 import time
@@ -17,7 +20,8 @@ from dataclasses import dataclass, field
 import torch
 
 from .alm import ALMConfig, ALMPGDSolver, replay_float64
-from .crown import BoundedQCQP, DeviceFrontier, input_split_branching, input_split_parallel
+from .crown import (BoundedQCQP, DeviceFrontier, child_keys, input_split_branching,
+                    input_split_parallel)
 
 
 def is_primal_dual_converged(P, D, abs_tol=1e-5, rel_tol=0.0):
@@ -34,7 +38,7 @@ def is_primal_dual_converged(P, D, abs_tol=1e-5, rel_tol=0.0):
 
 @dataclass
 class BaBConfig:
-    batch: int = 16384             # parents per iteration (2x children bounded)
+    batch: int = 16384             # 2 batch domains bounded per iteration
     cap: int = 1 << 18             # frontier capacity
     rel_tol: float = 1e-4
     abs_tol: float = 1e-5
@@ -43,9 +47,13 @@ class BaBConfig:
     time_limit: float = 60.0
     alm_interval: int = 0          # 0: ALM only at the root
     sort_interval: int = 1         # sort the frontier every k iterations (0: never)
-    # LP dual certificate on the children's CROWN planes (FP64 bounds only):
-    # PDHG iterations, 0 to skip (bounding.py:489-520, dual_lp_pdhg_iters)
+    # LP dual certificate on the bounded domains' CROWN planes: PDHG
+    # iterations, 0 to skip (bounding.py:489-520, dual_lp_pdhg_iters)
     dual_lp_iters: int = 0
+    # the reordered loop of the GEAR presets (reorder_bab: true): bound the
+    # picked domains, split them, give the children keys from the parents'
+    # planes; False: the regular loop (split, then bound the children)
+    reorder: bool = True
     alm: ALMConfig = field(default_factory=ALMConfig)
 
 
@@ -127,7 +135,14 @@ class QCQPBranchAndBound:
         return self.D
 
     def step(self, batch=None, sync=True):
-        """One iteration; returns the number of children bounded."""
+        """One iteration; returns the number of domains bounded."""
+        if self.cfg.reorder:
+            return self.step_reordered(batch, sync)
+        return self.step_regular(batch, sync)
+
+    def step_regular(self, batch=None, sync=True):
+        """Sort, pick `batch` parents, split them, bound the 2 batch
+        children, prune and append (batch_branch_and_bound.py:275-652)."""
         f = self.frontier
         si = self.cfg.sort_interval
         if si > 0 and self.stats.iterations % si == 0:
@@ -148,6 +163,42 @@ class QCQPBranchAndBound:
         self.stats.iterations += 1
         self.stats.boxes_bounded += 2 * nb
         return 2 * nb
+
+    def step_reordered(self, batch=None, sync=True):
+        """The reordered iteration (batch_branch_and_bound.py:500-560 with
+        reorder_bab): sort, pick `batch` domains, bound them (their bound is
+        the larger of the new CROWN bound and the key they were stored
+        with), [LP certificate], branch and split, give each child the key
+        of gh_child_keys_f64 (the parent's bound, raised on the objective row
+        by the parent's plane over the child box), prune and append the
+        children.  Children of a verified parent inherit its verified rows,
+        so the one prune of the children drops them.  Like the reference
+        (batch_branch_and_bound.py:1387-1389) it picks 2 `batch` domains, so
+        an iteration bounds as many domains as the regular one."""
+        f = self.frontier
+        si = self.cfg.sort_interval
+        if si > 0 and self.stats.iterations % si == 0:
+            f.sort(self.thr)
+        nb = min(2 * (batch or self.cfg.batch), len(f), f.cap - len(f))
+        if nb <= 0:
+            return 0
+        pL, pU, pkey = f.pick_out(nb)
+        large = getattr(self.net, "large", False)
+        if large:
+            lb = self.net.bound(pL, pU)
+            lA = lbias = None
+        else:
+            lb, lA, lbias = self.net.bound(pL, pU, return_A=True)
+        lb = torch.maximum(lb, pkey)
+        if lA is not None and self.cfg.dual_lp_iters > 0 and self.P < float("inf"):
+            self.raise_by_lp(lb, lA, lbias, pL, pU)
+        idx = input_split_branching(self.net, None, pL, pU, None, None, "naive")
+        cL, cU, _ = input_split_parallel(pL, pU, i_idx=idx)
+        keys = child_keys(lb, lA, lbias, cL, cU) if lA is not None else torch.cat([lb, lb])
+        f.filter_append(keys, self.thr, cL, cU, sync=sync)
+        self.stats.iterations += 1
+        self.stats.boxes_bounded += nb
+        return nb
 
     def raise_by_lp(self, lb, lA, lbias, cL, cU):
         """Solving-mode merge of the LP certificate into the objective row

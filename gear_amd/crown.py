@@ -195,6 +195,18 @@ def input_split_parallel(x_L, x_U, shape=None, split_depth=1, i_idx=None, split_
     return oL, oU, 1
 
 
+def child_keys(plb, lA, lbias, cL, cU, stream=None):
+    """Keys (2K, S) of the 2K children of K parents (gh_child_keys_f64,
+    _raise_child_keys of batch_branch_and_bound.py:211-225)."""
+    K, S = plb.shape
+    n = cL.shape[1]
+    keys = torch.empty(2 * K, S, dtype=torch.float64, device=plb.device)
+    check(lib().gh_child_keys_f64(K, n, S, ptr(plb.contiguous()), ptr(lA), ptr(lbias),
+                                  ptr(cL.contiguous()), ptr(cU.contiguous()), ptr(keys),
+                                  stream_ptr(stream)), "gh_child_keys_f64")
+    return keys
+
+
 class DeviceFrontier:
     """Device-resident domain storage with stack order (pop from the end),
     as StackTensorStorage (complete_verifier/tensor_storage.py:351-406).

@@ -279,3 +279,58 @@ extern "C" int gh_min_strided_f64(int64_t N, int64_t stride, const double* x, do
                        x, out);
     return status();
 }
+
+// ------------------------------------------------------------ child keys
+namespace {
+
+// Child k (of 2K, [upper halves; lower halves]) of parent k mod K: row 0 is
+// max(parent bound, the parent's objective plane minimised over the child
+// box and rounded down); rows >= 1 keep the parent's bound
+// (batch_branch_and_bound.py:189-225 _plane_minimum_down / _raise_child_keys).
+// The minimum sum_i (a_i c_i - |a_i| r_i) + b is lowered by
+// (n + 4) u (sum_i |a_i| (|c_i| + r_i) + |b|) and one step toward -inf.
+__global__ void __launch_bounds__(256)
+child_keys_kernel(int K, int n, int S, const double *__restrict__ plb,
+                  const double *__restrict__ lA, const double *__restrict__ lbias,
+                  const double *__restrict__ cL, const double *__restrict__ cU,
+                  double *__restrict__ keys)
+{
+    // one thread per child; the sums run in index order, the order the key
+    // is specified in, so keys (and the sort and prune that read them) are
+    // the same on every device and in the restatement
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= 2 * K) return;
+    const int p = k % K;
+    const double *a = lA + (size_t)p * S * n;  // objective row
+    const double *xl = cL + (size_t)k * n, *xu = cU + (size_t)k * n;
+    double pm = 0.0, mg = 0.0;
+    bool bad = false;
+    for (int i = 0; i < n; ++i) {
+        const double l = xl[i], u = xu[i];
+        const double c = (l + u) / 2, r = (u - l) / 2;
+        const double ai = a[i];
+        pm += ai * c - fabs(ai) * r;
+        mg += fabs(ai) * (fabs(c) + r);
+        bad |= !(l <= u);
+    }
+    for (int s = 0; s < S; ++s) keys[(size_t)k * S + s] = plb[(size_t)p * S + s];
+    const double b = lbias[(size_t)p * S];
+    const double u53 = 1.1102230246251565404236316680908203125e-16;
+    pm = pm + b;
+    const double margin = (n + 4) * u53 * (mg + fabs(b));
+    const double key = nextafter(pm - margin, -INFINITY);
+    if (!bad && isfinite(key)) keys[(size_t)k * S] = fmax(plb[(size_t)p * S], key);
+}
+
+}  // namespace
+
+extern "C" int gh_child_keys_f64(int K, int n, int S, const double *plb, const double *lA,
+                                 const double *lbias, const double *cL, const double *cU,
+                                 double *keys, hipStream_t stream)
+{
+    if (K == 0) return GH_OK;
+    if (K < 0 || n < 1 || S < 1 || !plb || !lA || !lbias || !cL || !cU || !keys) return GH_EINVAL;
+    hipLaunchKernelGGL(child_keys_kernel, dim3((2 * K + 255) / 256), dim3(256), 0, stream, K, n,
+                       S, plb, lA, lbias, cL, cU, keys);
+    return hipGetLastError() == hipSuccess ? GH_OK : GH_ELAUNCH;
+}

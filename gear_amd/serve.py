@@ -31,7 +31,8 @@ import torch
 
 from .alm import ALMConfig, ALMPGDSolver, replay_float64
 from .bab import is_primal_dual_converged
-from .crown import BoundedQCQP, DeviceFrontier, input_split_branching, input_split_parallel
+from .crown import (BoundedQCQP, DeviceFrontier, child_keys, input_split_branching,
+                    input_split_parallel)
 from .problems import flops_per_box, qcqp_dense
 
 
@@ -81,9 +82,13 @@ class ServeConfig:
     alm_interval_noinc: int = 5
     alm_boxes: int = 8
     alm_box_restarts: int = 4
-    # FP64 LP dual certificate on the children's FP32-sound planes, merged in
-    # solving mode (bounding.py:489-520, 698-723); PDHG iterations, 0 to skip
+    # FP64 LP dual certificate on the bounded domains' FP32-sound planes,
+    # merged in solving mode (bounding.py:489-520, 698-723); PDHG
+    # iterations, 0 to skip
     dual_lp_iters: int = 300
+    # the reordered step of the GEAR presets (reorder_bab: true): bound the
+    # picked domains and key their children by the parents' planes
+    reorder: bool = True
 
 
 def batch_size(n, S, cfg):
@@ -103,10 +108,12 @@ def frontier_cap(n, S, cfg):
 
 def footprint(n, S, cfg):
     """Device bytes a session holds: frontier, one step's children and
-    bound / filter workspaces, program data and plans."""
+    bound / filter workspaces and planes, program data and plans."""
     row = (2 * n + S) * 8
     nb = batch_size(n, S, cfg)
     child = 2 * nb * (row + 64) + 2 * nb * 8
+    if cfg.reorder:                    # 2 nb parents and their 4 nb children
+        child = 2 * child + 2 * nb * (S * n + S) * 8
     prog = S * n * n * 8 * 3 + S * (n + 2) * 8
     large = 0
     if n > 64:
@@ -145,7 +152,7 @@ class Session:
             if self.bound_fn is None:
                 net = BoundedQCQP(p, self.device, precision=cfg.precision)
                 self.bound, self.dev = net.bound, net.dev
-                self.planes = cfg.dual_lp_iters > 0 and not net.large
+                self.planes = not net.large
             else:
                 self.dev = p.to_device(self.device)
                 self.bound = self.bound_fn(self.dev)
@@ -176,6 +183,7 @@ class Session:
             self._pending = True
             self._have_glb = False
             self._nb = 0
+            self._bounded = 0
             self.event = torch.cuda.Event()
             self.event.record(self.stream)
         self.status = "active"
@@ -200,27 +208,63 @@ class Session:
         hit = lp >= (self.P - near)
         lb[:, 0] = torch.where(hit, torch.maximum(lb[:, 0], lp), lb[:, 0])
 
+    def _lp(self):
+        return self.planes and self.cfg.dual_lp_iters > 0 and self.P < math.inf
+
+    def _step_regular(self, f):
+        """Pick `batch` parents, bisect, bound the children [LP], prune and
+        append; returns the parents picked."""
+        nb = min(self.batch, len(f), f.cap - len(f))
+        if nb > 0:
+            pL, pU, _ = f.pick_out(nb)
+            idx = input_split_branching(None, None, pL, pU, None, None, "naive")
+            cL, cU, _ = input_split_parallel(pL, pU, i_idx=idx)
+            if self._lp():
+                lb, lA, lbias = self.bound(cL, cU, return_A=True)
+                self._raise_by_lp(lb, lA, lbias, cL, cU)
+            else:
+                lb = self.bound(cL, cU)
+            f.filter_append(lb, self.thr, cL, cU, sync=False)
+            self._bounded = 2 * nb
+        return nb
+
+    def _step_reordered(self, f):
+        """Pick 2 `batch` domains, bound them (never below their keys) [LP],
+        bisect, key the children by the parents' planes, prune and append;
+        returns the domains picked."""
+        nb = min(2 * self.batch, len(f), f.cap - len(f))
+        if nb > 0:
+            pL, pU, pkey = f.pick_out(nb)
+            if self.planes:
+                lb, lA, lbias = self.bound(pL, pU, return_A=True)
+            else:
+                lb, lA, lbias = self.bound(pL, pU), None, None
+            lb = torch.maximum(lb, pkey)
+            if self._lp():
+                self._raise_by_lp(lb, lA, lbias, pL, pU)
+            idx = input_split_branching(None, None, pL, pU, None, None, "naive")
+            cL, cU, _ = input_split_parallel(pL, pU, i_idx=idx)
+            keys = child_keys(lb, lA, lbias, cL, cU) if lA is not None else torch.cat([lb, lb])
+            f.filter_append(keys, self.thr, cL, cU, sync=False)
+            self._bounded = nb
+        return nb
+
     def launch(self):
         """Enqueue the global bound of the current frontier, the frontier
         sort and one step."""
         f = self.frontier
+        cfg = self.cfg
         with torch.cuda.stream(self.stream):
             f.global_lb(self._glb)
             f.sort(self.thr)
             self._host[0:1].copy_(self._glb, non_blocking=True)
             self._have_glb = True
-            nb = min(self.batch, len(f), f.cap - len(f))
+            if cfg.reorder:
+                nb = self._step_reordered(f)
+            else:
+                nb = self._step_regular(f)
             self._nb = nb
             if nb > 0:
-                pL, pU, _ = f.pick_out(nb)
-                idx = input_split_branching(None, None, pL, pU, None, None, "naive")
-                cL, cU, _ = input_split_parallel(pL, pU, i_idx=idx)
-                if self.planes and self.P < math.inf:
-                    lb, lA, lbias = self.bound(cL, cU, return_A=True)
-                    self._raise_by_lp(lb, lA, lbias, cL, cU)
-                else:
-                    lb = self.bound(cL, cU)
-                f.filter_append(lb, self.thr, cL, cU, sync=False)
                 self._total.copy_(f.total)
                 self._host_total.copy_(self._total, non_blocking=True)
                 self._pending = True
@@ -236,7 +280,7 @@ class Session:
             self._pending = False
             if self._nb:
                 self.steps += 1
-                self.boxes += 2 * self._nb
+                self.boxes += self._bounded
         disc = float(self._host[1])
         if self._have_glb:
             self.D = float(self._host[0])

@@ -189,6 +189,22 @@ int gh_split_f64(int B, int n, const double *xL, const double *xU,
                  const int32_t *idx, double *oL, double *oU,
                  hipStream_t stream);
 
+/*
+ * Child keys of a split (solving mode): for the 2K children of K parents
+ * (layout of gh_split_f64: child k belongs to parent k mod K),
+ * keys[k][0] = max(plb[p][0], the minimum of the parent's objective plane
+ * lA[p][0] x + lbias[p][0] over the child box, lowered by
+ * (n + 4) u (sum |a| (|c| + r) + |b|) and one step toward -inf), and
+ * keys[k][s] = plb[p][s] for s >= 1.  Replaces _raise_child_keys /
+ * _plane_minimum_down (complete_verifier/input_split/
+ * batch_branch_and_bound.py:189-225) of the reordered loop.
+ *   plb (K, S), lA (K, S, n), lbias (K, S): the parents' bounds and planes;
+ *   cL, cU (2K, n): children; keys (2K, S) out.
+ */
+int gh_child_keys_f64(int K, int n, int S, const double *plb, const double *lA,
+                      const double *lbias, const double *cL, const double *cU, double *keys,
+                      hipStream_t stream);
+
 /* Bytes of device workspace gh_filter_append_f64 needs for B boxes. */
 size_t gh_filter_workspace_bytes(int B);
 

@@ -202,3 +202,36 @@ def lp_dual_value(c, c0, G, h, l, u, y):
     E = E + 4 * U64 * (abs(val) + E)
     lb = val - E
     return lb if np.isfinite(lb) else -np.inf
+
+
+def plane_minimum_down(a, b, xl, xu):
+    """Minimum of the plane a x + b over the box, never above the exact value:
+    sum(a c - |a| r) + b lowered by (n + 4) u (sum |a| (|c| + r) + |b|) and
+    one step toward -inf (complete_verifier/input_split/
+    batch_branch_and_bound.py:189-208).  Both sums run in index order."""
+    one = np.ndim(xl) == 1
+    a, xl, xu = (np.atleast_2d(np.asarray(v, dtype=np.float64)) for v in (a, xl, xu))
+    c, r = (xl + xu) / 2, (xu - xl) / 2
+    pm = np.zeros(xl.shape[0])
+    mg = np.zeros(xl.shape[0])
+    for i in range(xl.shape[1]):
+        pm = pm + (a[:, i] * c[:, i] - np.abs(a[:, i]) * r[:, i])
+        mg = mg + np.abs(a[:, i]) * (np.abs(c[:, i]) + r[:, i])
+    pm = pm + b
+    margin = (xl.shape[1] + 4) * U64 * (mg + np.abs(b))
+    out = np.nextafter(pm - margin, -np.inf)
+    return out[0] if one else out
+
+
+def child_keys(plb, lA, lbias, cL, cU):
+    """Keys (2K, S) of the 2K children ([upper; lower] halves) of K parents
+    (batch_branch_and_bound.py:211-225): row 0 max(parent bound, the
+    parent's objective plane minimised over the child box), rows >= 1 the
+    parent's bound."""
+    K = plb.shape[0]
+    keys = np.vstack([plb, plb]).copy()
+    par = np.arange(2 * K) % K
+    pm = plane_minimum_down(lA[par, 0], lbias[par, 0], cL, cU)
+    ok = np.all(cL <= cU, axis=1) & np.isfinite(pm)
+    keys[ok, 0] = np.maximum(keys[ok, 0], pm[ok])
+    return keys

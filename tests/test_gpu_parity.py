@@ -121,12 +121,13 @@ def test_filter_append_matches_oracle(cuda, B):
 
 
 def test_bab_iterations_match_oracle_loop(cuda):
-    """Six B&B iterations on the device reproduce the oracle-driven loop
-    (sort by the largest margin lb - thr, descending and stable; pick from
-    the end, widest branch, bisect, bound, prune, append)."""
+    """Six iterations of the regular B&B order on the device reproduce the
+    oracle-driven loop (sort by the largest margin lb - thr, descending and
+    stable; pick from the end, widest branch, bisect, bound, prune,
+    append)."""
     from gear_amd.bab import BaBConfig, QCQPBranchAndBound
     p = qcqp_dense(10, 3, seed=11)
-    bb = QCQPBranchAndBound(p, BaBConfig(batch=64, cap=4096), device=cuda)
+    bb = QCQPBranchAndBound(p, BaBConfig(batch=64, cap=4096, reorder=False), device=cuda)
     bb.root(alm=False)
     bb.set_incumbent(-1.0, None)       # an injected incumbent prunes part of the tree
     # oracle mirror
@@ -152,3 +153,62 @@ def test_bab_iterations_match_oracle_loop(cuda):
         assert c == len(FL), it
         assert _bits_equal(bb.frontier.L[:c].cpu().numpy(), FL)
         assert _bits_equal(bb.frontier.lb[:c].cpu().numpy(), Flb)
+
+
+def test_child_keys_match_oracle(cuda):
+    """gh_child_keys_f64 equals the oracle's child keys bit for bit on the
+    planes of real parents, including invalid child boxes and an infinite
+    plane (which keep the parent's bound)."""
+    from gear_amd.crown import child_keys
+    p = qcqp_dense(12, 4, seed=5)
+    L, U = random_bisection_boxes(p.xl, p.xu, 200, seed=6)
+    plb, lA, lbias = O.qcqp_crown(p.Q, p.c, p.d, L, U, return_A=True)
+    lbias[7, 0] = np.inf
+    cL, cU = O.split(L, U, O.branch_widest(L, U))
+    cL[3, 1], cU[3, 1] = 1.0, -1.0
+    ref = O.child_keys(plb, lA, lbias, cL, cU)
+    got = child_keys(_dev(plb, cuda), _dev(lA, cuda), _dev(lbias, cuda),
+                     _dev(cL, cuda), _dev(cU, cuda)).cpu().numpy()
+    assert _bits_equal(got, ref)
+    assert (got[:, 0] >= np.concatenate([plb[:, 0], plb[:, 0]])).all()
+    assert (got[:, 0] > np.concatenate([plb[:, 0], plb[:, 0]])).any()
+
+
+def test_reordered_bab_iterations_match_oracle_loop(cuda):
+    """Six iterations of the reordered B&B order (the default) on the device
+    reproduce the oracle-driven loop: sort, pick 2 batch domains from the
+    end, bound them and keep the larger of that bound and the stored key,
+    widest branch, bisect, child keys from the parents' planes, prune,
+    append."""
+    from gear_amd.bab import BaBConfig, QCQPBranchAndBound
+    p = qcqp_dense(10, 3, seed=11)
+    bb = QCQPBranchAndBound(p, BaBConfig(batch=32, cap=4096), device=cuda)
+    assert bb.cfg.reorder
+    bb.root(alm=False)
+    bb.set_incumbent(-1.0, None)
+    FL, FU = p.xl[None].copy(), p.xu[None].copy()
+    Flb = O.qcqp_crown(p.Q, p.c, p.d, FL, FU)
+    thr = np.concatenate([[-1.0], p.rhs[1:] + 1e-6])
+    raised = 0
+    for it in range(6):
+        keys = (Flb - thr).max(axis=1)
+        order = np.argsort(-keys, kind="stable")
+        FL, FU, Flb = FL[order], FU[order], Flb[order]
+        nb = min(64, len(FL))
+        pL, pU, pk = FL[-nb:], FU[-nb:], Flb[-nb:]
+        FL, FU, Flb = FL[:-nb], FU[:-nb], Flb[:-nb]
+        lb, lA, lbias = O.qcqp_crown(p.Q, p.c, p.d, pL, pU, return_A=True)
+        lb = np.maximum(lb, pk)
+        cL, cU = O.split(pL, pU, O.branch_widest(pL, pU))
+        ck = O.child_keys(lb, lA, lbias, cL, cU)
+        raised += int((ck[:, 0] > np.concatenate([lb[:, 0], lb[:, 0]])).sum())
+        keep, _, _ = O.prune(ck, thr, cL, cU)
+        FL = np.vstack([FL, cL[keep]])
+        FU = np.vstack([FU, cU[keep]])
+        Flb = np.vstack([Flb, ck[keep]])
+        assert bb.step() == nb
+        c = len(bb.frontier)
+        assert c == len(FL), it
+        assert _bits_equal(bb.frontier.L[:c].cpu().numpy(), FL)
+        assert _bits_equal(bb.frontier.lb[:c].cpu().numpy(), Flb)
+    assert raised > 0
