@@ -24,10 +24,10 @@ SIGNATURES = {
     "gh_version": [],
     "gh_qcqp_pack_bytes": [_I, _I],
     "gh_qcqp_pack_f64": [_I, _I, _P, _P, _P],
-    "gh_qcqp_crown_f64": [_I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P],
+    "gh_qcqp_crown_f64": [_I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _I, _P],
     "gh_qcqp_pack_f32_bytes": [_I, _I],
     "gh_qcqp_pack_f32": [_I, _I, _P, _P, _P],
-    "gh_qcqp_crown_f32s": [_I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P],
+    "gh_qcqp_crown_f32s": [_I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I, _P],
     "gh_branch_widest_f64": [_I, _I, _P, _P, _P, _P],
     "gh_split_f64": [_I, _I, _P, _P, _P, _P, _P, _P],
     "gh_filter_workspace_bytes": [_I],
@@ -41,7 +41,8 @@ SIGNATURES = {
     "gh_alm_prepare": [_I, _I, _P, _P, _P],
     "gh_alm_pgd_f64": [_I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P],
     "gh_lp_dual_bound_f64": [_I, _I, _I, _P, _P, _P, _P, _P, _I, _P, _P, _P],
-    "gh_child_keys_f64": [_I, _I, _I, _P, _P, _P, _P, _P, _P, _P],
+    "gh_child_keys_f64": [_I, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P],
+    "gh_constrained_concretize_f64": [_I, _I, _I, _I, _P, _P, _P, _P, _P, _I, _P, _P, _P, _P],
     "gh_dag_plan_create": [_I, _P, ctypes.POINTER(_P)],
     "gh_dag_plan_destroy": [_P],
     "gh_dag_plan_info": [_P, _P, _P, _P],

@@ -184,13 +184,16 @@ class QCQPBranchAndBound:
             return 0
         pL, pU, pkey = f.pick_out(nb)
         large = getattr(self.net, "large", False)
+        lp = self.cfg.dual_lp_iters > 0 and self.P < float("inf")
         if large:
             lb = self.net.bound(pL, pU)
             lA = lbias = None
         else:
-            lb, lA, lbias = self.net.bound(pL, pU, return_A=True)
+            # the LP reads every row's plane, the child keys the objective's
+            lb, lA, lbias = self.net.bound(pL, pU, return_A=True,
+                                           plane_rows=self.p.S if lp else 1)
         lb = torch.maximum(lb, pkey)
-        if lA is not None and self.cfg.dual_lp_iters > 0 and self.P < float("inf"):
+        if lA is not None and lp:
             self.raise_by_lp(lb, lA, lbias, pL, pU)
         idx = input_split_branching(self.net, None, pL, pU, None, None, "naive")
         cL, cU, _ = input_split_parallel(pL, pU, i_idx=idx)

@@ -1,0 +1,41 @@
+"""Constrained concretization on the device (gh_constrained_concretize_f64):
+lower bounds of min obj[b, h] . x over a box intersected with linear cuts
+A[b, k] . x + bc[b, k] <= 0 -- constraints_solving after
+sort_out_constr_batches and drop_redundant_constraints
+(auto_LiRPA/concretize_func.py:269-850), the concretization of
+clip_type 'complete' (complete_verifier/input_split/batch_branch_and_bound.py:
+420-428)."""
+import torch
+
+from ._lib import check, lib, ptr, stream_ptr
+
+MAX_X = 64
+
+
+def constrained_concretize(obj, A, bc, x_L, x_U, max_rows=16, stream=None):
+    """(root (B, H), flags (B) int32, kept (B) int32) for obj (B, H, X),
+    cuts A (B, K, X), bc (B, K), boxes x_L, x_U (B, X), all float64 on one
+    device.  flags bit 0: a cut misses the box (infeasible domain); bit 1:
+    every cut holds on the whole box."""
+    B, H, X = obj.shape
+    K = A.shape[1] if A is not None else 0
+    if X > MAX_X:
+        raise ValueError(f"constrained concretization supports X <= {MAX_X}")
+    dev = obj.device
+    obj = obj.contiguous()
+    x_L = x_L.contiguous()
+    x_U = x_U.contiguous()
+    if K:
+        A = A.contiguous()
+        bc = bc.contiguous()
+        if A.shape != (B, K, X) or bc.shape != (B, K):
+            raise ValueError("cuts must be (B, K, X) and (B, K)")
+    root = torch.empty(B, H, dtype=torch.float64, device=dev)
+    flags = torch.empty(B, dtype=torch.int32, device=dev)
+    kept = torch.empty(B, dtype=torch.int32, device=dev)
+    check(lib().gh_constrained_concretize_f64(B, H, K, X, ptr(obj), ptr(A if K else None),
+                                              ptr(bc if K else None), ptr(x_L), ptr(x_U),
+                                              int(max_rows), ptr(root), ptr(flags), ptr(kept),
+                                              stream_ptr(stream)),
+          "gh_constrained_concretize_f64")
+    return root, flags, kept

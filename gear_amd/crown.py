@@ -86,12 +86,14 @@ class BoundedQCQP:
                                          stream_ptr()), "gh_qcqp_pack_f32")
         self.last_flagged = 0
 
-    def bound(self, x_L, x_U, return_A=False, out=None, stream=None):
-        """lb (B, S) [, lA (B, S, n), lbias (B, S)] for boxes x_L, x_U (B, n).
-        In the FP32-sound form the planes are lA x + lbias <= f_s(x) on the
-        box with the planes' rigorous error folded into lbias (n <= 64)."""
+    def bound(self, x_L, x_U, return_A=False, out=None, stream=None, plane_rows=None):
+        """lb (B, S) [, lA (B, R, n), lbias (B, R)] for boxes x_L, x_U (B, n),
+        with the planes of the leading R = plane_rows rows (default S).  In
+        the FP32-sound form the planes are lA x + lbias <= f_s(x) on the box
+        with the planes' rigorous error folded into lbias (n <= 64)."""
         if self.precision == "fp32s":
-            return self.bound_fp32s(x_L, x_U, out=out, stream=stream, return_A=return_A)
+            return self.bound_fp32s(x_L, x_U, out=out, stream=stream, return_A=return_A,
+                                    plane_rows=plane_rows)
         x_L = x_L.contiguous()
         x_U = x_U.contiguous()
         if x_L.dtype != torch.float64 or x_U.dtype != torch.float64:
@@ -102,16 +104,17 @@ class BoundedQCQP:
         lb = out if out is not None else torch.empty(B, self.S, dtype=torch.float64,
                                                      device=self.device)
         lA = lbias = None
+        R = self.S if plane_rows is None else int(plane_rows)
         if return_A:
-            lA = torch.empty(B, self.S, self.n, dtype=torch.float64, device=self.device)
-            lbias = torch.empty(B, self.S, dtype=torch.float64, device=self.device)
+            lA = torch.empty(B, R, self.n, dtype=torch.float64, device=self.device)
+            lbias = torch.empty(B, R, dtype=torch.float64, device=self.device)
         d = self.dev
         check(lib().gh_qcqp_crown_f64(B, self.n, self.S, ptr(self.packed), ptr(d.c), ptr(d.d),
-                                      ptr(x_L), ptr(x_U), ptr(lb), ptr(lA), ptr(lbias),
+                                      ptr(x_L), ptr(x_U), ptr(lb), ptr(lA), ptr(lbias), R,
                                       stream_ptr(stream)), "gh_qcqp_crown_f64")
         return (lb, lA, lbias) if return_A else lb
 
-    def bound_fp32s(self, x_L, x_U, out=None, stream=None, return_A=False):
+    def bound_fp32s(self, x_L, x_U, out=None, stream=None, return_A=False, plane_rows=None):
         """FP32-sound lower bounds (B, S) as float64 values, each <= the FP64
         bound of the same box.  `last_flag` marks boxes where a near-tie
         McCormick choice was settled in FP64 (diagnostic)."""
@@ -141,12 +144,13 @@ class BoundedQCQP:
         if self.packed32 is None:
             raise _lib.GearHipError("constructed without precision='fp32s'")
         lA = lbias = None
+        R = self.S if plane_rows is None else int(plane_rows)
         if return_A:
-            lA = torch.empty(B, self.S, self.n, dtype=torch.float64, device=self.device)
-            lbias = torch.empty(B, self.S, dtype=torch.float64, device=self.device)
+            lA = torch.empty(B, R, self.n, dtype=torch.float64, device=self.device)
+            lbias = torch.empty(B, R, dtype=torch.float64, device=self.device)
         check(lib().gh_qcqp_crown_f32s(B, self.n, self.S, ptr(self.packed32), ptr(d.Q), ptr(d.c),
                                        ptr(d.d), ptr(x_L), ptr(x_U), ptr(lb), ptr(flag), ptr(lA),
-                                       ptr(lbias), stream_ptr(stream)), "gh_qcqp_crown_f32s")
+                                       ptr(lbias), R, stream_ptr(stream)), "gh_qcqp_crown_f32s")
         self.last_flag = flag
         return (lb, lA, lbias) if return_A else lb
 
@@ -196,12 +200,18 @@ def input_split_parallel(x_L, x_U, shape=None, split_depth=1, i_idx=None, split_
 
 
 def child_keys(plb, lA, lbias, cL, cU, stream=None):
-    """Keys (2K, S) of the 2K children of K parents (gh_child_keys_f64,
-    _raise_child_keys of batch_branch_and_bound.py:211-225)."""
+    """Keys (2K, S) of the 2K children of K parents from the parents'
+    bounds plb (K, S) and planes lA (K, R, n), lbias (K, R) of R >= 1 rows
+    (gh_child_keys_f64, _raise_child_keys of batch_branch_and_bound.py:
+    211-225)."""
     K, S = plb.shape
     n = cL.shape[1]
+    R = lA.shape[1]
+    if lA.shape != (K, R, n) or lbias.shape != (K, R) or not lA.is_contiguous() \
+            or not lbias.is_contiguous():
+        raise ValueError("planes must be contiguous (K, R, n) and (K, R)")
     keys = torch.empty(2 * K, S, dtype=torch.float64, device=plb.device)
-    check(lib().gh_child_keys_f64(K, n, S, ptr(plb.contiguous()), ptr(lA), ptr(lbias),
+    check(lib().gh_child_keys_f64(K, n, S, R, ptr(plb.contiguous()), ptr(lA), ptr(lbias),
                                   ptr(cL.contiguous()), ptr(cU.contiguous()), ptr(keys),
                                   stream_ptr(stream)), "gh_child_keys_f64")
     return keys

@@ -290,7 +290,7 @@ namespace {
 // The minimum sum_i (a_i c_i - |a_i| r_i) + b is lowered by
 // (n + 4) u (sum_i |a_i| (|c_i| + r_i) + |b|) and one step toward -inf.
 __global__ void __launch_bounds__(256)
-child_keys_kernel(int K, int n, int S, const double *__restrict__ plb,
+child_keys_kernel(int K, int n, int S, int R, const double *__restrict__ plb,
                   const double *__restrict__ lA, const double *__restrict__ lbias,
                   const double *__restrict__ cL, const double *__restrict__ cU,
                   double *__restrict__ keys)
@@ -301,7 +301,7 @@ child_keys_kernel(int K, int n, int S, const double *__restrict__ plb,
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= 2 * K) return;
     const int p = k % K;
-    const double *a = lA + (size_t)p * S * n;  // objective row
+    const double *a = lA + (size_t)p * R * n;  // objective row
     const double *xl = cL + (size_t)k * n, *xu = cU + (size_t)k * n;
     double pm = 0.0, mg = 0.0;
     bool bad = false;
@@ -314,7 +314,7 @@ child_keys_kernel(int K, int n, int S, const double *__restrict__ plb,
         bad |= !(l <= u);
     }
     for (int s = 0; s < S; ++s) keys[(size_t)k * S + s] = plb[(size_t)p * S + s];
-    const double b = lbias[(size_t)p * S];
+    const double b = lbias[(size_t)p * R];
     const double u53 = 1.1102230246251565404236316680908203125e-16;
     pm = pm + b;
     const double margin = (n + 4) * u53 * (mg + fabs(b));
@@ -324,13 +324,14 @@ child_keys_kernel(int K, int n, int S, const double *__restrict__ plb,
 
 }  // namespace
 
-extern "C" int gh_child_keys_f64(int K, int n, int S, const double *plb, const double *lA,
-                                 const double *lbias, const double *cL, const double *cU,
-                                 double *keys, hipStream_t stream)
+extern "C" int gh_child_keys_f64(int K, int n, int S, int R, const double *plb,
+                                 const double *lA, const double *lbias, const double *cL,
+                                 const double *cU, double *keys, hipStream_t stream)
 {
     if (K == 0) return GH_OK;
-    if (K < 0 || n < 1 || S < 1 || !plb || !lA || !lbias || !cL || !cU || !keys) return GH_EINVAL;
+    if (K < 0 || n < 1 || S < 1 || R < 1 || R > S || !plb || !lA || !lbias || !cL || !cU || !keys)
+        return GH_EINVAL;
     hipLaunchKernelGGL(child_keys_kernel, dim3((2 * K + 255) / 256), dim3(256), 0, stream, K, n,
-                       S, plb, lA, lbias, cL, cU, keys);
+                       S, R, plb, lA, lbias, cL, cU, keys);
     return hipGetLastError() == hipSuccess ? GH_OK : GH_ELAUNCH;
 }

@@ -81,7 +81,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void qc
     int B, int n, int S, const double* __restrict__ QB, const double* __restrict__ QA,
     const double* __restrict__ Cm, const double* __restrict__ d, const double* __restrict__ xL,
     const double* __restrict__ xU, double* __restrict__ lb, double* __restrict__ lA,
-    double* __restrict__ lbias, QcqpConsts k)
+    double* __restrict__ lbias, int PR, QcqpConsts k)
 {
     constexpr int NCH = (KT + 3) / 4;    // 16-wide tiles of j (and of i)
     constexpr int NM = KT;               // per-lane indices q + 4m
@@ -246,7 +246,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void qc
                     p_base[v] = fma(a, x0, p_base[v]);
                     p_et[v] = fma(fabs(a), ep, p_et[v]);
                     p_ma[v] = fma(fabs(a), mx, p_ma[v]);
-                    if (lA && live) lA[((size_t)box * S + s) * n + i] = a;
+                    if (lA && live && s < PR) lA[((size_t)box * PR + s) * n + i] = a;
                 }
             }
         }
@@ -273,7 +273,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void qc
             double root = gh::widen(base - et, ma * k.kn, -1.0);
             if (!isfinite(root)) root = -INFINITY;
             lb[(size_t)box * S + s] = gh::add_enclosed(bias, root, -1.0, k.k1);
-            if (lbias) lbias[(size_t)box * S + s] = a;
+            if (lbias && s < PR) lbias[(size_t)box * PR + s] = a;
         }
     }
 }
@@ -281,11 +281,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void qc
 template <int KT>
 int launch_f64(int B, int n, int S, const double* QB, const double* QA, const double* Cm,
                const double* d, const double* xL, const double* xU, double* lb, double* lA,
-               double* lbias, const QcqpConsts& k, hipStream_t stream)
+               double* lbias, int PR, const QcqpConsts& k, hipStream_t stream)
 {
     dim3 grid((B + 15) / 16);
     hipLaunchKernelGGL(qcqp_crown_f64_kernel<KT>, grid, dim3(64), 0, stream, B, n, S, QB, QA,
-                       Cm, d, xL, xU, lb, lA, lbias, k);
+                       Cm, d, xL, xU, lb, lA, lbias, PR, k);
     return hipGetLastError() == hipSuccess ? GH_OK : GH_ELAUNCH;
 }
 
@@ -314,9 +314,10 @@ extern "C" int gh_qcqp_pack_f64(int n, int S, const double* Q, void* packed, hip
 
 extern "C" int gh_qcqp_crown_f64(int B, int n, int S, const void* packed, const double* Cm,
                                  const double* d, const double* xL, const double* xU, double* lb,
-                                 double* lA, double* lbias, hipStream_t stream)
+                                 double* lA, double* lbias, int plane_rows, hipStream_t stream)
 {
     if (B < 0 || n <= 0 || S <= 0) return GH_EINVAL;
+    if ((lA || lbias) && (plane_rows < 1 || plane_rows > S)) return GH_EINVAL;
     if (n > 64) return GH_EUNSUPPORTED;
     if (B == 0) return GH_OK;
     if (!packed || !Cm || !d || !xL || !xU || !lb) return GH_EINVAL;
@@ -332,7 +333,8 @@ extern "C" int gh_qcqp_crown_f64(int B, int n, int S, const void* packed, const 
     const double* QB = (const double*)packed;
     const double* QA = QB + (size_t)S * nch * kt * 64;
 #define GH_KT(K) \
-    case K: return launch_f64<K>(B, n, S, QB, QA, Cm, d, xL, xU, lb, lA, lbias, k, stream);
+    case K: return launch_f64<K>(B, n, S, QB, QA, Cm, d, xL, xU, lb, lA, lbias, plane_rows, k, \
+                                 stream);
     switch (kt) {
         GH_KT(1) GH_KT(2) GH_KT(3) GH_KT(4) GH_KT(5) GH_KT(6) GH_KT(7) GH_KT(8)
         GH_KT(9) GH_KT(10) GH_KT(11) GH_KT(12) GH_KT(13) GH_KT(14) GH_KT(15) GH_KT(16)

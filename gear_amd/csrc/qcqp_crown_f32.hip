@@ -112,7 +112,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void qc
     int B, int n, int S, const float* __restrict__ QB, const float* __restrict__ QA,
     const double* __restrict__ Q64, const double* __restrict__ Cm, const double* __restrict__ d,
     const double* __restrict__ xL, const double* __restrict__ xU, double* __restrict__ lb,
-    int* __restrict__ flag, double* __restrict__ lA, double* __restrict__ lbias, F32Consts k)
+    int* __restrict__ flag, double* __restrict__ lA, double* __restrict__ lbias, int PR,
+    F32Consts k)
 {
     constexpr int NCH = (KT + 3) / 4;
     constexpr int NM = KT;
@@ -264,7 +265,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void qc
                     ep = ep + (ep + fabsf(x0)) * (4.0f * k.u);
                     float mx = fmaxf(-lo, hi);
                     float a = acm[m] + Da[c][v];
-                    if (lA && live) lA[((size_t)box * S + s) * n + i] = (double)a;
+                    if (lA && live && s < PR) lA[((size_t)box * PR + s) * n + i] = (double)a;
                     p_base[v] = fmaf(a, x0, p_base[v]);
                     p_et[v] = fmaf(fabsf(a), ep, p_et[v]);
                     p_ma[v] = fmaf(fabsf(a), mx, p_ma[v]);
@@ -284,13 +285,13 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void qc
             float out = (bias + root) - ec;
             double o = isfinite(out) ? (double)out : -INFINITY;
             lb[(size_t)box * S + s] = o;
-            if (lbias) {
+            if (lbias && s < PR) {
                 // the plane a x + lbias is below f_s on the box: the bound's
                 // error without its concretization terms (kc ma, root)
                 float ep = (k.kb * own + 3.0f * k.u * m1s + k.g * gabs + k.u * fabsf(ds)
                             + 4.0f * k.u * fabsf(bias) + k.tiny) * 1.01f;
                 double pb = (double)bias - (double)ep;
-                lbias[(size_t)box * S + s] = isfinite(pb) ? pb : -INFINITY;
+                lbias[(size_t)box * PR + s] = isfinite(pb) ? pb : -INFINITY;
             }
         }
     }
@@ -303,11 +304,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void qc
 template <int KT>
 int launch_f32(int B, int n, int S, const float* QB, const float* QA, const double* Q64,
                const double* Cm, const double* d, const double* xL, const double* xU, double* lb,
-               int* flag, double* lA, double* lbias, const F32Consts& k, hipStream_t stream)
+               int* flag, double* lA, double* lbias, int PR, const F32Consts& k,
+               hipStream_t stream)
 {
     dim3 grid((B + 15) / 16);
     hipLaunchKernelGGL(qcqp_crown_f32s_kernel<KT>, grid, dim3(64), 0, stream, B, n, S, QB, QA, Q64,
-                       Cm, d, xL, xU, lb, flag, lA, lbias, k);
+                       Cm, d, xL, xU, lb, flag, lA, lbias, PR, k);
     return hipGetLastError() == hipSuccess ? GH_OK : GH_ELAUNCH;
 }
 
@@ -337,9 +339,10 @@ extern "C" int gh_qcqp_pack_f32(int n, int S, const double* Q, void* packed, hip
 extern "C" int gh_qcqp_crown_f32s(int B, int n, int S, const void* packed, const double* Q,
                                   const double* Cm, const double* d, const double* xL,
                                   const double* xU, double* lb, int32_t* flag, double* lA,
-                                  double* lbias, hipStream_t stream)
+                                  double* lbias, int plane_rows, hipStream_t stream)
 {
     if (B < 0 || n <= 0 || S <= 0) return GH_EINVAL;
+    if ((lA || lbias) && (plane_rows < 1 || plane_rows > S)) return GH_EINVAL;
     if (n > 64) return GH_EUNSUPPORTED;
     if (B == 0) return GH_OK;
     if (!packed || !Q || !Cm || !d || !xL || !xU || !lb || !flag) return GH_EINVAL;
@@ -362,7 +365,8 @@ extern "C" int gh_qcqp_crown_f32s(int B, int n, int S, const void* packed, const
     const float* QB = (const float*)packed;
     const float* QA = QB + (size_t)S * nch * kt * 64;
 #define GH_KT(K) \
-    case K: return launch_f32<K>(B, n, S, QB, QA, Q, Cm, d, xL, xU, lb, flag, lA, lbias, k, stream);
+    case K: return launch_f32<K>(B, n, S, QB, QA, Q, Cm, d, xL, xU, lb, flag, lA, lbias, \
+                                 plane_rows, k, stream);
     switch (kt) {
         GH_KT(1) GH_KT(2) GH_KT(3) GH_KT(4) GH_KT(5) GH_KT(6) GH_KT(7) GH_KT(8)
         GH_KT(9) GH_KT(10) GH_KT(11) GH_KT(12) GH_KT(13) GH_KT(14) GH_KT(15) GH_KT(16)

@@ -235,12 +235,14 @@ class Session:
         nb = min(2 * self.batch, len(f), f.cap - len(f))
         if nb > 0:
             pL, pU, pkey = f.pick_out(nb)
+            lp = self._lp()
             if self.planes:
-                lb, lA, lbias = self.bound(pL, pU, return_A=True)
+                lb, lA, lbias = self.bound(pL, pU, return_A=True,
+                                          plane_rows=self.p.S if lp else 1)
             else:
                 lb, lA, lbias = self.bound(pL, pU), None, None
             lb = torch.maximum(lb, pkey)
-            if self._lp():
+            if lp:
                 self._raise_by_lp(lb, lA, lbias, pL, pU)
             idx = input_split_branching(None, None, pL, pU, None, None, "naive")
             cL, cU, _ = input_split_parallel(pL, pU, i_idx=idx)

@@ -60,14 +60,16 @@ int gh_qcqp_pack_f64(int n, int S, const double *Q, void *packed,
  *   packed: gh_qcqp_pack_f64 of Q (S, n, n); Cm (S, n), d (S);
  *   xL, xU (B, n): boxes;
  *   lb (B, S): lower bounds (out);
- *   lA (B, S, n), lbias (B, S): the linear lower planes lA x + lbias
- *   before concretization (out, either may be NULL).
+ *   lA (B, R, n), lbias (B, R) with R = plane_rows: the linear lower planes
+ *   lA x + lbias before concretization of the leading R rows (out, either
+ *   may be NULL; 1 <= R <= S when one is given).  R = S is the reference's
+ *   lA; R = 1 (the objective row) is what gh_child_keys_f64 reads.
  * Supported: 1 <= n <= 64.  Results are bit-identical to the FP64 oracle.
  */
 int gh_qcqp_crown_f64(int B, int n, int S, const void *packed,
                       const double *Cm, const double *d, const double *xL,
                       const double *xU, double *lb, double *lA, double *lbias,
-                      hipStream_t stream);
+                      int plane_rows, hipStream_t stream);
 
 /*
  * FP32-sound form of gh_qcqp_crown_f64: the same relaxation evaluated in
@@ -82,9 +84,10 @@ int gh_qcqp_crown_f64(int B, int n, int S, const void *packed,
  *   Q (S, n, n) FP64 (for the tie cases), Cm, d, xL, xU as
  *   gh_qcqp_crown_f64 (FP64; boxes are rounded outward to FP32);
  *   lb (B, S) FP64 (FP32 values); flag (B) int32;
- *   lA (B, S, n), lbias (B, S) (either may be NULL): the lower planes,
- *   lA x + lbias <= f_s(x) on the box with lbias lowered by the planes'
- *   rigorous error (for gh_lp_dual_bound_f64).
+ *   lA (B, R, n), lbias (B, R) of the leading R = plane_rows rows (either
+ *   may be NULL): the lower planes, lA x + lbias <= f_s(x) on the box with
+ *   lbias lowered by the planes' rigorous error (for gh_lp_dual_bound_f64
+ *   with R = S, for gh_child_keys_f64 with any R).
  * Supported: 1 <= n <= 64.
  */
 size_t gh_qcqp_pack_f32_bytes(int n, int S);
@@ -93,7 +96,8 @@ int gh_qcqp_pack_f32(int n, int S, const double *Q, void *packed,
 int gh_qcqp_crown_f32s(int B, int n, int S, const void *packed,
                        const double *Q, const double *Cm, const double *d,
                        const double *xL, const double *xU, double *lb,
-                       int32_t *flag, double *lA, double *lbias, hipStream_t stream);
+                       int32_t *flag, double *lA, double *lbias, int plane_rows,
+                       hipStream_t stream);
 
 /*
  * FP32-sound CROWN lower bound of a QCQP of any size.  Same contract and
@@ -190,6 +194,34 @@ int gh_split_f64(int B, int n, const double *xL, const double *xU,
                  hipStream_t stream);
 
 /*
+ * Constrained concretization: root[b][h] = a lower bound of
+ * min obj[b][h] . x over the box [xL[b], xU[b]] intersected with the cuts
+ * A[b][k] . x + bc[b][k] <= 0, by the sequential one-row dual sweep.
+ * Replaces constraints_solving with a batch-specific objective, after
+ * sort_out_constr_batches(no_return_inf=True) and
+ * drop_redundant_constraints(max_rows) (auto_LiRPA/concretize_func.py:
+ * 269-850; its Triton sweep concretize_triton.py:159-217), as
+ * PerturbationLpNorm.concretize_matrix calls it (auto_LiRPA/
+ * perturbations.py:241-263) for clip_type 'complete'.
+ *   obj (B, H, X); A (B, K, X), bc (B, K) (may be NULL when K = 0);
+ *   xL, xU (B, X); max_rows: keep at most this many binding cuts per box,
+ *   the largest worst-case values (<= 0: no cap);
+ *   root (B, H) out: never below the box bound of obj over [xL, xU] (the
+ *   larger of the two is returned, as the reference does);
+ *   flags (B) out: bit 0 = some cut misses the box (the domain is
+ *   infeasible: its bounds may be set to +inf), bit 1 = every cut holds on
+ *   the whole box (root is the box bound); nkept (B) out (may be NULL):
+ *   cuts the sweep visited.
+ * Supported: 1 <= X <= 64, 0 <= K <= 256, min(K, max_rows) <= 64.
+ * Bit-identical to the oracle's restatement (its sums run in a stated
+ * order); the error term counts the box's own kept cuts.
+ */
+int gh_constrained_concretize_f64(int B, int H, int K, int X, const double *obj,
+                                  const double *A, const double *bc, const double *xL,
+                                  const double *xU, int max_rows, double *root,
+                                  int32_t *flags, int32_t *nkept, hipStream_t stream);
+
+/*
  * Child keys of a split (solving mode): for the 2K children of K parents
  * (layout of gh_split_f64: child k belongs to parent k mod K),
  * keys[k][0] = max(plb[p][0], the minimum of the parent's objective plane
@@ -198,10 +230,11 @@ int gh_split_f64(int B, int n, const double *xL, const double *xU,
  * keys[k][s] = plb[p][s] for s >= 1.  Replaces _raise_child_keys /
  * _plane_minimum_down (complete_verifier/input_split/
  * batch_branch_and_bound.py:189-225) of the reordered loop.
- *   plb (K, S), lA (K, S, n), lbias (K, S): the parents' bounds and planes;
- *   cL, cU (2K, n): children; keys (2K, S) out.
+ *   plb (K, S): the parents' bounds; lA (K, R, n), lbias (K, R): their
+ *   planes of the leading R >= 1 rows (row 0 is read); cL, cU (2K, n):
+ *   children; keys (2K, S) out.  Both sums run in index order.
  */
-int gh_child_keys_f64(int K, int n, int S, const double *plb, const double *lA,
+int gh_child_keys_f64(int K, int n, int S, int R, const double *plb, const double *lA,
                       const double *lbias, const double *cL, const double *cU, double *keys,
                       hipStream_t stream);
 

@@ -56,6 +56,11 @@ def lib():
         h.gho_split.restype = None
         h.gho_prune.argtypes = [ctypes.c_int] * 3 + [_D] * 4 + [_U8, ctypes.POINTER(ctypes.c_double)]
         h.gho_prune.restype = ctypes.c_long
+        _C = ctypes.c_int
+        h.gho_constrained_concretize.argtypes = [_C] * 4 + [_D] * 5 + [_C, _D, _I32, _I32]
+        h.gho_constrained_concretize.restype = ctypes.c_int
+        h.gho_constraint_rows.argtypes = [_C, _C, _D, _D, _D, _D, _C, _I32, _D, _I32]
+        h.gho_constraint_rows.restype = ctypes.c_int
         _lib = h
     return _lib
 
@@ -202,6 +207,41 @@ def lp_dual_value(c, c0, G, h, l, u, y):
     E = E + 4 * U64 * (abs(val) + E)
     lb = val - E
     return lb if np.isfinite(lb) else -np.inf
+
+
+def constrained_concretize(obj, A, b, xL, xU, max_rows=16):
+    """Lower bounds root (B, H) of min obj[b, h] . x over the box [xL, xU]
+    intersected with the cuts A[b, k] . x + b[b, k] <= 0 (constraints_solving
+    with sort_out_constr_batches, drop_redundant_constraints and the
+    sequential dual sweep, concretize_func.py:269-850; oracle/concretize.c
+    states the order of evaluation).  Returns (root, flags, kept): flags bit
+    0 = some cut misses the box (infeasible), bit 1 = every cut holds on the
+    whole box; kept = rows the sweep visited."""
+    obj, A, b, xL, xU = (_c(v) for v in (obj, A, b, xL, xU))
+    B, H, X = obj.shape
+    K = A.shape[1]
+    root = np.empty((B, H))
+    flags = np.empty(B, dtype=np.int32)
+    kept = np.empty(B, dtype=np.int32)
+    rc = lib().gho_constrained_concretize(B, H, K, X, obj, A.reshape(B, K, X) if K else
+                                          np.zeros((B, 1, X)), b.reshape(B, K) if K else
+                                          np.zeros((B, 1)), xL, xU, int(max_rows), root, flags,
+                                          kept)
+    if rc:
+        raise ValueError("unsupported size (X <= 64)")
+    return root, flags, kept
+
+
+def constraint_rows(A, b, xL, xU, max_rows=0):
+    """Rows (indices, centred offsets d, flags) the sweep of one box visits
+    (drop_redundant_constraints after sort_out_constr_batches)."""
+    A, b, xL, xU = (_c(v) for v in (A, b, xL, xU))
+    K, X = A.shape
+    keep = np.empty(max(K, 1), dtype=np.int32)
+    d = np.empty(max(K, 1))
+    flags = np.empty(1, dtype=np.int32)
+    nk = lib().gho_constraint_rows(K, X, A, b, xL, xU, int(max_rows), keep, d, flags)
+    return keep[:nk].copy(), d[:nk].copy(), int(flags[0])
 
 
 def plane_minimum_down(a, b, xl, xu):

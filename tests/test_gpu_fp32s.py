@@ -86,3 +86,8 @@ def test_fp32s_planes_are_valid_lower_planes(cuda):
         # the plane's box minimum is at least the concretized bound's value
         pm = np.minimum(lA[b] * L[b], lA[b] * U[b]).sum(1) + lbias[b]
         assert np.all(pm >= lb[b] - 1e-6 * (1 + np.abs(lb[b])))
+    # the objective-row form writes row 0 of the same planes
+    _, lA1, lbias1 = net.bound(torch.as_tensor(L, device=cuda), torch.as_tensor(U, device=cuda),
+                               return_A=True, plane_rows=1)
+    assert np.array_equal(lA1.cpu().numpy(), lA[:, :1])
+    assert np.array_equal(lbias1.cpu().numpy(), lbias[:, :1])

@@ -34,6 +34,13 @@ def test_qcqp_crown_f64_bit_exact(cuda, n, m, B):
     assert _bits_equal(lb.cpu().numpy(), lb_o)
     assert _bits_equal(lA.cpu().numpy(), lA_o)
     assert _bits_equal(lbias.cpu().numpy(), lbias_o)
+    # the leading-rows form writes the same planes for the rows it keeps
+    R = min(2, p.S)
+    lb2, lA2, lbias2 = net.bound(_dev(L, cuda), _dev(U, cuda), return_A=True, plane_rows=R)
+    assert lA2.shape == (B, R, n) and lbias2.shape == (B, R)
+    assert _bits_equal(lb2.cpu().numpy(), lb_o)
+    assert _bits_equal(lA2.cpu().numpy(), lA_o[:, :R])
+    assert _bits_equal(lbias2.cpu().numpy(), lbias_o[:, :R])
 
 
 def test_compute_bounds_interface(cuda):
