@@ -39,3 +39,39 @@ def constrained_concretize(obj, A, bc, x_L, x_U, max_rows=16, stream=None):
                                               stream_ptr(stream)),
           "gh_constrained_concretize_f64")
     return root, flags, kept
+
+
+def constrained_bound(lA, lbias, A, bc, x_L, x_U, lb, max_rows=16, stream=None):
+    """In place on lb (B, H): max(lb, lbias + the constrained root of lA),
+    +inf on every row of a box a cut misses (gh_constrained_bound_f64).
+    Returns (lb, flags, kept)."""
+    B, H, X = lA.shape
+    K = A.shape[1]
+    if X > MAX_X:
+        raise ValueError(f"constrained concretization supports X <= {MAX_X}")
+    for t in (lA, lbias, A, bc, x_L, x_U, lb):
+        if not t.is_contiguous():
+            raise ValueError("constrained_bound takes contiguous tensors")
+    flags = torch.empty(B, dtype=torch.int32, device=lA.device)
+    kept = torch.empty(B, dtype=torch.int32, device=lA.device)
+    check(lib().gh_constrained_bound_f64(B, H, K, X, ptr(lA), ptr(lbias), ptr(A), ptr(bc),
+                                         ptr(x_L), ptr(x_U), int(max_rows), ptr(lb), ptr(flags),
+                                         ptr(kept), stream_ptr(stream)),
+          "gh_constrained_bound_f64")
+    return lb, flags, kept
+
+
+def clip_domains(lA, lbias, thr, x_L, x_U, out=None, stream=None):
+    """Clipped boxes (oL, oU) and state (B) int32 of boxes x_L, x_U (B, n) by
+    the planes lA (P, S, n), lbias (P, S) of set b % P against thr (S)
+    (gh_clip_domains_f64).  out=(oL, oU) may be (x_L, x_U) for in place."""
+    P, S, n = lA.shape
+    B = x_L.shape[0]
+    x_L = x_L.contiguous()
+    x_U = x_U.contiguous()
+    oL, oU = out if out is not None else (torch.empty_like(x_L), torch.empty_like(x_U))
+    state = torch.empty(B, dtype=torch.int32, device=x_L.device)
+    check(lib().gh_clip_domains_f64(B, P, n, S, ptr(lA.contiguous()), ptr(lbias.contiguous()),
+                                    ptr(thr.contiguous()), ptr(x_L), ptr(x_U), ptr(oL), ptr(oU),
+                                    ptr(state), stream_ptr(stream)), "gh_clip_domains_f64")
+    return oL, oU, state

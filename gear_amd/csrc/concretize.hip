@@ -151,7 +151,7 @@ __global__ __launch_bounds__(64 * kWaves) void constrained_concretize_kernel(
     int H, int K, int X, const double* __restrict__ obj, const double* __restrict__ A,
     const double* __restrict__ b, const double* __restrict__ xL, const double* __restrict__ xU,
     int max_rows, int Kc, double* __restrict__ root, int32_t* __restrict__ flags,
-    int32_t* __restrict__ nkept)
+    int32_t* __restrict__ nkept, const double* __restrict__ bias, double* __restrict__ lb_io)
 {
     extern __shared__ double lds[];
     double* rows = lds;                    // Kc x 64
@@ -274,8 +274,42 @@ __global__ __launch_bounds__(64 * kWaves) void constrained_concretize_kernel(
             if (!isfinite(fin)) fin = naive;
             out = fmax(fin, naive);
         }
-        if (lane == 0) root[(size_t)bx * H + h] = out;
+        if (lane == 0) {
+            const size_t bh = (size_t)bx * H + h;
+            if (bias) {
+                // the bound: lbias widened and added as the box concretization
+                // adds it, never below the given bound; +inf where a cut
+                // misses the box (the domain is infeasible)
+                double v = INFINITY;
+                if (!infeasible) {
+                    const double bb = bias[bh];
+                    const double bw = gh::widen(bb, gh::finite_abs(bb) * k_round(X), -1.0);
+                    v = fmax(gh::add_enclosed(bw, out, -1.0, k_round(1)), lb_io[bh]);
+                }
+                lb_io[bh] = v;
+            } else {
+                root[bh] = out;
+            }
+        }
     }
+}
+
+int launch_constrained(int B, int H, int K, int X, const double* obj, const double* A,
+                       const double* b, const double* xL, const double* xU, int max_rows,
+                       double* root, int32_t* flags, int32_t* nkept, const double* bias,
+                       double* lb_io, hipStream_t stream)
+{
+    if (B < 0 || H < 1 || K < 0 || X < 1) return GH_EINVAL;
+    if (X > 64 || K > 256) return GH_EUNSUPPORTED;
+    const int Kc = max_rows > 0 ? (K < max_rows ? K : max_rows) : K;
+    if (Kc > 64) return GH_EUNSUPPORTED;
+    if (B == 0) return GH_OK;
+    if (!obj || !xL || !xU || !flags || (K > 0 && (!A || !b))) return GH_EINVAL;
+    const size_t lds = (size_t)Kc * 64 * sizeof(double) + 2 * (size_t)K * sizeof(double) +
+                       ((size_t)K + Kc + 3) * sizeof(int);
+    hipLaunchKernelGGL(constrained_concretize_kernel, dim3(B), dim3(64 * kWaves), lds, stream, H,
+                       K, X, obj, A, b, xL, xU, max_rows, Kc, root, flags, nkept, bias, lb_io);
+    return hipGetLastError() == hipSuccess ? GH_OK : GH_ELAUNCH;
 }
 
 }  // namespace
@@ -285,15 +319,116 @@ extern "C" int gh_constrained_concretize_f64(int B, int H, int K, int X, const d
                                              const double* xU, int max_rows, double* root,
                                              int32_t* flags, int32_t* nkept, hipStream_t stream)
 {
-    if (B < 0 || H < 1 || K < 0 || X < 1) return GH_EINVAL;
-    if (X > 64 || K > 256) return GH_EUNSUPPORTED;
-    const int Kc = max_rows > 0 ? (K < max_rows ? K : max_rows) : K;
-    if (Kc > 64) return GH_EUNSUPPORTED;
+    if (B > 0 && !root) return GH_EINVAL;
+    return launch_constrained(B, H, K, X, obj, A, b, xL, xU, max_rows, root, flags, nkept,
+                              nullptr, nullptr, stream);
+}
+
+extern "C" int gh_constrained_bound_f64(int B, int H, int K, int X, const double* lA,
+                                        const double* lbias, const double* A, const double* b,
+                                        const double* xL, const double* xU, int max_rows,
+                                        double* lb, int32_t* flags, int32_t* nkept,
+                                        hipStream_t stream)
+{
+    if (B > 0 && (!lbias || !lb)) return GH_EINVAL;
+    return launch_constrained(B, H, K, X, lA, A, b, xL, xU, max_rows, nullptr, flags, nkept,
+                              lbias, lb, stream);
+}
+
+// ------------------------------------------------------------ clipping
+namespace {
+
+__device__ __forceinline__ double nan_min(double a, double b)
+{
+    return (isnan(a) || isnan(b)) ? NAN : (a < b ? a : b);
+}
+__device__ __forceinline__ double nan_max(double a, double b)
+{
+    return (isnan(a) || isnan(b)) ? NAN : (a > b ? a : b);
+}
+
+// One wave per box, lane i = coordinate i.  Box b takes plane set b % P.
+// Every value a box's lanes need is read before its outputs are written,
+// so oL, oU may alias xL, xU.
+__global__ __launch_bounds__(256) void clip_domains_kernel(
+    int B, int P, int n, int S, const double* __restrict__ lA, const double* __restrict__ lbias,
+    const double* __restrict__ thr, const double* xL, const double* xU, double* oL, double* oU,
+    int32_t* __restrict__ state, double und, double cfac)
+{
+    const int lane = threadIdx.x & 63;
+    const int b = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (b >= B) return;
+    const bool on = lane < n;
+    const double l = on ? xL[(size_t)b * n + lane] : 0.0;
+    const double u = on ? xU[(size_t)b * n + lane] : 0.0;
+    const double xh = (u + l) / 2, ep = (u - l) / 2;
+    const double mx = gh::finite_abs(fmax(fabs(l), fabs(u)));
+    const double* A = lA + (size_t)(b % P) * S * n;
+    const double* c = lbias + (size_t)(b % P) * S;
+    const double kn = (double)(n + 16) * kEps * 4.0;
+    double accU = INFINITY, accL = -INFINITY;
+    bool fin = true, cert = false;
+    for (int s = 0; s < S; ++s) {
+        const double a = on ? A[(size_t)s * n + lane] : 0.0;
+        const double cs = c[s], ts = thr[s];
+        fin = fin && isfinite(a) && isfinite(cs) && (isfinite(ts) || (isinf(ts) && ts > 0));
+        const double t1 = red64(on ? a * xh : 0.0);
+        const double t2 = red64(on ? fabs(a) * ep : 0.0);
+        const double dm = (t1 - t2) + cs;
+        const double bre = (red64(on ? gh::finite_abs(a) * mx : 0.0) + gh::finite_abs(cs)) * kn;
+        cert = cert || ((dm - bre) - und > ts);
+        const double derr = bre * cfac + und;
+        const double lx = a * xh, le = fabs(a) * ep;
+        const double cm1 = (dm - lx) + le;
+        const double num = ts - cm1;
+        const double mag =
+            ((gh::finite_abs(ts) + gh::finite_abs(dm)) + gh::finite_abs(lx)) + gh::finite_abs(le);
+        const double nerr = (mag * (4 * kEps) + und) + derr;
+        double lo = num, hi = num;
+        if (isfinite(num)) {
+            lo = num - nerr;
+            hi = num + nerr;
+        }
+        const double qlo = lo / a, qhi = hi / a;
+        const double cu = nextafter(nan_max(qlo, qhi), INFINITY);
+        const double cl = nextafter(nan_min(qlo, qhi), -INFINITY);
+        accU = nan_min(accU, a > 0 ? cu : INFINITY);
+        accL = nan_max(accL, a < 0 ? cl : -INFINITY);
+    }
+    fin = __ballot(on && !fin) == 0;
+    double nl = l, nu = u;
+    int st = 0;
+    if (fin) {
+        if (cert) {
+            if (lane == 0) nl = nextafter(u, INFINITY);
+            st = 2;
+        } else {
+            nu = nan_min(accU, u);
+            nl = nan_max(accL, l);
+            st = __ballot(on && !(nl <= nu)) ? 3 : 1;
+        }
+    }
+    if (on) {
+        oL[(size_t)b * n + lane] = nl;
+        oU[(size_t)b * n + lane] = nu;
+    }
+    if (lane == 0) state[b] = st;
+}
+
+}  // namespace
+
+extern "C" int gh_clip_domains_f64(int B, int P, int n, int S, const double* lA,
+                                   const double* lbias, const double* thr, const double* xL,
+                                   const double* xU, double* oL, double* oU, int32_t* state,
+                                   hipStream_t stream)
+{
+    if (B < 0 || P < 1 || n < 1 || S < 1) return GH_EINVAL;
+    if (n > 64) return GH_EUNSUPPORTED;
     if (B == 0) return GH_OK;
-    if (!obj || !xL || !xU || !root || !flags || (K > 0 && (!A || !b))) return GH_EINVAL;
-    const size_t lds = (size_t)Kc * 64 * sizeof(double) + 2 * (size_t)K * sizeof(double) +
-                       ((size_t)K + Kc + 3) * sizeof(int);
-    hipLaunchKernelGGL(constrained_concretize_kernel, dim3(B), dim3(64 * kWaves), lds, stream, H,
-                       K, X, obj, A, b, xL, xU, max_rows, Kc, root, flags, nkept);
+    if (!lA || !lbias || !thr || !xL || !xU || !oL || !oU || !state) return GH_EINVAL;
+    const double und = (double)(n + 16) * 2.2250738585072014e-308 * kEps;
+    const double cfac = (double)(n + 4) / (double)(4 * (n + 16));
+    hipLaunchKernelGGL(clip_domains_kernel, dim3((B + 3) / 4), dim3(256), 0, stream, B, P, n, S,
+                       lA, lbias, thr, xL, xU, oL, oU, state, und, cfac);
     return hipGetLastError() == hipSuccess ? GH_OK : GH_ELAUNCH;
 }

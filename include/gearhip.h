@@ -222,6 +222,39 @@ int gh_constrained_concretize_f64(int B, int H, int K, int X, const double *obj,
                                   int32_t *flags, int32_t *nkept, hipStream_t stream);
 
 /*
+ * The bound form of gh_constrained_concretize_f64, for the planes of a CROWN
+ * pass: lb[b][h] = max(lb[b][h], lbias[b][h] widened and added to the
+ * constrained root of lA[b][h] as the box concretization adds it
+ * (auto_LiRPA/concretize_bounds.py:280-304)), and +inf on every row of a
+ * box that a cut misses (input_split/bounding.py:345-350).  lb (B, H) holds
+ * the box bounds on entry (e.g. gh_qcqp_crown_f64's lb) and the result on
+ * return; lA (B, H, X), lbias (B, H); the rest as above.
+ */
+int gh_constrained_bound_f64(int B, int H, int K, int X, const double *lA,
+                             const double *lbias, const double *A, const double *bc,
+                             const double *xL, const double *xU, int max_rows, double *lb,
+                             int32_t *flags, int32_t *nkept, hipStream_t stream);
+
+/*
+ * Domain clipping: shrink each box [xL[b], xU[b]] along every coordinate to
+ * the part where some point may still be unverified by the planes
+ * lA[p][s] x + lbias[p][s] against thr[s] (p = b % P: the 2K children of a
+ * split share their parent's planes), rounding every candidate bound
+ * outward so no exactly feasible point is removed; a box that one plane
+ * already certifies is emptied (oL[b][0] = nextafter(xU[b][0], +inf)).
+ * Replaces clip_domains(x_L, x_U, thresholds, lA, lbias, num_iters=1)
+ * (complete_verifier/input_split/clip.py:147-503) of clip_type 'complete'.
+ *   lA (P, S, n), lbias (P, S); thr (S) (+inf: an inactive row);
+ *   xL, xU (B, n); oL, oU (B, n) out (may alias xL, xU);
+ *   state (B) out: 0 left as is (non-finite plane data), 1 clipped,
+ *   2 certified and emptied, 3 clipped to empty.
+ * Supported: 1 <= n <= 64.  Bit-identical to the oracle's restatement.
+ */
+int gh_clip_domains_f64(int B, int P, int n, int S, const double *lA, const double *lbias,
+                        const double *thr, const double *xL, const double *xU, double *oL,
+                        double *oU, int32_t *state, hipStream_t stream);
+
+/*
  * Child keys of a split (solving mode): for the 2K children of K parents
  * (layout of gh_split_f64: child k belongs to parent k mod K),
  * keys[k][0] = max(plb[p][0], the minimum of the parent's objective plane

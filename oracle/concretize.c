@@ -202,7 +202,8 @@ static double solve_beta(int X, const double *c, const double *a, double d, cons
  */
 int gho_constrained_concretize(int B, int H, int K, int X, const double *obj, const double *A,
                                const double *b, const double *xL, const double *xU,
-                               int max_rows, double *root, int32_t *flags, int32_t *nkept)
+                               int max_rows, double *root, int32_t *flags, int32_t *nkept,
+                               const double *bias, double *lb_io)
 {
     if (X < 1 || X > XMAX || K < 0 || H < 1) return 1;
 #pragma omp parallel for schedule(dynamic, 16)
@@ -265,7 +266,24 @@ int gho_constrained_concretize(int B, int H, int K, int X, const double *obj, co
                 if (!isfinite(fin)) fin = naive;
                 out = fmax(fin, naive);
             }
-            root[(size_t)bx * H + h] = out;
+            if (bias) {
+                /* the bound: lbias widened and added as the box
+                 * concretization adds it (concretize_bounds.py:280-304),
+                 * never below the given bound; +inf for an infeasible
+                 * domain (bounding.py:345-350) */
+                size_t bh = (size_t)bx * H + h;
+                double v = INFINITY;
+                if (!infeasible) {
+                    double bb = bias[bh];
+                    double bw = widen(bb, finite_abs(bb) * k_round(X), -1.0);
+                    double r = bw + out;
+                    r = widen(r, finite_abs(r) * k_round(1), -1.0);
+                    v = fmax(r, lb_io[bh]);
+                }
+                lb_io[bh] = v;
+            } else {
+                root[(size_t)bx * H + h] = out;
+            }
         }
         free(keep);
         free(dk);
@@ -291,4 +309,93 @@ int gho_constraint_rows(int K, int X, const double *A, const double *b, const do
     *flags = (inf ? 1 : 0) | (cov ? 2 : 0);
     free(kp);
     return nk;
+}
+
+/* ------------------------------------------------------------------------
+ * Domain clipping (complete_verifier/input_split/clip.py:147-503,
+ * clip_domains with lbias given, dm_lb recomputed, one iteration, no base
+ * input constraints, thresholds as given): shrink each box along every
+ * coordinate to the part where some point may still be unverified by the
+ * planes lA_s x + lbias_s against thresholds thr_s.  Box b takes the planes
+ * of plane set b % P (the children of a split share their parent's).
+ *   concretize_bounds                clip.py:490-503
+ *   _underflow, _concretize_error    clip.py:126-145
+ *   certified domains emptied        clip.py:395-421
+ *   _clip_main_fn (i = 0)            clip.py:147-212
+ * Torch's min / max propagate NaN; nan_min / nan_max below do the same.
+ */
+#define TINY64 2.2250738585072014e-308
+
+static double nan_min(double a, double b) { return (isnan(a) || isnan(b)) ? NAN : (a < b ? a : b); }
+static double nan_max(double a, double b) { return (isnan(a) || isnan(b)) ? NAN : (a > b ? a : b); }
+
+int gho_clip_domains(int B, int P, int n, int S, const double *lA, const double *lbias,
+                     const double *thr, const double *xL, const double *xU, double *oL,
+                     double *oU, int32_t *state)
+{
+    if (n < 1 || n > XMAX || P < 1 || S < 1) return 1;
+    const double und = (double)(n + 16) * TINY64 * EPS64;
+    const double cfac = (double)(n + 4) / (double)(4 * (n + 16));
+#pragma omp parallel for schedule(static)
+    for (int b = 0; b < B; ++b) {
+        const double *l = xL + (size_t)b * n, *u = xU + (size_t)b * n;
+        const double *A = lA + (size_t)(b % P) * S * n, *c = lbias + (size_t)(b % P) * S;
+        double *ol = oL + (size_t)b * n, *ou = oU + (size_t)b * n;
+        for (int i = 0; i < n; ++i) { ol[i] = l[i]; ou[i] = u[i]; }
+        int fin = 1;
+        for (int s = 0; s < S; ++s) {
+            fin = fin && isfinite(c[s]) && (isfinite(thr[s]) || (isinf(thr[s]) && thr[s] > 0));
+            for (int i = 0; i < n; ++i) fin = fin && isfinite(A[(size_t)s * n + i]);
+        }
+        if (!fin) { state[b] = 0; continue; }
+        double xh[XMAX], ep[XMAX], mx[XMAX], p[XMAX];
+        double accU[XMAX], accL[XMAX];
+        for (int i = 0; i < n; ++i) {
+            xh[i] = (u[i] + l[i]) / 2;
+            ep[i] = (u[i] - l[i]) / 2;
+            mx[i] = finite_abs(fmax(fabs(l[i]), fabs(u[i])));
+            accU[i] = INFINITY;
+            accL[i] = -INFINITY;
+        }
+        int cert = 0;
+        for (int s = 0; s < S; ++s) {
+            const double *a = A + (size_t)s * n;
+            for (int i = 0; i < XMAX; ++i) p[i] = i < n ? a[i] * xh[i] : 0.0;
+            double t1 = red64(p);
+            for (int i = 0; i < XMAX; ++i) p[i] = i < n ? fabs(a[i]) * ep[i] : 0.0;
+            double t2 = red64(p);
+            double dm = (t1 - t2) + c[s];
+            for (int i = 0; i < XMAX; ++i) p[i] = i < n ? finite_abs(a[i]) * mx[i] : 0.0;
+            double bre = (red64(p) + finite_abs(c[s])) * k_round(n);
+            if ((dm - bre) - und > thr[s]) cert = 1;
+            double derr = bre * cfac + und;
+            for (int i = 0; i < n; ++i) {
+                double lx = a[i] * xh[i], le = fabs(a[i]) * ep[i];
+                double cm1 = (dm - lx) + le;
+                double num = thr[s] - cm1;
+                double mag = ((finite_abs(thr[s]) + finite_abs(dm)) + finite_abs(lx)) + finite_abs(le);
+                double nerr = (mag * (4 * EPS64) + und) + derr;
+                double lo = num, hi = num;
+                if (isfinite(num)) { lo = num - nerr; hi = num + nerr; }
+                double qlo = lo / a[i], qhi = hi / a[i];
+                double cu = nextafter(nan_max(qlo, qhi), INFINITY);
+                double cl = nextafter(nan_min(qlo, qhi), -INFINITY);
+                accU[i] = nan_min(accU[i], a[i] > 0 ? cu : INFINITY);
+                accL[i] = nan_max(accL[i], a[i] < 0 ? cl : -INFINITY);
+            }
+        }
+        if (cert) {
+            ol[0] = nextafter(u[0], INFINITY);
+            state[b] = 2;
+        } else {
+            for (int i = 0; i < n; ++i) {
+                ou[i] = nan_min(accU[i], u[i]);
+                ol[i] = nan_max(accL[i], l[i]);
+            }
+            int empty = 0;
+            for (int i = 0; i < n; ++i) empty = empty || !(ol[i] <= ou[i]);
+            state[b] = empty ? 3 : 1;
+        }
+    }
+    return 0;
 }

@@ -57,10 +57,13 @@ def lib():
         h.gho_prune.argtypes = [ctypes.c_int] * 3 + [_D] * 4 + [_U8, ctypes.POINTER(ctypes.c_double)]
         h.gho_prune.restype = ctypes.c_long
         _C = ctypes.c_int
-        h.gho_constrained_concretize.argtypes = [_C] * 4 + [_D] * 5 + [_C, _D, _I32, _I32]
+        _V = ctypes.c_void_p
+        h.gho_constrained_concretize.argtypes = [_C] * 4 + [_D] * 5 + [_C, _V, _I32, _I32, _V, _V]
         h.gho_constrained_concretize.restype = ctypes.c_int
         h.gho_constraint_rows.argtypes = [_C, _C, _D, _D, _D, _D, _C, _I32, _D, _I32]
         h.gho_constraint_rows.restype = ctypes.c_int
+        h.gho_clip_domains.argtypes = [_C] * 4 + [_D] * 7 + [_I32]
+        h.gho_clip_domains.restype = ctypes.c_int
         _lib = h
     return _lib
 
@@ -225,11 +228,29 @@ def constrained_concretize(obj, A, b, xL, xU, max_rows=16):
     kept = np.empty(B, dtype=np.int32)
     rc = lib().gho_constrained_concretize(B, H, K, X, obj, A.reshape(B, K, X) if K else
                                           np.zeros((B, 1, X)), b.reshape(B, K) if K else
-                                          np.zeros((B, 1)), xL, xU, int(max_rows), root, flags,
-                                          kept)
+                                          np.zeros((B, 1)), xL, xU, int(max_rows),
+                                          root.ctypes.data, flags, kept, None, None)
     if rc:
         raise ValueError("unsupported size (X <= 64)")
     return root, flags, kept
+
+
+def constrained_bound(lA, lbias, A, b, xL, xU, lb, max_rows=16):
+    """Bounds (B, S) of the planes lA x + lbias over the boxes and cuts:
+    max(lb, lbias widened + the constrained root), +inf where a cut misses
+    the box (gh_constrained_bound_f64's contract).  Returns (bound, flags,
+    kept)."""
+    lA, lbias, A, b, xL, xU, lb = (_c(v) for v in (lA, lbias, A, b, xL, xU, lb))
+    B, H, X = lA.shape
+    K = A.shape[1]
+    out = lb.copy()
+    flags = np.empty(B, dtype=np.int32)
+    kept = np.empty(B, dtype=np.int32)
+    rc = lib().gho_constrained_concretize(B, H, K, X, lA, A, b, xL, xU, int(max_rows), None,
+                                          flags, kept, lbias.ctypes.data, out.ctypes.data)
+    if rc:
+        raise ValueError("unsupported size (X <= 64)")
+    return out, flags, kept
 
 
 def constraint_rows(A, b, xL, xU, max_rows=0):
@@ -242,6 +263,23 @@ def constraint_rows(A, b, xL, xU, max_rows=0):
     flags = np.empty(1, dtype=np.int32)
     nk = lib().gho_constraint_rows(K, X, A, b, xL, xU, int(max_rows), keep, d, flags)
     return keep[:nk].copy(), d[:nk].copy(), int(flags[0])
+
+
+def clip_domains(lA, lbias, thr, xL, xU):
+    """Clipped boxes (oL, oU) and state (B) (0 skipped: non-finite data,
+    1 clipped, 2 certified and emptied, 3 clipped to empty) of boxes
+    xL, xU (B, n) by the planes lA (P, S, n), lbias (P, S) of set b % P
+    against thresholds thr (S) (clip_domains, complete_verifier/input_split/
+    clip.py:147-503, one iteration; oracle/concretize.c)."""
+    lA, lbias, thr, xL, xU = (_c(v) for v in (lA, lbias, thr, xL, xU))
+    P, S, n = lA.shape
+    B = xL.shape[0]
+    oL = np.empty_like(xL)
+    oU = np.empty_like(xU)
+    state = np.empty(B, dtype=np.int32)
+    if lib().gho_clip_domains(B, P, n, S, lA, lbias, thr, xL, xU, oL, oU, state):
+        raise ValueError("unsupported size (n <= 64)")
+    return oL, oU, state
 
 
 def plane_minimum_down(a, b, xl, xu):

@@ -46,3 +46,58 @@ def test_matches_oracle(cuda, B, H, K, X, max_rows, special):
         f"max diff {np.nanmax(np.abs(got - r0))}"
     if special:
         assert (f0 & 1).any() and (f0 & 2).any() and (k0 > 0).any()
+
+
+def test_bound_form_matches_oracle(cuda):
+    """The bound form on real CROWN planes of a QCQP with the constraint
+    rows' planes as cuts: equal to the oracle bit for bit, never below the
+    box bound, above it on some rows, +inf on infeasible domains."""
+    from gear_amd.concretize import constrained_bound
+    from gear_amd.crown import BoundedQCQP
+    from gear_amd.problems import qcqp_dense, random_bisection_boxes
+    p = qcqp_dense(20, 6, seed=4)
+    L, U = random_bisection_boxes(p.xl, p.xu, 400, seed=3)
+    lb0, lA, lbias = O.qcqp_crown(p.Q, p.c, p.d, L, U, return_A=True)
+    thr = np.concatenate([[np.inf], p.rhs[1:] + 1e-6])
+    cutA = lA[:, 1:].copy()
+    cutb = lbias[:, 1:] - thr[1:]
+    cutb[::11] -= 50.0            # a few boxes with stronger cuts
+    cutb[5::37, 0] += 1e4         # and a few infeasible ones
+    ref, f0, k0 = O.constrained_bound(lA, lbias, cutA, cutb, L, U, lb0)
+    d = lambda a: torch.as_tensor(np.ascontiguousarray(a), device=cuda)
+    lb = d(lb0)
+    got, f, k = constrained_bound(d(lA), d(lbias), d(cutA), d(cutb), d(L), d(U), lb)
+    torch.cuda.synchronize()
+    g = got.cpu().numpy()
+    assert np.array_equal(f.cpu().numpy(), f0) and np.array_equal(k.cpu().numpy(), k0)
+    assert np.array_equal(g.view(np.int64), ref.view(np.int64))
+    assert np.all(g >= lb0)
+    assert (g > lb0).any()
+    assert np.all(np.isinf(g[(f0 & 1) == 1]))
+    assert ((f0 & 1) == 1).any()
+
+
+@pytest.mark.parametrize("P,S,n,B", [(64, 21, 50, 128), (7, 3, 5, 14), (1, 1, 1, 9),
+                                     (16, 4, 64, 32)])
+def test_clip_matches_oracle(cuda, P, S, n, B):
+    from gear_amd.concretize import clip_domains
+    print(f"case P={P} S={S} n={n} B={B}", flush=True)
+    rng = np.random.default_rng(P + S + n)
+    A = rng.standard_normal((P, S, n))
+    c = rng.standard_normal((P, S))
+    thr = np.abs(rng.standard_normal(S)) * 3
+    thr[0] = np.inf if S > 2 else thr[0]
+    lo = -rng.random((B, n)) - 0.2
+    hi = rng.random((B, n)) + 0.2
+    if P > 2:
+        A[1, 0, 0] = np.nan                  # non-finite planes: left as is
+        c[2, -1] = 1e3                       # a row that certifies its boxes
+    oL0, oU0, st0 = O.clip_domains(A, c, thr, lo, hi)
+    d = lambda a: torch.as_tensor(np.ascontiguousarray(a), device=cuda)
+    xL, xU = d(lo), d(hi)
+    oL, oU, st = clip_domains(d(A), d(c), d(thr), xL, xU, out=(xL, xU))    # in place
+    torch.cuda.synchronize()
+    assert np.array_equal(st.cpu().numpy(), st0)
+    assert np.array_equal(oL.cpu().numpy().view(np.int64), oL0.view(np.int64))
+    assert np.array_equal(oU.cpu().numpy().view(np.int64), oU0.view(np.int64))
+    assert (st0 == 1).any()

@@ -171,3 +171,107 @@ def test_one_row_multiplier_maximises_the_dual():
         g = max(c @ x0 + be * d - np.abs(c + be * a) @ e for be in cand)
         assert abs(r[0, 0] - g) <= 1e-9 * (1 + abs(g)), (t, r[0, 0], g)
         assert r[0, 0] <= g + 1e-12
+
+
+# ---------------------------------------------------------------- clipping
+from fractions import Fraction as Fr
+
+
+def _exact_min(a, c, lo, hi):
+    v = Fr(float(c))
+    for ai, l, u in zip(a, lo, hi):
+        ai, l, u = Fr(float(ai)), Fr(float(l)), Fr(float(u))
+        v += min(ai * l, ai * u)
+    return v
+
+
+def _plain_clip(lo, hi, a, c, thr):
+    xh, ep = (hi + lo) / 2, (hi - lo) / 2
+    dm = a @ xh - np.abs(a) @ ep + c
+    cx = (thr - (dm - a * xh + np.abs(a) * ep)) / np.where(a == 0, np.nan, a)
+    xu = np.where(a > 0, cx, np.inf)
+    xl = np.where(a < 0, cx, -np.inf)
+    return np.maximum(xl, lo), np.minimum(xu, hi)
+
+
+def test_clip_removes_no_exactly_feasible_point():
+    """complete_verifier/tests/test_clip_outward.py:29-70: points exactly on
+    the plane's threshold stay inside the clipped box (Fraction fuzz), a box
+    is emptied only when the plane's exact minimum is above the threshold,
+    and the clipped volume is within 1e-12 (log, relative) of the plain
+    floating-point clip."""
+    rng = np.random.default_rng(0)
+    bad = 0
+    worst = 0.0
+    for trial in range(300):
+        n = int(rng.integers(1, 5))
+        scale = 10.0 ** int(rng.integers(-3, 4))
+        lo = rng.standard_normal(n) - 1
+        hi = lo + rng.random(n) * 2 + 0.1
+        a = rng.standard_normal(n) * scale
+        c = rng.standard_normal() * scale
+        mid = _exact_min(a, c, lo, hi)
+        top = -_exact_min(-a, -c, lo, hi)
+        thr = float(mid + (top - mid) * Fr(int(rng.integers(5, 95)), 100))
+        oL, oU, st = O.clip_domains(a[None, None], np.array([[c]]), np.array([thr]),
+                                    lo[None], hi[None])
+        oL, oU = oL[0], oU[0]
+        if (oL > oU).any():
+            bad += Fr(thr) >= mid
+            continue
+        for j in range(n):
+            if a[j] == 0:
+                continue
+            for _ in range(5):
+                x = [Fr(float(lo[k])) + (Fr(float(hi[k])) - Fr(float(lo[k]))) *
+                     Fr(int(rng.integers(0, 1001)), 1000) for k in range(n)]
+                rest = Fr(float(c)) + sum(Fr(float(a[k])) * x[k] for k in range(n) if k != j)
+                x[j] = (Fr(thr) - rest) / Fr(float(a[j]))
+                if not (Fr(float(lo[j])) <= x[j] <= Fr(float(hi[j]))):
+                    continue
+                if not all(Fr(float(oL[k])) <= x[k] <= Fr(float(oU[k])) for k in range(n)):
+                    bad += 1
+        rL, rU = _plain_clip(lo, hi, a, c, thr)
+        wn, wo = np.maximum(oU - oL, 0), np.maximum(rU - rL, 0)
+        if (wo > 0).all():
+            worst = max(worst, abs(float(np.log(wn).sum() - np.log(wo).sum())))
+    assert bad == 0
+    assert worst < 1e-12, worst
+
+
+def test_clip_certified_needs_margin():
+    """complete_verifier/tests/test_clip_outward.py:73-86."""
+    lo, hi = np.zeros((1, 2)), np.ones((1, 2))
+    a, c = np.array([[[1.0, 1.0]]]), np.array([[0.1]])
+    lb = (a[0, 0] @ ((hi + lo) / 2)[0] - np.abs(a[0, 0]) @ ((hi - lo) / 2)[0]) + 0.1
+    oL, oU, st = O.clip_domains(a, c, np.array([np.nextafter(lb, -np.inf)]), lo, hi)
+    assert (oL <= oU).all() and st[0] == 1
+    oL, oU, st = O.clip_domains(a, c, np.array([lb - 1e-6]), lo, hi)
+    assert (oL > oU).any() and st[0] == 2
+
+
+def test_clip_keeps_every_unverified_point():
+    """Several planes at once (the rows of a QCQP's CROWN planes): no sampled
+    point with every plane at or below its threshold leaves the clipped box;
+    an infinite threshold (no incumbent yet) clips nothing; boxes of the
+    same parent share its planes (b % P)."""
+    rng = np.random.default_rng(5)
+    P, S, n, B = 4, 5, 6, 16
+    A = rng.standard_normal((P, S, n))
+    c = rng.standard_normal((P, S))
+    thr = np.abs(rng.standard_normal(S)) * 2
+    lo = -rng.random((B, n)) - 0.2
+    hi = rng.random((B, n)) + 0.2
+    oL, oU, st = O.clip_domains(A, c, thr, lo, hi)
+    shrunk = 0
+    for b in range(B):
+        X = lo[b] + rng.random((4000, n)) * (hi[b] - lo[b])
+        keep = ((X @ A[b % P].T + c[b % P]) <= thr).all(1)
+        inside = ((X >= oL[b]) & (X <= oU[b])).all(1)
+        assert not (keep & ~inside).any(), b
+        shrunk += (oL[b] > lo[b]).any() or (oU[b] < hi[b]).any()
+    assert shrunk > 0
+    thr_inf = thr.copy()
+    thr_inf[:] = np.inf
+    oL, oU, st = O.clip_domains(A, c, thr_inf, lo, hi)
+    assert np.array_equal(oL, lo) and np.array_equal(oU, hi)
