@@ -54,6 +54,14 @@ class BaBConfig:
     # picked domains, split them, give the children keys from the parents'
     # planes; False: the regular loop (split, then bound the children)
     reorder: bool = True
+    # clip_type 'complete' of the GEAR presets (clip_input_domain): each
+    # domain stores its parent's planes as cuts, its bound is concretized
+    # over the box and those cuts, and children are clipped by the parent's
+    # planes (batch_branch_and_bound.py:420-428, 523-534; presets
+    # gear-f0.yaml:466-472); at most clip_max_rows binding cuts per domain
+    # (concretize_max_constraint_rows).  "none": box concretization only.
+    clip: str = "none"
+    clip_max_rows: int = 16
     alm: ALMConfig = field(default_factory=ALMConfig)
 
 
@@ -74,7 +82,20 @@ class QCQPBranchAndBound:
         self.cfg = cfg or BaBConfig()
         self.net = BoundedQCQP(problem, device, precision=precision)
         self.device = self.net.device
-        self.frontier = DeviceFrontier(problem.n, problem.S, self.cfg.cap, self.device)
+        if self.cfg.clip not in ("none", "complete"):
+            raise ValueError(f"clip {self.cfg.clip!r}: 'none' or 'complete'")
+        self.clip = self.cfg.clip == "complete"
+        if self.clip and (not self.cfg.reorder or self.net.large or problem.n > 64):
+            raise ValueError("clip 'complete' runs in the reordered loop with n <= 64")
+        n, S = problem.n, problem.S
+        self.frontier = DeviceFrontier(n, S, self.cfg.cap, self.device,
+                                       payload=S * (n + 1) if self.clip else 0)
+        # the cut rows of a domain without any: inactive (0 x - 1 <= 0)
+        self._no_cuts = None
+        if self.clip:
+            nc = torch.zeros(1, S * (n + 1), dtype=torch.float64, device=self.device)
+            nc[0, S * n:] = -1.0
+            self._no_cuts = nc
         self.alm = ALMPGDSolver(self.cfg.alm)
         self.P = float("inf")
         self.x_best = None
@@ -105,7 +126,7 @@ class QCQPBranchAndBound:
         if alm:
             self.run_alm(L, U)
         lb = self.net.bound(L, U)
-        self.frontier.filter_append(lb, self.thr, L, U)
+        self.frontier.filter_append(lb, self.thr, L, U, payload=self._no_cuts)
         self.update_dual()
 
     def presplit(self, leaves, alm=True):
@@ -121,7 +142,7 @@ class QCQPBranchAndBound:
             idx = input_split_branching(self.net, None, L, U, None, None, "naive")
             L, U, _ = input_split_parallel(L, U, i_idx=idx)
         lb = self.net.bound(L, U)
-        self.frontier.filter_append(lb, self.thr, L, U)
+        self.frontier.filter_append(lb, self.thr, L, U, payload=self._no_cuts)
         self.stats.boxes_bounded += L.shape[0]
         self.update_dual()
 
@@ -185,23 +206,49 @@ class QCQPBranchAndBound:
         pL, pU, pkey = f.pick_out(nb)
         large = getattr(self.net, "large", False)
         lp = self.cfg.dual_lp_iters > 0 and self.P < float("inf")
+        S, n = self.p.S, self.p.n
         if large:
             lb = self.net.bound(pL, pU)
             lA = lbias = None
         else:
-            # the LP reads every row's plane, the child keys the objective's
+            # the LP and the cuts read every row's plane, the child keys the
+            # objective's
             lb, lA, lbias = self.net.bound(pL, pU, return_A=True,
-                                           plane_rows=self.p.S if lp else 1)
+                                           plane_rows=S if (lp or self.clip) else 1)
+        cuts = None
+        if self.clip:
+            from .concretize import clip_domains, constrained_bound
+            stored = f.pick_payload(nb)
+            constrained_bound(lA, lbias, stored[:, :S * n].view(nb, S, n), stored[:, S * n:],
+                              pL, pU, lb, max_rows=self.cfg.clip_max_rows)
+            # the domains' planes become their children's cuts
+            # (construct_cut_constraints, input_split/utils.py:113-129)
+            cuts = self.cut_rows(lA, lbias)
         lb = torch.maximum(lb, pkey)
         if lA is not None and lp:
             self.raise_by_lp(lb, lA, lbias, pL, pU)
         idx = input_split_branching(self.net, None, pL, pU, None, None, "naive")
         cL, cU, _ = input_split_parallel(pL, pU, i_idx=idx)
+        if self.clip:
+            clip_domains(lA, lbias, self.thr, cL, cU, out=(cL, cU))
         keys = child_keys(lb, lA, lbias, cL, cU) if lA is not None else torch.cat([lb, lb])
-        f.filter_append(keys, self.thr, cL, cU, sync=sync)
+        f.filter_append(keys, self.thr, cL, cU, sync=sync, payload=cuts)
         self.stats.iterations += 1
         self.stats.boxes_bounded += nb
         return nb
+
+    def cut_rows(self, lA, lbias):
+        """Cut rows (B, S (n + 1)) lA x + (lbias - thr) <= 0 of planes
+        lA (B, S, n), lbias (B, S); a row with thr = +inf (the objective
+        before an incumbent exists) is the inactive 0 x - 1 <= 0
+        (construct_cut_constraints, complete_verifier/input_split/utils.py:
+        113-129)."""
+        B, S, n = lA.shape
+        out = torch.empty(B, S * (n + 1), dtype=torch.float64, device=lA.device)
+        act = torch.isfinite(self.thr)
+        out[:, :S * n].view(B, S, n).copy_(torch.where(act[None, :, None], lA, 0.0))
+        out[:, S * n:].copy_(torch.where(act[None, :], lbias - self.thr, -1.0))
+        return out
 
     def raise_by_lp(self, lb, lA, lbias, cL, cU):
         """Solving-mode merge of the LP certificate into the objective row

@@ -44,19 +44,24 @@ def constrained_concretize(obj, A, bc, x_L, x_U, max_rows=16, stream=None):
 def constrained_bound(lA, lbias, A, bc, x_L, x_U, lb, max_rows=16, stream=None):
     """In place on lb (B, H): max(lb, lbias + the constrained root of lA),
     +inf on every row of a box a cut misses (gh_constrained_bound_f64).
-    Returns (lb, flags, kept)."""
+    A (B, K, X) and bc (B, K) need contiguous rows only (a box stride of
+    their own, e.g. views into the frontier's per-domain records).  Returns
+    (lb, flags, kept)."""
     B, H, X = lA.shape
     K = A.shape[1]
     if X > MAX_X:
         raise ValueError(f"constrained concretization supports X <= {MAX_X}")
-    for t in (lA, lbias, A, bc, x_L, x_U, lb):
+    for t in (lA, lbias, x_L, x_U, lb):
         if not t.is_contiguous():
-            raise ValueError("constrained_bound takes contiguous tensors")
+            raise ValueError("constrained_bound takes contiguous planes, boxes and bounds")
+    if A.stride(2) != 1 or A.stride(1) != X or bc.stride(1) != 1:
+        raise ValueError("cut rows must be contiguous inside each box")
     flags = torch.empty(B, dtype=torch.int32, device=lA.device)
     kept = torch.empty(B, dtype=torch.int32, device=lA.device)
-    check(lib().gh_constrained_bound_f64(B, H, K, X, ptr(lA), ptr(lbias), ptr(A), ptr(bc),
-                                         ptr(x_L), ptr(x_U), int(max_rows), ptr(lb), ptr(flags),
-                                         ptr(kept), stream_ptr(stream)),
+    check(lib().gh_constrained_bound_f64(B, H, K, X, ptr(lA), ptr(lbias), ptr(A), A.stride(0),
+                                         ptr(bc), bc.stride(0), ptr(x_L), ptr(x_U),
+                                         int(max_rows), ptr(lb), ptr(flags), ptr(kept),
+                                         stream_ptr(stream)),
           "gh_constrained_bound_f64")
     return lb, flags, kept
 

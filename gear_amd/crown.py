@@ -220,14 +220,19 @@ def child_keys(plb, lA, lbias, cL, cU, stream=None):
 class DeviceFrontier:
     """Device-resident domain storage with stack order (pop from the end),
     as StackTensorStorage (complete_verifier/tensor_storage.py:351-406).
-    Rows: x_L, x_U (n), lb (S)."""
+    Rows: x_L, x_U (n), lb (S), and `payload` more doubles per domain when
+    asked for (the cut rows that clip_type 'complete' stores with each
+    domain)."""
 
-    def __init__(self, n, S, cap, device="cuda"):
+    def __init__(self, n, S, cap, device="cuda", payload=0):
         self.n, self.S, self.cap = n, S, int(cap)
         self.device = torch.device(device)
         self.L = torch.empty(self.cap, n, dtype=torch.float64, device=self.device)
         self.U = torch.empty_like(self.L)
         self.lb = torch.empty(self.cap, S, dtype=torch.float64, device=self.device)
+        self.W = int(payload)
+        self.pay = (torch.empty(self.cap, self.W, dtype=torch.float64, device=self.device)
+                    if self.W else None)
         self.count = 0
         self.total = torch.zeros(1, dtype=torch.int64, device=self.device)
         self.disc_min = torch.full((1,), float("inf"), dtype=torch.float64, device=self.device)
@@ -263,6 +268,8 @@ class DeviceFrontier:
         self.L[:c] = self.L[:c].index_select(0, order)
         self.U[:c] = self.U[:c].index_select(0, order)
         self.lb[:c] = self.lb[:c].index_select(0, order)
+        if self.pay is not None:
+            self.pay[:c] = self.pay[:c].index_select(0, order)
 
     def pick_out(self, batch):
         """Views of the last `batch` rows (the parents of the next split);
@@ -272,16 +279,21 @@ class DeviceFrontier:
         self.count = s
         return self.L[s:s + nb], self.U[s:s + nb], self.lb[s:s + nb]
 
+    def pick_payload(self, nb):
+        """The payload rows of the domains the last pick_out(nb) returned."""
+        return self.pay[self.count:self.count + nb]
+
     def workspace(self, B):
         need = int(lib().gh_filter_workspace_bytes(B))
         if self._ws is None or self._ws.numel() < need:
             self._ws = torch.empty(need, dtype=torch.uint8, device=self.device)
         return self._ws
 
-    def filter_append(self, lb, thr, x_L, x_U, sync=True):
+    def filter_append(self, lb, thr, x_L, x_U, sync=True, payload=None):
         """Prune boxes verified against thresholds thr (S,) and append the
-        survivors in order.  Returns the survivor count (a host sync) when
-        sync is set, else None (call finish_append later)."""
+        survivors in order.  payload (P, W): box k carries row k % P (needed
+        when the frontier holds a payload).  Returns the survivor count (a
+        host sync) when sync is set, else None (call finish_append later)."""
         B = x_L.shape[0]
         ws = self.workspace(B)
         check(lib().gh_filter_append_f64(B, self.S, self.n, ptr(lb), ptr(thr), ptr(x_L),
@@ -289,6 +301,13 @@ class DeviceFrontier:
                                          self.count, self.cap, ptr(self.total),
                                          ptr(self.disc_min), ptr(ws), stream_ptr()),
               "gh_filter_append_f64")
+        if self.pay is not None:
+            if payload is None or payload.shape[1] != self.W:
+                raise ValueError(f"this frontier stores a payload of {self.W} per domain")
+            payload = payload.contiguous()
+            check(lib().gh_filter_payload_f64(B, self.W, payload.shape[0], ptr(payload), ptr(ws),
+                                              ptr(self.pay), self.count, self.cap, stream_ptr()),
+                  "gh_filter_payload_f64")
         if sync:
             return self.finish_append()
         return None

@@ -207,6 +207,18 @@ __global__ __launch_bounds__(256) void min_strided_kernel(int64_t N, int64_t str
 
 __global__ void set_i64_kernel(int64_t* p, int64_t v) { *p = v; }
 
+// Payload row of each survivor (row k % P of pay) to its frontier row.
+__global__ void filter_payload_kernel(int B, int W, int P, const double* __restrict__ pay,
+                                      const int32_t* __restrict__ dst, double* __restrict__ Fpay,
+                                      int64_t base)
+{
+    size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)B * W) return;
+    int k = (int)(t / W), w = (int)(t - (size_t)k * W);
+    int r = dst[k];
+    if (r >= 0) Fpay[(size_t)(base + r) * W + w] = pay[(size_t)(k % P) * W + w];
+}
+
 inline int status() { return hipGetLastError() == hipSuccess ? GH_OK : GH_ELAUNCH; }
 
 }  // namespace
@@ -265,6 +277,18 @@ extern "C" int gh_filter_append_f64(int B, int S, int n, const double* lb, const
     size_t items = (size_t)B * (n + S);
     hipLaunchKernelGGL(filter_copy_kernel, dim3((unsigned)((items + 255) / 256)), dim3(256), 0,
                        stream, B, S, n, lb, cL, cU, dst, FL, FU, Flb, base);
+    return status();
+}
+
+extern "C" int gh_filter_payload_f64(int B, int W, int P, const double* pay, const void* ws,
+                                     double* Fpay, int64_t base, int64_t cap, hipStream_t stream)
+{
+    if (B < 0 || W < 0 || P < 1 || base < 0) return GH_EINVAL;
+    if (B == 0 || W == 0) return GH_OK;
+    if (!pay || !ws || !Fpay || base + (int64_t)B > cap) return GH_EINVAL;
+    size_t items = (size_t)B * W;
+    hipLaunchKernelGGL(filter_payload_kernel, dim3((unsigned)((items + 255) / 256)), dim3(256), 0,
+                       stream, B, W, P, pay, (const int32_t*)ws, Fpay, base);
     return status();
 }
 

@@ -150,8 +150,9 @@ __device__ double solve_beta(int X, int lane, double c, double a, double d, doub
 __global__ __launch_bounds__(64 * kWaves) void constrained_concretize_kernel(
     int H, int K, int X, const double* __restrict__ obj, const double* __restrict__ A,
     const double* __restrict__ b, const double* __restrict__ xL, const double* __restrict__ xU,
-    int max_rows, int Kc, double* __restrict__ root, int32_t* __restrict__ flags,
-    int32_t* __restrict__ nkept, const double* __restrict__ bias, double* __restrict__ lb_io)
+    int64_t lda, int64_t ldb, int max_rows, int Kc, double* __restrict__ root,
+    int32_t* __restrict__ flags, int32_t* __restrict__ nkept, const double* __restrict__ bias,
+    double* __restrict__ lb_io)
 {
     extern __shared__ double lds[];
     double* rows = lds;                    // Kc x 64
@@ -172,8 +173,8 @@ __global__ __launch_bounds__(64 * kWaves) void constrained_concretize_kernel(
 
     // ---- cut rows: validity, sort-out on the box, centred offset, redundancy
     for (int k = wv; k < K; k += kWaves) {
-        const double ai = lane < X ? A[((size_t)bx * K + k) * X + lane] : 0.0;
-        double bk = b[(size_t)bx * K + k];
+        const double ai = lane < X ? A[(size_t)bx * lda + (size_t)k * X + lane] : 0.0;
+        double bk = b[(size_t)bx * ldb + k];
         const bool valid = isfinite(bk) && __ballot(lane < X && !isfinite(ai)) == 0;
         const double row = valid ? ai : 0.0;
         if (!valid) bk = -1.0;
@@ -228,7 +229,7 @@ __global__ __launch_bounds__(64 * kWaves) void constrained_concretize_kernel(
     const int nk = sh[2];
     for (int idx = threadIdx.x; idx < nk * 64; idx += 64 * kWaves) {
         const int t = idx >> 6, i = idx & 63;
-        rows[idx] = i < X ? A[((size_t)bx * K + keep[t]) * X + i] : 0.0;
+        rows[idx] = i < X ? A[(size_t)bx * lda + (size_t)keep[t] * X + i] : 0.0;
     }
     const bool infeasible = sh[0] != 0;
     const bool covered = K == 0 || sh[1] == 0;
@@ -295,11 +296,12 @@ __global__ __launch_bounds__(64 * kWaves) void constrained_concretize_kernel(
 }
 
 int launch_constrained(int B, int H, int K, int X, const double* obj, const double* A,
-                       const double* b, const double* xL, const double* xU, int max_rows,
-                       double* root, int32_t* flags, int32_t* nkept, const double* bias,
-                       double* lb_io, hipStream_t stream)
+                       int64_t lda, const double* b, int64_t ldb, const double* xL,
+                       const double* xU, int max_rows, double* root, int32_t* flags,
+                       int32_t* nkept, const double* bias, double* lb_io, hipStream_t stream)
 {
     if (B < 0 || H < 1 || K < 0 || X < 1) return GH_EINVAL;
+    if (K > 0 && (lda < (int64_t)K * X || ldb < K)) return GH_EINVAL;
     if (X > 64 || K > 256) return GH_EUNSUPPORTED;
     const int Kc = max_rows > 0 ? (K < max_rows ? K : max_rows) : K;
     if (Kc > 64) return GH_EUNSUPPORTED;
@@ -308,7 +310,8 @@ int launch_constrained(int B, int H, int K, int X, const double* obj, const doub
     const size_t lds = (size_t)Kc * 64 * sizeof(double) + 2 * (size_t)K * sizeof(double) +
                        ((size_t)K + Kc + 3) * sizeof(int);
     hipLaunchKernelGGL(constrained_concretize_kernel, dim3(B), dim3(64 * kWaves), lds, stream, H,
-                       K, X, obj, A, b, xL, xU, max_rows, Kc, root, flags, nkept, bias, lb_io);
+                       K, X, obj, A, b, xL, xU, lda, ldb, max_rows, Kc, root, flags, nkept, bias,
+                       lb_io);
     return hipGetLastError() == hipSuccess ? GH_OK : GH_ELAUNCH;
 }
 
@@ -320,19 +323,19 @@ extern "C" int gh_constrained_concretize_f64(int B, int H, int K, int X, const d
                                              int32_t* flags, int32_t* nkept, hipStream_t stream)
 {
     if (B > 0 && !root) return GH_EINVAL;
-    return launch_constrained(B, H, K, X, obj, A, b, xL, xU, max_rows, root, flags, nkept,
-                              nullptr, nullptr, stream);
+    return launch_constrained(B, H, K, X, obj, A, (int64_t)K * X, b, K, xL, xU, max_rows, root,
+                              flags, nkept, nullptr, nullptr, stream);
 }
 
 extern "C" int gh_constrained_bound_f64(int B, int H, int K, int X, const double* lA,
-                                        const double* lbias, const double* A, const double* b,
-                                        const double* xL, const double* xU, int max_rows,
-                                        double* lb, int32_t* flags, int32_t* nkept,
-                                        hipStream_t stream)
+                                        const double* lbias, const double* A, int64_t lda,
+                                        const double* b, int64_t ldb, const double* xL,
+                                        const double* xU, int max_rows, double* lb,
+                                        int32_t* flags, int32_t* nkept, hipStream_t stream)
 {
     if (B > 0 && (!lbias || !lb)) return GH_EINVAL;
-    return launch_constrained(B, H, K, X, lA, A, b, xL, xU, max_rows, nullptr, flags, nkept,
-                              lbias, lb, stream);
+    return launch_constrained(B, H, K, X, lA, A, lda, b, ldb, xL, xU, max_rows, nullptr, flags,
+                              nkept, lbias, lb, stream);
 }
 
 // ------------------------------------------------------------ clipping

@@ -228,12 +228,15 @@ int gh_constrained_concretize_f64(int B, int H, int K, int X, const double *obj,
  * (auto_LiRPA/concretize_bounds.py:280-304)), and +inf on every row of a
  * box that a cut misses (input_split/bounding.py:345-350).  lb (B, H) holds
  * the box bounds on entry (e.g. gh_qcqp_crown_f64's lb) and the result on
- * return; lA (B, H, X), lbias (B, H); the rest as above.
+ * return; lA (B, H, X), lbias (B, H); box b's cuts are A + b lda
+ * ((K, X) row-major) and bc + b ldb (K) (lda >= K X, ldb >= K: the cut rows
+ * may sit inside a wider per-domain record); the rest as above.
  */
 int gh_constrained_bound_f64(int B, int H, int K, int X, const double *lA,
-                             const double *lbias, const double *A, const double *bc,
-                             const double *xL, const double *xU, int max_rows, double *lb,
-                             int32_t *flags, int32_t *nkept, hipStream_t stream);
+                             const double *lbias, const double *A, int64_t lda,
+                             const double *bc, int64_t ldb, const double *xL,
+                             const double *xU, int max_rows, double *lb, int32_t *flags,
+                             int32_t *nkept, hipStream_t stream);
 
 /*
  * Domain clipping: shrink each box [xL[b], xU[b]] along every coordinate to
@@ -301,6 +304,18 @@ int gh_filter_append_f64(int B, int S, int n, const double *lb,
                          double *Flb, int64_t base, int64_t cap,
                          int64_t *total, double *disc_min, void *ws,
                          hipStream_t stream);
+
+/*
+ * Companion of gh_filter_append_f64 for domains that carry a payload (the
+ * cut rows of clip_type 'complete', stored with each domain by
+ * UnsortedInputDomainList.add, complete_verifier/input_split/
+ * branching_domains.py:560-830): called right after gh_filter_append_f64
+ * with the same B, ws and base, it copies the payload of each survivor k,
+ * row k % P of pay (P, W) (the children of a split share their parent's),
+ * to row base + rank of Fpay (cap, W).
+ */
+int gh_filter_payload_f64(int B, int W, int P, const double *pay, const void *ws,
+                          double *Fpay, int64_t base, int64_t cap, hipStream_t stream);
 
 /*
  * out (device double, in/out) = min(out, min_b x[b * stride]) over
