@@ -60,6 +60,9 @@ def parse():
     ap.add_argument("--serve-rate", type=float, default=8.0, help="arrivals per second")
     ap.add_argument("--serve-plain-budget", type=float, default=60.0,
                     help="seconds the plain sequential form may run")
+    ap.add_argument("--no-clip", action="store_true", help="skip the clip_type 'complete' section")
+    ap.add_argument("--clip-only", action="store_true",
+                    help="only the clip_type 'complete' section (kernels and time to gap)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device 5 times (a known-size "
                          "reference for the HBM byte counters) and exit")
@@ -276,6 +279,114 @@ def bench_serving(requests=24, rate=8.0, plain_budget=60.0, plain=True):
     return out
 
 
+def _tight_qcqp():
+    """A small QCQP with binding constraints (rhs < 0) and a nonconvex
+    objective: the instance where the constrained bound and clipping
+    matter (seeded, default_rng(8))."""
+    import numpy as np
+    from gear_amd.problems import QCQP
+    rng = np.random.default_rng(8)
+    n, m = 6, 3
+    A = rng.standard_normal((n, n))
+    Q = np.zeros((m + 1, n, n))
+    Q[0] = A @ A.T / n - 0.5 * np.eye(n)
+    c = rng.standard_normal((m + 1, n))
+    rhs = np.concatenate([[0.0], -np.abs(rng.standard_normal(m)) * 0.3])
+    return QCQP(Q=Q, c=c, d=np.zeros(m + 1), rhs=rhs, xl=-np.ones(n), xu=np.ones(n))
+
+
+def bench_clip(boxes=32768, reps=10):
+    """clip_type 'complete': (1) the constrained bound and the clip kernels
+    on one B&B step's worth of n = 50, m = 20 domains whose cuts are their
+    parents' CROWN planes; (2) time and domains to the 1e-4 gap on a tight
+    instance with and without it."""
+    import numpy as np
+    import torch
+    from gear_amd.bab import BaBConfig, QCQPBranchAndBound
+    from gear_amd.concretize import clip_domains, constrained_bound
+    from gear_amd.crown import BoundedQCQP, input_split_branching, input_split_parallel
+    from gear_amd.problems import config2_problem, random_bisection_boxes
+    dev = torch.device("cuda:0")
+    p = config2_problem()
+    n, S = p.n, p.S
+    net = BoundedQCQP(p, dev)
+    L, U = random_bisection_boxes(p.xl, p.xu, boxes // 2, seed=5)
+    pL, pU = torch.as_tensor(L, device=dev), torch.as_tensor(U, device=dev)
+    _, plA, plbias = net.bound(pL, pU, return_A=True)
+    # an incumbent at the 30th percentile of the parents' objective bounds
+    # makes the objective row a cut too
+    thr = torch.as_tensor(np.concatenate([[0.0], p.rhs[1:] + 1e-6]), device=dev)
+    lb_p = net.bound(pL, pU)
+    thr[0] = torch.quantile(lb_p[:, 0], 0.3)
+    idx = input_split_branching(None, None, pL, pU, None, None, "naive")
+    cL, cU, _ = input_split_parallel(pL, pU, i_idx=idx)
+    K = cL.shape[0] // 2
+    par = torch.arange(2 * K, device=dev) % K
+    cutA = plA[par].contiguous()
+    act = torch.isfinite(thr)
+    cutb = torch.where(act[None, :], plbias[par] - thr, -1.0).contiguous()
+    lb, lA, lbias = net.bound(cL, cU, return_A=True)
+    stream = torch.cuda.current_stream()
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps * 1e-3
+
+    lb_c = lb.clone()
+    _, flags, kept = constrained_bound(lA, lbias, cutA, cutb, cL, cU, lb_c)
+    t_cb = timed(lambda: constrained_bound(lA, lbias, cutA, cutb, cL, cU, lb.clone()))
+    t_box = timed(lambda: net.bound(cL, cU, return_A=True))
+    oL, oU = cL.clone(), cU.clone()
+    _, _, st = clip_domains(plA, plbias, thr, cL, cU, out=(oL, oU))
+    t_clip = timed(lambda: clip_domains(plA, plbias, thr, cL, cU, out=(oL, oU)))
+    shrunk = int((((oL > cL).any(1)) | ((oU < cU).any(1))).sum())
+    raised = int((lb_c[:, 0] > lb[:, 0]).sum())
+    B = cL.shape[0]
+    cut_bytes = S * (n + 1) * 8
+    kernels = {
+        "domains": B, "n": n, "cuts_per_domain": S,
+        "constrained_bound": {"ms": round(t_cb * 1e3, 3), "domains_per_s": round(B / t_cb, 1),
+                              "cuts_kept_mean": round(float(kept.float().mean()), 2),
+                              "objective_bounds_raised": raised,
+                              "infeasible_domains": int((flags & 1).sum()),
+                              "bytes_per_domain": S * n * 8 + S * 8 + cut_bytes + 2 * n * 8 + S * 8,
+                              "bound": "VALU (sort network and FP64 compare per cut and row)"},
+        "crown_planes_same_domains": {"ms": round(t_box * 1e3, 3),
+                                      "domains_per_s": round(B / t_box, 1)},
+        "clip": {"ms": round(t_clip * 1e3, 3), "domains_per_s": round(B / t_clip, 1),
+                 "shrunk": shrunk, "emptied": int((st >= 2).sum()),
+                 "bytes_per_domain": 2 * 2 * n * 8},
+    }
+    # time and domains to the 1e-4 gap on the tight instance
+    tight = _tight_qcqp()
+    solve = {}
+    for clip in ("none", "complete"):
+        bb = QCQPBranchAndBound(tight, BaBConfig(batch=256, cap=1 << 16, clip=clip,
+                                                 max_iters=400, time_limit=15.0), device=dev)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        bb.solve()
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        from gear_amd.bab import is_primal_dual_converged
+        solve[clip] = {"P": bb.P, "D": bb.D,
+                       "converged": bool(is_primal_dual_converged(bb.P, bb.D, 1e-5, 1e-4)),
+                       "domains_bounded": bb.stats.boxes_bounded,
+                       "iterations": bb.stats.iterations, "stop": bb.stats.stop,
+                       "wall_s": round(wall, 3)}
+    return {"kernels": kernels, "time_to_gap_tight_n6_m3": solve,
+            "note": "clip_type 'complete' (the GEAR presets' clip_input_domain): domains carry "
+                    "their parent's planes as cuts; the time-to-gap runs use the same loop, "
+                    "batch 256, iteration cap 400, time limit 15 s"}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -289,6 +400,9 @@ def main():
         return
     if args.large_only:
         print(json.dumps(bench_precision_large(args.large_boxes, cpu=not args.no_cpu)), flush=True)
+        return
+    if args.clip_only:
+        print(json.dumps(bench_clip()), flush=True)
         return
     if args.serve_only:
         print(json.dumps(bench_serving(args.serve_requests, args.serve_rate,
@@ -339,8 +453,7 @@ def main():
     sample_parents = None
     if len(bb.frontier) >= args.batch:
         s0 = len(bb.frontier) - args.batch
-        sample_parents = (bb.frontier.L[s0:s0 + args.batch].clone(),
-                          bb.frontier.U[s0:s0 + args.batch].clone())
+        sample_parents = bb.frontier.rows(s0, s0 + args.batch)[:2]
     import gc
     gc.collect()
     gc.disable()    # no collector pause inside the timed steps
@@ -509,6 +622,8 @@ def main():
         result["uneven_dag"] = bench_uneven_dag(cpu=not args.no_cpu)
     if not args.no_large:
         result["precision_large"] = bench_precision_large(args.large_boxes, cpu=not args.no_cpu)
+    if not args.no_clip:
+        result["clip_complete"] = bench_clip()
     if not args.no_serve:
         result["serving"] = bench_serving(args.serve_requests, args.serve_rate,
                                           args.serve_plain_budget, plain=not args.no_plain)

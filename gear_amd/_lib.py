@@ -42,11 +42,14 @@ SIGNATURES = {
     "gh_alm_pgd_f64": [_I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P],
     "gh_lp_dual_bound_f64": [_I, _I, _I, _P, _P, _P, _P, _P, _I, _P, _P, _P],
     "gh_child_keys_f64": [_I, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P],
+    "gh_branch_split_keys_f64": [_I, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P],
     "gh_constrained_concretize_f64": [_I, _I, _I, _I, _P, _P, _P, _P, _P, _I, _P, _P, _P, _P],
     "gh_constrained_bound_f64": [_I, _I, _I, _I, _P, _P, _P, ctypes.c_int64, _P, ctypes.c_int64,
                                  _P, _P, _I, _P, _P, _P, _P],
     "gh_clip_domains_f64": [_I, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P],
     "gh_filter_payload_f64": [_I, _I, _I, _P, _P, _P, ctypes.c_int64, ctypes.c_int64, _P],
+    "gh_frontier_keys_f64": [ctypes.c_int64, _I, _P, _P, _P, _P, _P],
+    "gh_min_rows_f64": [ctypes.c_int64, ctypes.c_int64, _P, _P, _P, _P],
     "gh_dag_plan_create": [_I, _P, ctypes.POINTER(_P)],
     "gh_dag_plan_destroy": [_P],
     "gh_dag_plan_info": [_P, _P, _P, _P],

@@ -20,8 +20,8 @@ from dataclasses import dataclass, field
 import torch
 
 from .alm import ALMConfig, ALMPGDSolver, replay_float64
-from .crown import (BoundedQCQP, DeviceFrontier, child_keys, input_split_branching,
-                    input_split_parallel)
+from .crown import (BoundedQCQP, DeviceFrontier, branch_split_keys, child_keys,
+                    input_split_branching, input_split_parallel)
 
 
 def is_primal_dual_converged(P, D, abs_tol=1e-5, rel_tol=0.0):
@@ -71,6 +71,7 @@ class BaBStats:
     boxes_bounded: int = 0
     bound_time: float = 0.0
     wall: float = 0.0
+    stop: str = ""               # converged, empty, iterations, time, frontier_full
     history: list = field(default_factory=list)
 
 
@@ -227,11 +228,13 @@ class QCQPBranchAndBound:
         lb = torch.maximum(lb, pkey)
         if lA is not None and lp:
             self.raise_by_lp(lb, lA, lbias, pL, pU)
-        idx = input_split_branching(self.net, None, pL, pU, None, None, "naive")
-        cL, cU, _ = input_split_parallel(pL, pU, i_idx=idx)
         if self.clip:
+            idx = input_split_branching(self.net, None, pL, pU, None, None, "naive")
+            cL, cU, _ = input_split_parallel(pL, pU, i_idx=idx)
             clip_domains(lA, lbias, self.thr, cL, cU, out=(cL, cU))
-        keys = child_keys(lb, lA, lbias, cL, cU) if lA is not None else torch.cat([lb, lb])
+            keys = child_keys(lb, lA, lbias, cL, cU)
+        else:
+            cL, cU, keys = branch_split_keys(lb, pL, pU, lA, lbias)
         f.filter_append(keys, self.thr, cL, cU, sync=sync, payload=cuts)
         self.stats.iterations += 1
         self.stats.boxes_bounded += nb
@@ -267,20 +270,28 @@ class QCQPBranchAndBound:
         cfg = self.cfg
         t0 = time.time()
         self.root()
+        self.stats.stop = "iterations"
         while len(self.frontier) > 0 and self.stats.iterations < cfg.max_iters:
             if is_primal_dual_converged(self.P, self.D, cfg.abs_tol, cfg.rel_tol):
+                self.stats.stop = "converged"
                 break
             if time.time() - t0 > cfg.time_limit:
+                self.stats.stop = "time"
                 break
             if cfg.alm_interval and self.stats.iterations % cfg.alm_interval == 0 and \
                     self.stats.iterations > 0:
                 k = min(len(self.frontier), 64)
                 s = len(self.frontier) - k
-                self.run_alm(self.frontier.L[s:s + k], self.frontier.U[s:s + k], restarts=8)
-            self.step()
+                wL, wU = self.frontier.rows(s, s + k)[:2]
+                self.run_alm(wL, wU, restarts=8)
+            if self.step() == 0:
+                # no room for one batch of children: the frontier is full
+                self.stats.stop = "frontier_full"
+                break
             self.update_dual()
             self.stats.history.append((time.time() - t0, self.P, self.D, len(self.frontier)))
         self.stats.wall = time.time() - t0
         if len(self.frontier) == 0:
+            self.stats.stop = "empty"
             self.D = min(self.P, float(self.frontier.disc_min.item()))
         return self

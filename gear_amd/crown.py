@@ -217,39 +217,126 @@ def child_keys(plb, lA, lbias, cL, cU, stream=None):
     return keys
 
 
+def branch_split_keys(plb, pL, pU, lA=None, lbias=None, stream=None):
+    """Widest branching, bisection and child keys of K parents in one kernel
+    (gh_branch_split_keys_f64): (cL, cU (2K, n), keys (2K, S)), equal to
+    input_split_branching + input_split_parallel + child_keys."""
+    K, S = plb.shape
+    n = pL.shape[1]
+    R = 1
+    if lA is not None:
+        R = lA.shape[1]
+        if lA.shape != (K, R, n) or lbias.shape != (K, R) or not lA.is_contiguous() \
+                or not lbias.is_contiguous():
+            raise ValueError("planes must be contiguous (K, R, n) and (K, R)")
+    cL = torch.empty(2 * K, n, dtype=torch.float64, device=pL.device)
+    cU = torch.empty_like(cL)
+    keys = torch.empty(2 * K, S, dtype=torch.float64, device=pL.device)
+    check(lib().gh_branch_split_keys_f64(K, n, S, R, ptr(pL.contiguous()), ptr(pU.contiguous()),
+                                         ptr(plb.contiguous()), ptr(lA), ptr(lbias), ptr(cL),
+                                         ptr(cU), ptr(keys), stream_ptr(stream)),
+          "gh_branch_split_keys_f64")
+    return cL, cU, keys
+
+
 class DeviceFrontier:
     """Device-resident domain storage with stack order (pop from the end),
     as StackTensorStorage (complete_verifier/tensor_storage.py:351-406).
-    Rows: x_L, x_U (n), lb (S), and `payload` more doubles per domain when
-    asked for (the cut rows that clip_type 'complete' stores with each
-    domain)."""
 
-    def __init__(self, n, S, cap, device="cuda", payload=0):
+    A domain's record -- x_L, x_U (n), lb (S), and `payload` more doubles
+    when asked for (the cut rows that clip_type 'complete' stores with each
+    domain) -- stays in one physical row from its append until it is picked
+    or pruned.  The stack order is a separate logical array of int64 slot
+    ids, so re-sorting the frontier moves 8 bytes per domain, not the
+    record.  Appends go to a physical high-water mark; when the slack above
+    it runs out, the live records are compacted in logical order.  `L`,
+    `U`, `lb`, `pay` and `rows()` give logical-order copies."""
+
+    def __init__(self, n, S, cap, device="cuda", payload=0, slack=None):
         self.n, self.S, self.cap = n, S, int(cap)
         self.device = torch.device(device)
-        self.L = torch.empty(self.cap, n, dtype=torch.float64, device=self.device)
-        self.U = torch.empty_like(self.L)
-        self.lb = torch.empty(self.cap, S, dtype=torch.float64, device=self.device)
         self.W = int(payload)
-        self.pay = (torch.empty(self.cap, self.W, dtype=torch.float64, device=self.device)
-                    if self.W else None)
-        self.count = 0
+        # physical rows: cap plus the slack that lets appends wait for a
+        # compaction (children are appended before their parents' rows are
+        # reclaimed)
+        self.phys = self.cap + int(slack if slack is not None else max(self.cap // 2, 4096))
+        self._alloc(self.phys)
+        self.order = torch.empty(self.cap, dtype=torch.int64, device=self.device)
+        self._iota = torch.arange(self.cap, dtype=torch.int64, device=self.device)
+        self.count = 0          # logical rows
+        self.hw = 0             # physical rows in use (live or picked)
         self.total = torch.zeros(1, dtype=torch.int64, device=self.device)
         self.disc_min = torch.full((1,), float("inf"), dtype=torch.float64, device=self.device)
         self._ws = None
+        self._keys = None
+        self._pending = None
+
+    def _alloc(self, rows):
+        d = dict(dtype=torch.float64, device=self.device)
+        self._L = torch.empty(rows, self.n, **d)
+        self._U = torch.empty(rows, self.n, **d)
+        self._lb = torch.empty(rows, self.S, **d)
+        self._pay = torch.empty(rows, self.W, **d) if self.W else None
 
     def __len__(self):
         return self.count
+
+    # -- logical views (copies), for callers that read rows by position
+    def rows(self, start=0, end=None):
+        """(x_L, x_U, lb[, payload]) of logical rows start..end, copies."""
+        end = self.count if end is None else min(end, self.count)
+        idx = self.order[start:end]
+        out = (self._L.index_select(0, idx), self._U.index_select(0, idx),
+               self._lb.index_select(0, idx))
+        if self._pay is not None:
+            out = out + (self._pay.index_select(0, idx),)
+        return out
+
+    @property
+    def L(self):
+        return self.rows()[0]
+
+    @property
+    def U(self):
+        return self.rows()[1]
+
+    @property
+    def lb(self):
+        return self.rows()[2]
+
+    @property
+    def pay(self):
+        return self.rows()[3] if self._pay is not None else None
 
     def push(self, x_L, x_U, lb):
         """Unconditional append (root)."""
         B = x_L.shape[0]
         if self.count + B > self.cap:
             raise RuntimeError("frontier capacity exceeded")
-        self.L[self.count:self.count + B] = x_L
-        self.U[self.count:self.count + B] = x_U
-        self.lb[self.count:self.count + B] = lb
+        self._room(B)
+        self._L[self.hw:self.hw + B] = x_L
+        self._U[self.hw:self.hw + B] = x_U
+        self._lb[self.hw:self.hw + B] = lb
+        torch.add(self._iota[:B], self.hw, out=self.order[self.count:self.count + B])
         self.count += B
+        self.hw += B
+
+    def _room(self, B):
+        """Make B physical rows free above the high-water mark, compacting
+        the live records in logical order when the slack has run out."""
+        if self.hw + B <= self.phys:
+            return
+        live = self.order[:self.count]
+        old = (self._L, self._U, self._lb, self._pay)
+        self._alloc(self.phys)
+        for src, dst in zip(old, (self._L, self._U, self._lb, self._pay)):
+            if src is not None:
+                torch.index_select(src, 0, live, out=dst[:self.count])
+        del old
+        self.order[:self.count].copy_(self._iota[:self.count])
+        self.hw = self.count
+        if self.hw + B > self.phys:
+            raise RuntimeError("frontier capacity exceeded")
 
     def sort(self, thr):
         """Order the rows by their largest margin max_s (lb_s - thr_s),
@@ -259,29 +346,33 @@ class DeviceFrontier:
         (complete_verifier/input_split/branching_domains.py:1171-1241,
         976-981; gear/presets/gear-f0.yaml:409, 550-552).  Ties keep their
         current order (a stable sort; the reference's argsort leaves tie
-        order unspecified)."""
+        order unspecified).  Only the slot ids move."""
         c = self.count
         if c <= 1:
             return
-        keys = (self.lb[:c] - thr).amax(1)
-        order = torch.argsort(keys, descending=True, stable=True)
-        self.L[:c] = self.L[:c].index_select(0, order)
-        self.U[:c] = self.U[:c].index_select(0, order)
-        self.lb[:c] = self.lb[:c].index_select(0, order)
-        if self.pay is not None:
-            self.pay[:c] = self.pay[:c].index_select(0, order)
+        if self._keys is None or self._keys.numel() < c:
+            self._keys = torch.empty(self.cap, dtype=torch.float64, device=self.device)
+        keys = self._keys[:c]
+        check(lib().gh_frontier_keys_f64(c, self.S, ptr(self.order), ptr(self._lb),
+                                         ptr(thr.contiguous()), ptr(keys), stream_ptr()),
+              "gh_frontier_keys_f64")
+        perm = torch.argsort(keys, descending=True, stable=True)
+        self.order[:c] = self.order[:c].index_select(0, perm)
 
     def pick_out(self, batch):
-        """Views of the last `batch` rows (the parents of the next split);
-        the rows stay valid until the next append overwrites them."""
+        """(x_L, x_U, lb) of the last `batch` logical rows (the parents of the
+        next split), gathered; the rows leave the frontier."""
         nb = min(batch, self.count)
         s = self.count - nb
         self.count = s
-        return self.L[s:s + nb], self.U[s:s + nb], self.lb[s:s + nb]
+        idx = self.order[s:s + nb]
+        self._picked = idx
+        return (self._L.index_select(0, idx), self._U.index_select(0, idx),
+                self._lb.index_select(0, idx))
 
     def pick_payload(self, nb):
         """The payload rows of the domains the last pick_out(nb) returned."""
-        return self.pay[self.count:self.count + nb]
+        return self._pay.index_select(0, self._picked[:nb])
 
     def workspace(self, B):
         need = int(lib().gh_filter_workspace_bytes(B))
@@ -295,26 +386,40 @@ class DeviceFrontier:
         when the frontier holds a payload).  Returns the survivor count (a
         host sync) when sync is set, else None (call finish_append later)."""
         B = x_L.shape[0]
+        if self.count + B > self.cap:
+            raise RuntimeError("frontier capacity exceeded")
+        self._room(B)
         ws = self.workspace(B)
         check(lib().gh_filter_append_f64(B, self.S, self.n, ptr(lb), ptr(thr), ptr(x_L),
-                                         ptr(x_U), ptr(self.L), ptr(self.U), ptr(self.lb),
-                                         self.count, self.cap, ptr(self.total),
+                                         ptr(x_U), ptr(self._L), ptr(self._U), ptr(self._lb),
+                                         self.hw, self.phys, ptr(self.total),
                                          ptr(self.disc_min), ptr(ws), stream_ptr()),
               "gh_filter_append_f64")
-        if self.pay is not None:
+        if self._pay is not None:
             if payload is None or payload.shape[1] != self.W:
                 raise ValueError(f"this frontier stores a payload of {self.W} per domain")
             payload = payload.contiguous()
             check(lib().gh_filter_payload_f64(B, self.W, payload.shape[0], ptr(payload), ptr(ws),
-                                              ptr(self.pay), self.count, self.cap, stream_ptr()),
+                                              ptr(self._pay), self.hw, self.phys, stream_ptr()),
                   "gh_filter_payload_f64")
+        # survivor r sits in physical row hw + r; slots past the survivor
+        # count are written too and never read
+        torch.add(self._iota[:B], self.hw, out=self.order[self.count:self.count + B])
+        self._pending = B
         if sync:
             return self.finish_append()
         return None
 
+    def commit(self, kept):
+        """Account the survivors of the last filter_append (read from
+        `total`)."""
+        self.count += kept
+        self.hw += kept
+        self._pending = None
+
     def finish_append(self):
         kept = int(self.total.item())
-        self.count += kept
+        self.commit(kept)
         return kept
 
     def global_lb(self, out=None):
@@ -322,6 +427,6 @@ class DeviceFrontier:
         device scalar."""
         out = out if out is not None else torch.empty(1, dtype=torch.float64, device=self.device)
         out.copy_(self.disc_min)
-        check(lib().gh_min_strided_f64(self.count, self.S, ptr(self.lb), ptr(out), stream_ptr()),
-              "gh_min_strided_f64")
+        check(lib().gh_min_rows_f64(self.count, self.S, ptr(self.order), ptr(self._lb), ptr(out),
+                                    stream_ptr()), "gh_min_rows_f64")
         return out

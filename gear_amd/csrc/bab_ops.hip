@@ -207,6 +207,40 @@ __global__ __launch_bounds__(256) void min_strided_kernel(int64_t N, int64_t str
 
 __global__ void set_i64_kernel(int64_t* p, int64_t v) { *p = v; }
 
+// Sort key of logical row i: max_s (lb[order[i]][s] - thr[s]), NaN if any
+// term is NaN (torch's amax).
+__global__ void frontier_keys_kernel(int64_t N, int S, const int64_t* __restrict__ order,
+                                     const double* __restrict__ lb, const double* __restrict__ thr,
+                                     double* __restrict__ keys)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const double* r = lb + order[i] * S;
+    double m = -INFINITY;
+    bool nan = false;
+    for (int s = 0; s < S; ++s) {
+        const double v = r[s] - thr[s];
+        nan = nan || isnan(v);
+        m = v > m ? v : m;
+    }
+    keys[i] = nan ? NAN : m;
+}
+
+// min over i < N of x[idx[i] * stride] into *out (atomic min).
+__global__ __launch_bounds__(256) void min_rows_kernel(int64_t N, int64_t stride,
+                                                       const int64_t* __restrict__ idx,
+                                                       const double* __restrict__ x,
+                                                       double* __restrict__ out)
+{
+    __shared__ double red[4];
+    double m = INFINITY;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < N;
+         i += (int64_t)gridDim.x * 256)
+        m = fmin(m, x[idx[i] * stride]);
+    m = block_min(m, red);
+    if (threadIdx.x == 0 && m < INFINITY) atomic_min_f64(out, m);
+}
+
 // Payload row of each survivor (row k % P of pay) to its frontier row.
 __global__ void filter_payload_kernel(int B, int W, int P, const double* __restrict__ pay,
                                       const int32_t* __restrict__ dst, double* __restrict__ Fpay,
@@ -292,6 +326,28 @@ extern "C" int gh_filter_payload_f64(int B, int W, int P, const double* pay, con
     return status();
 }
 
+extern "C" int gh_frontier_keys_f64(int64_t N, int S, const int64_t* order, const double* lb,
+                                    const double* thr, double* keys, hipStream_t stream)
+{
+    if (N < 0 || S <= 0 || (N > 0 && (!order || !lb || !thr || !keys))) return GH_EINVAL;
+    if (N == 0) return GH_OK;
+    hipLaunchKernelGGL(frontier_keys_kernel, dim3((unsigned)((N + 255) / 256)), dim3(256), 0,
+                       stream, N, S, order, lb, thr, keys);
+    return status();
+}
+
+extern "C" int gh_min_rows_f64(int64_t N, int64_t stride, const int64_t* idx, const double* x,
+                               double* out, hipStream_t stream)
+{
+    if (N < 0 || stride <= 0 || !out || (N > 0 && (!x || !idx))) return GH_EINVAL;
+    if (N == 0) return GH_OK;
+    int64_t blocks = (N + 255) / 256;
+    if (blocks > 2048) blocks = 2048;
+    hipLaunchKernelGGL(min_rows_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, N, stride,
+                       idx, x, out);
+    return status();
+}
+
 extern "C" int gh_min_strided_f64(int64_t N, int64_t stride, const double* x, double* out,
                                   hipStream_t stream)
 {
@@ -307,46 +363,143 @@ extern "C" int gh_min_strided_f64(int64_t N, int64_t stride, const double* x, do
 // ------------------------------------------------------------ child keys
 namespace {
 
-// Child k (of 2K, [upper halves; lower halves]) of parent k mod K: row 0 is
-// max(parent bound, the parent's objective plane minimised over the child
-// box and rounded down); rows >= 1 keep the parent's bound
-// (batch_branch_and_bound.py:189-225 _plane_minimum_down / _raise_child_keys).
-// The minimum sum_i (a_i c_i - |a_i| r_i) + b is lowered by
-// (n + 4) u (sum_i |a_i| (|c_i| + r_i) + |b|) and one step toward -inf.
+// "red64s": lane l sums the terms i = l, l + 64, ... in order from +0.0,
+// then p[l] + p[l ^ o] for o = 32, 16, 8, 4, 2, 1 (every lane ends with
+// the sum).  The oracle's plane_minimum_down states the same order.
+__device__ __forceinline__ double red64x(double v)
+{
+    v = v + gh::xor32_f64(v);
+    v = v + gh::xor16_f64(v);
+    return gh::tree16(v);
+}
+
+constexpr double kU53 = 1.1102230246251565404236316680908203125e-16;
+
+// Key of one child box [xl, xu] from its parent's objective plane (a, b)
+// and bound row plb: max(plb[0], the plane's minimum over the box lowered
+// by (n + 4) u (sum |a| (|c| + r) + |b|) and one step toward -inf);
+// plb[0] where the box is empty or the minimum is not finite
+// (batch_branch_and_bound.py:189-225 _plane_minimum_down /
+// _raise_child_keys).  Wave-wide; every lane returns the key.
+__device__ __forceinline__ double child_key(int n, int lane, const double* __restrict__ a,
+                                            double b, const double* __restrict__ xl,
+                                            const double* __restrict__ xu, double p0)
+{
+    double pm = 0.0, mg = 0.0;
+    bool bad = false;
+    for (int i = lane; i < n; i += 64) {
+        const double l = xl[i], u = xu[i];
+        const double c = (l + u) / 2, r = (u - l) / 2;
+        const double ai = a[i];
+        pm = pm + (ai * c - fabs(ai) * r);
+        mg = mg + fabs(ai) * (fabs(c) + r);
+        bad = bad || !(l <= u);
+    }
+    pm = red64x(pm) + b;
+    mg = red64x(mg);
+    bad = __ballot(bad) != 0;
+    const double margin = (n + 4) * kU53 * (mg + fabs(b));
+    const double key = nextafter(pm - margin, -INFINITY);
+    return (!bad && isfinite(key)) ? fmax(p0, key) : p0;
+}
+
+// One wave per child k of parent k % K (children [upper halves; lower
+// halves], the layout of gh_split_f64).
 __global__ void __launch_bounds__(256)
 child_keys_kernel(int K, int n, int S, int R, const double *__restrict__ plb,
                   const double *__restrict__ lA, const double *__restrict__ lbias,
                   const double *__restrict__ cL, const double *__restrict__ cU,
                   double *__restrict__ keys)
 {
-    // one thread per child; the sums run in index order, the order the key
-    // is specified in, so keys (and the sort and prune that read them) are
-    // the same on every device and in the restatement
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
     if (k >= 2 * K) return;
     const int p = k % K;
-    const double *a = lA + (size_t)p * R * n;  // objective row
-    const double *xl = cL + (size_t)k * n, *xu = cU + (size_t)k * n;
-    double pm = 0.0, mg = 0.0;
-    bool bad = false;
-    for (int i = 0; i < n; ++i) {
-        const double l = xl[i], u = xu[i];
-        const double c = (l + u) / 2, r = (u - l) / 2;
-        const double ai = a[i];
-        pm += ai * c - fabs(ai) * r;
-        mg += fabs(ai) * (fabs(c) + r);
-        bad |= !(l <= u);
+    const double key = child_key(n, lane, lA + (size_t)p * R * n, lbias[(size_t)p * R],
+                                 cL + (size_t)k * n, cU + (size_t)k * n, plb[(size_t)p * S]);
+    for (int s = lane; s < S; s += 64)
+        keys[(size_t)k * S + s] = s == 0 ? key : plb[(size_t)p * S + s];
+}
+
+// Widest branching, bisection and both children's keys of parent p in one
+// pass (one wave per parent).  The index is the first of the widest
+// coordinates as branch_widest_kernel's sequential scan finds it (a NaN
+// width at 0 keeps 0, later NaN widths never win); the children are
+// split_kernel's; their keys are child_keys_kernel's (plb copied when
+// there are no planes).
+__global__ void __launch_bounds__(256)
+branch_split_keys_kernel(int K, int n, int S, int R, const double *__restrict__ pL,
+                         const double *__restrict__ pU, const double *__restrict__ plb,
+                         const double *__restrict__ lA, const double *__restrict__ lbias,
+                         double *__restrict__ cL, double *__restrict__ cU,
+                         double *__restrict__ keys)
+{
+    const int p = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (p >= K) return;
+    const double *l = pL + (size_t)p * n, *u = pU + (size_t)p * n;
+    // first widest coordinate: per-lane candidates in index order, then the
+    // largest width with the lowest index
+    double bw = -INFINITY;
+    int bi = -1;
+    const double w0 = u[0] - l[0];
+    for (int i = lane; i < n; i += 64) {
+        const double w = u[i] - l[i];
+        if (bi < 0 ? !isnan(w) : w > bw) {
+            bw = w;
+            bi = i;
+        }
     }
-    for (int s = 0; s < S; ++s) keys[(size_t)k * S + s] = plb[(size_t)p * S + s];
-    const double b = lbias[(size_t)p * R];
-    const double u53 = 1.1102230246251565404236316680908203125e-16;
-    pm = pm + b;
-    const double margin = (n + 4) * u53 * (mg + fabs(b));
-    const double key = nextafter(pm - margin, -INFINITY);
-    if (!bad && isfinite(key)) keys[(size_t)k * S] = fmax(plb[(size_t)p * S], key);
+    for (int o = 32; o >= 1; o >>= 1) {
+        const double w2 = __shfl_xor(bw, o);
+        const int i2 = __shfl_xor(bi, o);
+        if (i2 >= 0 && (bi < 0 || w2 > bw || (w2 == bw && i2 < bi))) {
+            bw = w2;
+            bi = i2;
+        }
+    }
+    const int idx = (isnan(w0) || bi < 0) ? 0 : bi;
+    // children: c = 0 the upper half (row p), c = 1 the lower half (row p + K)
+    for (int c = 0; c < 2; ++c) {
+        double *ol = cL + (size_t)(p + c * K) * n, *ou = cU + (size_t)(p + c * K) * n;
+        for (int i = lane; i < n; i += 64) {
+            double lo = l[i], hi = u[i];
+            if (i == idx) {
+                const double a = (c + 1) * lo + (2 - c - 1) * hi;
+                const double b = c * lo + (2 - c) * hi;
+                lo = a / 2.0;
+                hi = b / 2.0;
+            }
+            ol[i] = lo;
+            ou[i] = hi;
+        }
+    }
+    // each lane reads back only the coordinates it wrote (same strided
+    // mapping), so program order makes them visible
+    const double p0 = plb[(size_t)p * S];
+    for (int c = 0; c < 2; ++c) {
+        const size_t k = (size_t)p + (size_t)c * K;
+        const double key = lA ? child_key(n, lane, lA + (size_t)p * R * n, lbias[(size_t)p * R],
+                                          cL + k * n, cU + k * n, p0)
+                              : p0;
+        for (int s = lane; s < S; s += 64) keys[k * S + s] = s == 0 ? key : plb[(size_t)p * S + s];
+    }
 }
 
 }  // namespace
+
+extern "C" int gh_branch_split_keys_f64(int K, int n, int S, int R, const double *pL,
+                                        const double *pU, const double *plb, const double *lA,
+                                        const double *lbias, double *cL, double *cU,
+                                        double *keys, hipStream_t stream)
+{
+    if (K == 0) return GH_OK;
+    if (K < 0 || n < 1 || S < 1 || !pL || !pU || !plb || !cL || !cU || !keys) return GH_EINVAL;
+    if (lA && (R < 1 || R > S || !lbias)) return GH_EINVAL;
+    hipLaunchKernelGGL(branch_split_keys_kernel, dim3((K + 3) / 4), dim3(256), 0, stream, K, n, S,
+                       R, pL, pU, plb, lA, lbias, cL, cU, keys);
+    return hipGetLastError() == hipSuccess ? GH_OK : GH_ELAUNCH;
+}
 
 extern "C" int gh_child_keys_f64(int K, int n, int S, int R, const double *plb,
                                  const double *lA, const double *lbias, const double *cL,
@@ -355,7 +508,7 @@ extern "C" int gh_child_keys_f64(int K, int n, int S, int R, const double *plb,
     if (K == 0) return GH_OK;
     if (K < 0 || n < 1 || S < 1 || R < 1 || R > S || !plb || !lA || !lbias || !cL || !cU || !keys)
         return GH_EINVAL;
-    hipLaunchKernelGGL(child_keys_kernel, dim3((2 * K + 255) / 256), dim3(256), 0, stream, K, n,
-                       S, R, plb, lA, lbias, cL, cU, keys);
+    hipLaunchKernelGGL(child_keys_kernel, dim3((2 * K + 3) / 4), dim3(256), 0, stream, K, n, S,
+                       R, plb, lA, lbias, cL, cU, keys);
     return hipGetLastError() == hipSuccess ? GH_OK : GH_ELAUNCH;
 }

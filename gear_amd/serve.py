@@ -31,7 +31,7 @@ import torch
 
 from .alm import ALMConfig, ALMPGDSolver, replay_float64
 from .bab import is_primal_dual_converged
-from .crown import (BoundedQCQP, DeviceFrontier, child_keys, input_split_branching,
+from .crown import (BoundedQCQP, DeviceFrontier, branch_split_keys, input_split_branching,
                     input_split_parallel)
 from .problems import flops_per_box, qcqp_dense
 
@@ -118,7 +118,11 @@ def footprint(n, S, cfg):
     large = 0
     if n > 64:
         large = S * n * n * 4 * 2 + 2 * nb * ((3 * n + 5 * S * n) * 4 + 4 * S * 8 + 8)
-    return frontier_cap(n, S, cfg) * row + child + prog + large + (8 << 20)
+    cap = frontier_cap(n, S, cfg)
+    # physical rows (cap and half again of slack), the order array, and the
+    # transient second copy while live rows are compacted
+    front = (cap + max(cap // 2, 4096)) * row * 2 + cap * 16
+    return front + child + prog + large + (8 << 20)
 
 
 class Session:
@@ -244,9 +248,7 @@ class Session:
             lb = torch.maximum(lb, pkey)
             if lp:
                 self._raise_by_lp(lb, lA, lbias, pL, pU)
-            idx = input_split_branching(None, None, pL, pU, None, None, "naive")
-            cL, cU, _ = input_split_parallel(pL, pU, i_idx=idx)
-            keys = child_keys(lb, lA, lbias, cL, cU) if lA is not None else torch.cat([lb, lb])
+            cL, cU, keys = branch_split_keys(lb, pL, pU, lA, lbias)
             f.filter_append(keys, self.thr, cL, cU, sync=False)
             self._bounded = nb
         return nb
@@ -278,7 +280,7 @@ class Session:
         self.event.synchronize()
         f = self.frontier
         if self._pending:
-            f.count += int(self._host_total[0])
+            f.commit(int(self._host_total[0]))
             self._pending = False
             if self._nb:
                 self.steps += 1
@@ -303,7 +305,8 @@ class Session:
             k = min(cfg.alm_boxes, len(f))
             s0 = len(f) - k
             with torch.cuda.stream(self.stream):
-                self._incumbent(f.L[s0:s0 + k], f.U[s0:s0 + k], cfg.alm_box_restarts)
+                wL, wU = f.rows(s0, s0 + k)[:2]
+                self._incumbent(wL, wU, cfg.alm_box_restarts)
         return False
 
     def result(self):

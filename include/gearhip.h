@@ -268,11 +268,23 @@ int gh_clip_domains_f64(int B, int P, int n, int S, const double *lA, const doub
  * batch_branch_and_bound.py:189-225) of the reordered loop.
  *   plb (K, S): the parents' bounds; lA (K, R, n), lbias (K, R): their
  *   planes of the leading R >= 1 rows (row 0 is read); cL, cU (2K, n):
- *   children; keys (2K, S) out.  Both sums run in index order.
+ *   children; keys (2K, S) out.  Both sums run in the order the oracle's
+ *   plane_minimum_down states (64 lane-strided partials, then an xor tree).
  */
 int gh_child_keys_f64(int K, int n, int S, int R, const double *plb, const double *lA,
                       const double *lbias, const double *cL, const double *cU, double *keys,
                       hipStream_t stream);
+
+/*
+ * gh_branch_widest_f64, gh_split_f64 and gh_child_keys_f64 in one pass over
+ * K parents (the reordered loop's decision, split and child-key steps,
+ * batch_branch_and_bound.py:470-534): cL, cU (2K, n) and keys (2K, S) out,
+ * equal to the three calls in sequence.  lA / lbias may be NULL (no
+ * planes: keys are the parents' bounds).
+ */
+int gh_branch_split_keys_f64(int K, int n, int S, int R, const double *pL, const double *pU,
+                             const double *plb, const double *lA, const double *lbias,
+                             double *cL, double *cU, double *keys, hipStream_t stream);
 
 /* Bytes of device workspace gh_filter_append_f64 needs for B boxes. */
 size_t gh_filter_workspace_bytes(int B);
@@ -324,6 +336,20 @@ int gh_filter_payload_f64(int B, int W, int P, const double *pay, const void *ws
  */
 int gh_min_strided_f64(int64_t N, int64_t stride, const double *x,
                        double *out, hipStream_t stream);
+
+/*
+ * Frontier with a logical order over stable physical rows (rows never move
+ * when the frontier is re-sorted; only their int64 slot ids do):
+ * gh_frontier_keys_f64: keys[i] = max_s (lb[order[i]][s] - thr[s]) (NaN if
+ * a term is NaN), the sort key of UnsortedInputDomainList.sort with
+ * sort_heuristic 'lb_rhs' (branching_domains.py:1171-1241);
+ * gh_min_rows_f64: *out = min(*out, min_i x[idx[i] * stride]), the global
+ * lower bound over the logical rows (batch_branch_and_bound.py:620-634).
+ */
+int gh_frontier_keys_f64(int64_t N, int S, const int64_t *order, const double *lb,
+                         const double *thr, double *keys, hipStream_t stream);
+int gh_min_rows_f64(int64_t N, int64_t stride, const int64_t *idx, const double *x, double *out,
+                    hipStream_t stream);
 
 /* ------------------------------------------------ expression programs */
 

@@ -282,19 +282,32 @@ def clip_domains(lA, lbias, thr, xL, xU):
     return oL, oU, state
 
 
+def _red64s(T):
+    """Row sums of T (B, n) in the red64s order (see plane_minimum_down)."""
+    B, n = T.shape
+    P = np.zeros((B, 64))
+    for j0 in range(0, n, 64):
+        w = min(64, n - j0)
+        P[:, :w] = P[:, :w] + T[:, j0:j0 + w]
+    lanes = np.arange(64)
+    for o in (32, 16, 8, 4, 2, 1):
+        P = P + P[:, lanes ^ o]
+    return P[:, 0]
+
+
 def plane_minimum_down(a, b, xl, xu):
     """Minimum of the plane a x + b over the box, never above the exact value:
     sum(a c - |a| r) + b lowered by (n + 4) u (sum |a| (|c| + r) + |b|) and
     one step toward -inf (complete_verifier/input_split/
-    batch_branch_and_bound.py:189-208).  Both sums run in index order."""
+    batch_branch_and_bound.py:189-208).  Both sums run in the "red64s"
+    order: slot l sums the terms i = l, l + 64, ... in order from +0.0, then
+    p[l] + p[l ^ o] for o = 32, 16, 8, 4, 2, 1 (the margin covers any order;
+    this one is the kernels')."""
     one = np.ndim(xl) == 1
     a, xl, xu = (np.atleast_2d(np.asarray(v, dtype=np.float64)) for v in (a, xl, xu))
     c, r = (xl + xu) / 2, (xu - xl) / 2
-    pm = np.zeros(xl.shape[0])
-    mg = np.zeros(xl.shape[0])
-    for i in range(xl.shape[1]):
-        pm = pm + (a[:, i] * c[:, i] - np.abs(a[:, i]) * r[:, i])
-        mg = mg + np.abs(a[:, i]) * (np.abs(c[:, i]) + r[:, i])
+    pm = _red64s(a * c - np.abs(a) * r)
+    mg = _red64s(np.abs(a) * (np.abs(c) + r))
     pm = pm + b
     margin = (xl.shape[1] + 4) * U64 * (mg + np.abs(b))
     out = np.nextafter(pm - margin, -np.inf)

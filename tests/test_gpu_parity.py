@@ -181,15 +181,20 @@ def test_child_keys_match_oracle(cuda):
     assert (got[:, 0] > np.concatenate([plb[:, 0], plb[:, 0]])).any()
 
 
-def test_reordered_bab_iterations_match_oracle_loop(cuda):
-    """Six iterations of the reordered B&B order (the default) on the device
+@pytest.mark.parametrize("batch,cap,slack,iters", [(32, 4096, None, 6), (4, 128, 20, 14)])
+def test_reordered_bab_iterations_match_oracle_loop(cuda, batch, cap, slack, iters):
+    """Iterations of the reordered B&B order (the default) on the device
     reproduce the oracle-driven loop: sort, pick 2 batch domains from the
     end, bound them and keep the larger of that bound and the stored key,
     widest branch, bisect, child keys from the parents' planes, prune,
-    append."""
+    append.  The second case gives the frontier 20 rows of slack, so it
+    compacts its live records during the run, which must change nothing."""
     from gear_amd.bab import BaBConfig, QCQPBranchAndBound
+    from gear_amd.crown import DeviceFrontier
     p = qcqp_dense(10, 3, seed=11)
-    bb = QCQPBranchAndBound(p, BaBConfig(batch=32, cap=4096), device=cuda)
+    bb = QCQPBranchAndBound(p, BaBConfig(batch=batch, cap=cap), device=cuda)
+    if slack is not None:
+        bb.frontier = DeviceFrontier(p.n, p.S, cap, cuda, slack=slack)
     assert bb.cfg.reorder
     bb.root(alm=False)
     bb.set_incumbent(-1.0, None)
@@ -197,11 +202,12 @@ def test_reordered_bab_iterations_match_oracle_loop(cuda):
     Flb = O.qcqp_crown(p.Q, p.c, p.d, FL, FU)
     thr = np.concatenate([[-1.0], p.rhs[1:] + 1e-6])
     raised = 0
-    for it in range(6):
+    compactions = 0
+    for it in range(iters):
         keys = (Flb - thr).max(axis=1)
         order = np.argsort(-keys, kind="stable")
         FL, FU, Flb = FL[order], FU[order], Flb[order]
-        nb = min(64, len(FL))
+        nb = min(2 * batch, len(FL))
         pL, pU, pk = FL[-nb:], FU[-nb:], Flb[-nb:]
         FL, FU, Flb = FL[:-nb], FU[:-nb], Flb[:-nb]
         lb, lA, lbias = O.qcqp_crown(p.Q, p.c, p.d, pL, pU, return_A=True)
@@ -213,9 +219,58 @@ def test_reordered_bab_iterations_match_oracle_loop(cuda):
         FL = np.vstack([FL, cL[keep]])
         FU = np.vstack([FU, cU[keep]])
         Flb = np.vstack([Flb, ck[keep]])
+        hw0 = bb.frontier.hw
         assert bb.step() == nb
+        compactions += bb.frontier.hw < hw0
         c = len(bb.frontier)
         assert c == len(FL), it
         assert _bits_equal(bb.frontier.L[:c].cpu().numpy(), FL)
+        assert _bits_equal(bb.frontier.U[:c].cpu().numpy(), FU)
         assert _bits_equal(bb.frontier.lb[:c].cpu().numpy(), Flb)
     assert raised > 0
+    if slack is not None:
+        assert compactions > 0
+
+
+@pytest.mark.parametrize("K,n,m", [(300, 12, 4), (257, 50, 20), (40, 70, 2), (9, 1, 0)])
+def test_branch_split_keys_matches_oracle(cuda, K, n, m):
+    """The fused branch + split + child-key kernel equals the oracle's
+    widest branch, bisection and child keys, bit for bit, including NaN and
+    tied widths, invalid boxes and an infinite plane."""
+    from gear_amd.crown import branch_split_keys
+    print(f"case K={K} n={n} m={m}", flush=True)
+    p = qcqp_dense(n, m, seed=n + m) if n <= 64 else None
+    rng = np.random.default_rng(K + n)
+    L = -rng.random((K, n)) - 0.1
+    U = rng.random((K, n)) + 0.1
+    if n > 2:
+        U[::5, 1] = L[::5, 1] + (U[::5, 0] - L[::5, 0])    # tied widest widths
+        U[3::11, 0] = np.nan                                # NaN width at 0
+        U[4::13, 2] = np.nan                                # a later NaN width
+        U[6::17, 1] = L[6::17, 1] - 1.0                     # invalid box
+    S = m + 1
+    if p is not None:
+        Lc, Uc = np.nan_to_num(L, nan=0.5), np.nan_to_num(U, nan=0.5)
+        plb, lA, lbias = O.qcqp_crown(p.Q, p.c, p.d, np.minimum(Lc, Uc), np.maximum(Lc, Uc),
+                                      return_A=True)
+    else:
+        plb = rng.standard_normal((K, S))
+        lA = rng.standard_normal((K, S, n))
+        lbias = rng.standard_normal((K, S))
+    lbias[7 % K, 0] = np.inf
+    eL, eU = O.split(L, U, O.branch_widest(L, U))
+    ref = O.child_keys(plb, lA, lbias, eL, eU)
+    cL, cU, keys = branch_split_keys(_dev(plb, cuda), _dev(L, cuda), _dev(U, cuda),
+                                     _dev(lA, cuda), _dev(lbias, cuda))
+
+    def same(a, b):        # bit for bit, any NaN equal to any NaN
+        a, b = np.asarray(a), np.asarray(b)
+        nan = np.isnan(a) & np.isnan(b)
+        return np.array_equal(np.isnan(a), np.isnan(b)) and _bits_equal(np.where(nan, 0, a),
+                                                                         np.where(nan, 0, b))
+    assert same(cL.cpu().numpy(), eL)
+    assert same(cU.cpu().numpy(), eU)
+    assert same(keys.cpu().numpy(), ref)
+    # without planes the keys are the parents' bounds
+    _, _, k0 = branch_split_keys(_dev(plb, cuda), _dev(L, cuda), _dev(U, cuda))
+    assert _bits_equal(k0.cpu().numpy(), np.vstack([plb, plb]))
