@@ -262,6 +262,7 @@ class DeviceFrontier:
         self.phys = self.cap + int(slack if slack is not None else max(self.cap // 2, 4096))
         self._alloc(self.phys)
         self.order = torch.empty(self.cap, dtype=torch.int64, device=self.device)
+        self._order2 = torch.empty_like(self.order)    # the sort's other buffer
         self._iota = torch.arange(self.cap, dtype=torch.int64, device=self.device)
         self.count = 0          # logical rows
         self.hw = 0             # physical rows in use (live or picked)
@@ -357,7 +358,8 @@ class DeviceFrontier:
                                          ptr(thr.contiguous()), ptr(keys), stream_ptr()),
               "gh_frontier_keys_f64")
         perm = torch.argsort(keys, descending=True, stable=True)
-        self.order[:c] = self.order[:c].index_select(0, perm)
+        torch.index_select(self.order[:c], 0, perm, out=self._order2[:c])
+        self.order, self._order2 = self._order2, self.order
 
     def pick_out(self, batch):
         """(x_L, x_U, lb) of the last `batch` logical rows (the parents of the
@@ -367,8 +369,14 @@ class DeviceFrontier:
         self.count = s
         idx = self.order[s:s + nb]
         self._picked = idx
-        return (self._L.index_select(0, idx), self._U.index_select(0, idx),
-                self._lb.index_select(0, idx))
+        d = dict(dtype=torch.float64, device=self.device)
+        pL = torch.empty(nb, self.n, **d)
+        pU = torch.empty(nb, self.n, **d)
+        plb = torch.empty(nb, self.S, **d)
+        check(lib().gh_gather_rows_f64(nb, ptr(idx), self.n, ptr(self._L), ptr(pL), self.n,
+                                       ptr(self._U), ptr(pU), self.S, ptr(self._lb), ptr(plb),
+                                       stream_ptr()), "gh_gather_rows_f64")
+        return pL, pU, plb
 
     def pick_payload(self, nb):
         """The payload rows of the domains the last pick_out(nb) returned."""

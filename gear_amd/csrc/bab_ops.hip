@@ -50,10 +50,10 @@ __global__ void split_kernel(int B, int n, const double* __restrict__ xL,
 // Verification predicate of get_remaining_index (solving mode) plus the
 // invalid-box check.  Returns keep; *verified_thr gets thr[0] when the box
 // is verified with a finite objective threshold, +inf otherwise.
-__device__ __forceinline__ bool keep_box(int b, int S, int n, const double* __restrict__ lb,
-                                         const double* __restrict__ thr,
-                                         const double* __restrict__ xL,
-                                         const double* __restrict__ xU, double* verified_thr)
+// box_flags: bit 0 some coordinate non-finite, bit 1 some l > u.
+__device__ __forceinline__ bool keep_box(int b, int S, const double* __restrict__ lb,
+                                         const double* __restrict__ thr, int box_flags,
+                                         double* verified_thr)
 {
     const double* L = lb + (size_t)b * S;
     bool ver = false;
@@ -65,14 +65,7 @@ __device__ __forceinline__ bool keep_box(int b, int S, int n, const double* __re
     }
     ver = ver || (isinf(L[0]) && L[0] > 0 && isfinite(thr[0]));
     *verified_thr = (ver && isfinite(thr[0])) ? thr[0] : INFINITY;
-    bool nonfinite = false, invalid = false;
-    const double* l = xL + (size_t)b * n;
-    const double* u = xU + (size_t)b * n;
-    for (int i = 0; i < n; ++i) {
-        double a = l[i], c = u[i];
-        nonfinite = nonfinite || !isfinite(a) || !isfinite(c);
-        invalid = invalid || (a > c);
-    }
+    const bool nonfinite = box_flags & 1, invalid = box_flags & 2;
     return !ver && !(invalid && !nonfinite);
 }
 
@@ -106,11 +99,36 @@ __global__ __launch_bounds__(256) void filter_count_kernel(
     int32_t* __restrict__ counts, double* __restrict__ disc_min)
 {
     __shared__ double red[4];
+    __shared__ int bflags[256];
+    // box scan: 16 lanes per box, each reading every 16th coordinate of
+    // both bounds (coalesced 128-byte runs), 16 boxes per pass
+    const int g = threadIdx.x >> 4, gl = threadIdx.x & 15;
+    for (int p = 0; p < 16; ++p) {
+        const int lb_box = p * 16 + g;
+        const int bb = blockIdx.x * 256 + lb_box;
+        int fl = 0;
+        if (bb < B) {
+            const double* l = xL + (size_t)bb * n;
+            const double* u = xU + (size_t)bb * n;
+            for (int i = gl; i < n; i += 16) {
+                const double a = l[i], c = u[i];
+                fl |= (!isfinite(a) || !isfinite(c)) ? 1 : 0;
+                fl |= (a > c) ? 2 : 0;
+            }
+        }
+        // or over the 16 lanes of the group (xor tree inside a row of 16)
+        fl |= __shfl_xor(fl, 8, 16);
+        fl |= __shfl_xor(fl, 4, 16);
+        fl |= __shfl_xor(fl, 2, 16);
+        fl |= __shfl_xor(fl, 1, 16);
+        if (gl == 0) bflags[lb_box] = fl;
+    }
+    __syncthreads();
     int b = blockIdx.x * 256 + threadIdx.x;
     double vt = INFINITY;
     bool k = false;
     if (b < B) {
-        k = keep_box(b, S, n, lb, thr, xL, xU, &vt);
+        k = keep_box(b, S, lb, thr, bflags[threadIdx.x], &vt);
         dst[b] = k ? 1 : 0;
     }
     int cnt = __syncthreads_count(k);
@@ -226,6 +244,32 @@ __global__ void frontier_keys_kernel(int64_t N, int S, const int64_t* __restrict
     keys[i] = nan ? NAN : m;
 }
 
+// Rows idx[i] of up to three row-major arrays (widths w0, w1, w2) into
+// contiguous outputs, one thread per element.
+__global__ void gather_rows_kernel(int64_t N, const int64_t* __restrict__ idx, int w0,
+                                   const double* __restrict__ s0, double* __restrict__ d0, int w1,
+                                   const double* __restrict__ s1, double* __restrict__ d1, int w2,
+                                   const double* __restrict__ s2, double* __restrict__ d2)
+{
+    const int64_t W = (int64_t)w0 + w1 + w2;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= N * W) return;
+    const int64_t i = t / W;
+    int c = (int)(t - i * W);
+    const int64_t r = idx[i];
+    if (c < w0) {
+        d0[i * w0 + c] = s0[r * w0 + c];
+        return;
+    }
+    c -= w0;
+    if (c < w1) {
+        d1[i * w1 + c] = s1[r * w1 + c];
+        return;
+    }
+    c -= w1;
+    d2[i * w2 + c] = s2[r * w2 + c];
+}
+
 // min over i < N of x[idx[i] * stride] into *out (atomic min).
 __global__ __launch_bounds__(256) void min_rows_kernel(int64_t N, int64_t stride,
                                                        const int64_t* __restrict__ idx,
@@ -333,6 +377,20 @@ extern "C" int gh_frontier_keys_f64(int64_t N, int S, const int64_t* order, cons
     if (N == 0) return GH_OK;
     hipLaunchKernelGGL(frontier_keys_kernel, dim3((unsigned)((N + 255) / 256)), dim3(256), 0,
                        stream, N, S, order, lb, thr, keys);
+    return status();
+}
+
+extern "C" int gh_gather_rows_f64(int64_t N, const int64_t* idx, int w0, const double* s0,
+                                  double* d0, int w1, const double* s1, double* d1, int w2,
+                                  const double* s2, double* d2, hipStream_t stream)
+{
+    if (N < 0 || w0 < 0 || w1 < 0 || w2 < 0) return GH_EINVAL;
+    if (N == 0 || w0 + w1 + w2 == 0) return GH_OK;
+    if (!idx || (w0 && (!s0 || !d0)) || (w1 && (!s1 || !d1)) || (w2 && (!s2 || !d2)))
+        return GH_EINVAL;
+    const int64_t items = N * ((int64_t)w0 + w1 + w2);
+    hipLaunchKernelGGL(gather_rows_kernel, dim3((unsigned)((items + 255) / 256)), dim3(256), 0,
+                       stream, N, idx, w0, s0, d0, w1, s1, d1, w2, s2, d2);
     return status();
 }
 

@@ -350,6 +350,13 @@ int gh_frontier_keys_f64(int64_t N, int S, const int64_t *order, const double *l
                          const double *thr, double *keys, hipStream_t stream);
 int gh_min_rows_f64(int64_t N, int64_t stride, const int64_t *idx, const double *x, double *out,
                     hipStream_t stream);
+/* Rows idx[i] (i < N) of up to three row-major arrays s_k of widths w_k
+ * (a width 0 skips the array) into contiguous d_k (N, w_k): the picked
+ * records (x_L, x_U, lb) in one launch (pick_out_batch,
+ * branching_domains.py:831-962). */
+int gh_gather_rows_f64(int64_t N, const int64_t *idx, int w0, const double *s0, double *d0,
+                       int w1, const double *s1, double *d1, int w2, const double *s2,
+                       double *d2, hipStream_t stream);
 
 /* ------------------------------------------------ expression programs */
 
