@@ -36,6 +36,69 @@ def is_primal_dual_converged(P, D, abs_tol=1e-5, rel_tol=0.0):
     return gap / max(1.0, abs(P), abs(D)) <= rel_tol
 
 
+def cut_rows(lA, lbias, thr):
+    """Cut rows (B, S (n + 1)) lA x + (lbias - thr) <= 0 of planes
+    lA (B, S, n), lbias (B, S); a row with thr = +inf (the objective before
+    an incumbent exists) is the inactive 0 x - 1 <= 0
+    (construct_cut_constraints, complete_verifier/input_split/utils.py:
+    113-129)."""
+    B, S, n = lA.shape
+    out = torch.empty(B, S * (n + 1), dtype=torch.float64, device=lA.device)
+    act = torch.isfinite(thr)
+    out[:, :S * n].view(B, S, n).copy_(torch.where(act[None, :, None], lA, 0.0))
+    out[:, S * n:].copy_(torch.where(act[None, :], lbias - thr, -1.0))
+    return out
+
+
+def no_cuts_row(n, S, device):
+    """The stored cuts of a domain without any: every row inactive."""
+    nc = torch.zeros(1, S * (n + 1), dtype=torch.float64, device=device)
+    nc[0, S * n:] = -1.0
+    return nc
+
+
+def reordered_step(f, bound, thr, nb, n, S, planes=True, lp=None, clip=False,
+                   clip_max_rows=16, sync=True):
+    """One reordered B&B step on the last nb domains of frontier f
+    (batch_branch_and_bound.py:395-560 with reorder_bab): bound them
+    (CROWN; with clip 'complete' concretized over the box and the cuts they
+    carry, :420-428), never below the key they were stored with
+    (compare_with_old_bounds, bounding.py:361-363), [lp(lb, lA, lbias, L, U):
+    the LP certificate merge], branch on the widest input and bisect,
+    [clip the children by the parents' planes, :523-534], key each child
+    by its parent's bound and objective plane (:189-225), prune and append
+    (children of a verified parent inherit its verified rows, so this one
+    prune drops them).  bound(L, U, return_A=, plane_rows=) is the bound
+    of the served form (FP64 or FP32-sound)."""
+    pL, pU, pkey = f.pick_out(nb)
+    if planes:
+        # the LP and the cuts read every row's plane, the child keys the
+        # objective's
+        lb, lA, lbias = bound(pL, pU, return_A=True, plane_rows=S if (lp or clip) else 1)
+    else:
+        lb, lA, lbias = bound(pL, pU), None, None
+    cuts = None
+    if clip:
+        from .concretize import constrained_bound
+        stored = f.pick_payload(nb)
+        constrained_bound(lA, lbias, stored[:, :S * n].view(nb, S, n), stored[:, S * n:],
+                          pL, pU, lb, max_rows=clip_max_rows)
+        # the domains' planes become their children's cuts
+        cuts = cut_rows(lA, lbias, thr)
+    lb = torch.maximum(lb, pkey)
+    if lp is not None and lA is not None:
+        lp(lb, lA, lbias, pL, pU)
+    if clip:
+        from .concretize import clip_domains
+        idx = input_split_branching(None, None, pL, pU, None, None, "naive")
+        cL, cU, _ = input_split_parallel(pL, pU, i_idx=idx)
+        clip_domains(lA, lbias, thr, cL, cU, out=(cL, cU))
+        keys = child_keys(lb, lA, lbias, cL, cU)
+    else:
+        cL, cU, keys = branch_split_keys(lb, pL, pU, lA, lbias)
+    return f.filter_append(keys, thr, cL, cU, sync=sync, payload=cuts)
+
+
 @dataclass
 class BaBConfig:
     batch: int = 16384             # 2 batch domains bounded per iteration
@@ -92,11 +155,7 @@ class QCQPBranchAndBound:
         self.frontier = DeviceFrontier(n, S, self.cfg.cap, self.device,
                                        payload=S * (n + 1) if self.clip else 0)
         # the cut rows of a domain without any: inactive (0 x - 1 <= 0)
-        self._no_cuts = None
-        if self.clip:
-            nc = torch.zeros(1, S * (n + 1), dtype=torch.float64, device=self.device)
-            nc[0, S * n:] = -1.0
-            self._no_cuts = nc
+        self._no_cuts = no_cuts_row(n, S, self.device) if self.clip else None
         self.alm = ALMPGDSolver(self.cfg.alm)
         self.P = float("inf")
         self.x_best = None
@@ -188,15 +247,10 @@ class QCQPBranchAndBound:
 
     def step_reordered(self, batch=None, sync=True):
         """The reordered iteration (batch_branch_and_bound.py:500-560 with
-        reorder_bab): sort, pick `batch` domains, bound them (their bound is
-        the larger of the new CROWN bound and the key they were stored
-        with), [LP certificate], branch and split, give each child the key
-        of gh_child_keys_f64 (the parent's bound, raised on the objective row
-        by the parent's plane over the child box), prune and append the
-        children.  Children of a verified parent inherit its verified rows,
-        so the one prune of the children drops them.  Like the reference
-        (batch_branch_and_bound.py:1387-1389) it picks 2 `batch` domains, so
-        an iteration bounds as many domains as the regular one."""
+        reorder_bab): sort, then reordered_step on 2 `batch` domains.  Like
+        the reference (batch_branch_and_bound.py:1387-1389) it picks
+        2 `batch` domains, so an iteration bounds as many domains as the
+        regular one."""
         f = self.frontier
         si = self.cfg.sort_interval
         if si > 0 and self.stats.iterations % si == 0:
@@ -204,54 +258,16 @@ class QCQPBranchAndBound:
         nb = min(2 * (batch or self.cfg.batch), len(f), f.cap - len(f))
         if nb <= 0:
             return 0
-        pL, pU, pkey = f.pick_out(nb)
-        large = getattr(self.net, "large", False)
-        lp = self.cfg.dual_lp_iters > 0 and self.P < float("inf")
-        S, n = self.p.S, self.p.n
-        if large:
-            lb = self.net.bound(pL, pU)
-            lA = lbias = None
-        else:
-            # the LP and the cuts read every row's plane, the child keys the
-            # objective's
-            lb, lA, lbias = self.net.bound(pL, pU, return_A=True,
-                                           plane_rows=S if (lp or self.clip) else 1)
-        cuts = None
-        if self.clip:
-            from .concretize import clip_domains, constrained_bound
-            stored = f.pick_payload(nb)
-            constrained_bound(lA, lbias, stored[:, :S * n].view(nb, S, n), stored[:, S * n:],
-                              pL, pU, lb, max_rows=self.cfg.clip_max_rows)
-            # the domains' planes become their children's cuts
-            # (construct_cut_constraints, input_split/utils.py:113-129)
-            cuts = self.cut_rows(lA, lbias)
-        lb = torch.maximum(lb, pkey)
-        if lA is not None and lp:
-            self.raise_by_lp(lb, lA, lbias, pL, pU)
-        if self.clip:
-            idx = input_split_branching(self.net, None, pL, pU, None, None, "naive")
-            cL, cU, _ = input_split_parallel(pL, pU, i_idx=idx)
-            clip_domains(lA, lbias, self.thr, cL, cU, out=(cL, cU))
-            keys = child_keys(lb, lA, lbias, cL, cU)
-        else:
-            cL, cU, keys = branch_split_keys(lb, pL, pU, lA, lbias)
-        f.filter_append(keys, self.thr, cL, cU, sync=sync, payload=cuts)
+        lp = self.raise_by_lp if (self.cfg.dual_lp_iters > 0 and self.P < float("inf")) else None
+        reordered_step(f, self.net.bound, self.thr, nb, self.p.n, self.p.S,
+                       planes=not getattr(self.net, "large", False), lp=lp, clip=self.clip,
+                       clip_max_rows=self.cfg.clip_max_rows, sync=sync)
         self.stats.iterations += 1
         self.stats.boxes_bounded += nb
         return nb
 
     def cut_rows(self, lA, lbias):
-        """Cut rows (B, S (n + 1)) lA x + (lbias - thr) <= 0 of planes
-        lA (B, S, n), lbias (B, S); a row with thr = +inf (the objective
-        before an incumbent exists) is the inactive 0 x - 1 <= 0
-        (construct_cut_constraints, complete_verifier/input_split/utils.py:
-        113-129)."""
-        B, S, n = lA.shape
-        out = torch.empty(B, S * (n + 1), dtype=torch.float64, device=lA.device)
-        act = torch.isfinite(self.thr)
-        out[:, :S * n].view(B, S, n).copy_(torch.where(act[None, :, None], lA, 0.0))
-        out[:, S * n:].copy_(torch.where(act[None, :], lbias - self.thr, -1.0))
-        return out
+        return cut_rows(lA, lbias, self.thr)
 
     def raise_by_lp(self, lb, lA, lbias, cL, cU):
         """Solving-mode merge of the LP certificate into the objective row

@@ -30,8 +30,8 @@ import numpy as np
 import torch
 
 from .alm import ALMConfig, ALMPGDSolver, replay_float64
-from .bab import is_primal_dual_converged
-from .crown import (BoundedQCQP, DeviceFrontier, branch_split_keys, input_split_branching,
+from .bab import is_primal_dual_converged, no_cuts_row, reordered_step
+from .crown import (BoundedQCQP, DeviceFrontier, input_split_branching,
                     input_split_parallel)
 from .problems import flops_per_box, qcqp_dense
 
@@ -89,6 +89,11 @@ class ServeConfig:
     # the reordered step of the GEAR presets (reorder_bab: true): bound the
     # picked domains and key their children by the parents' planes
     reorder: bool = True
+    # clip_type of the presets' clip_input_domain: "complete" stores the
+    # parent's planes as cuts with each domain, concretizes over them and
+    # clips children (n <= 64 instances); "none": box concretization
+    clip: str = "none"
+    clip_max_rows: int = 16
 
 
 def batch_size(n, S, cfg):
@@ -99,17 +104,24 @@ def batch_size(n, S, cfg):
     return int(min(cfg.max_batch, max(cfg.min_batch, b)))
 
 
+def _row_bytes(n, S, cfg):
+    row = (2 * n + S) * 8
+    if cfg.clip == "complete" and n <= 64:
+        row += S * (n + 1) * 8             # the stored cut rows
+    return row
+
+
 def frontier_cap(n, S, cfg):
     """Frontier rows of a session: at most cfg.cap rows and cfg.session_bytes
     bytes, at least 4096 rows."""
-    row = (2 * n + S) * 8
+    row = _row_bytes(n, S, cfg)
     return int(max(4096, min(cfg.cap, cfg.session_bytes // row)))
 
 
 def footprint(n, S, cfg):
     """Device bytes a session holds: frontier, one step's children and
     bound / filter workspaces and planes, program data and plans."""
-    row = (2 * n + S) * 8
+    row = _row_bytes(n, S, cfg)
     nb = batch_size(n, S, cfg)
     child = 2 * nb * (row + 64) + 2 * nb * 8
     if cfg.reorder:                    # 2 nb parents and their 4 nb children
@@ -160,7 +172,9 @@ class Session:
             else:
                 self.dev = p.to_device(self.device)
                 self.bound = self.bound_fn(self.dev)
-            self.frontier = DeviceFrontier(p.n, p.S, frontier_cap(p.n, p.S, cfg), self.device)
+            self.clip = cfg.clip == "complete" and self.planes and p.n <= 64
+            self.frontier = DeviceFrontier(p.n, p.S, frontier_cap(p.n, p.S, cfg), self.device,
+                                           payload=p.S * (p.n + 1) if self.clip else 0)
             thr = torch.as_tensor(p.rhs, dtype=torch.float64).clone()
             thr[0] = math.inf
             thr[1:] += cfg.constr_viol_tol
@@ -174,7 +188,9 @@ class Session:
                 idx = input_split_branching(None, None, L, U, None, None, "naive")
                 L, U, _ = input_split_parallel(L, U, i_idx=idx)
             lb = self.bound(L, U)
-            self.frontier.filter_append(lb, self.thr, L, U, sync=False)
+            self.frontier.filter_append(lb, self.thr, L, U, sync=False,
+                                        payload=no_cuts_row(p.n, p.S, self.device)
+                                        if self.clip else None)
             self.boxes += L.shape[0]
             self._host = torch.empty(3, dtype=torch.float64, pin_memory=True)
             self._glb = torch.empty(1, dtype=torch.float64, device=self.device)
@@ -238,18 +254,9 @@ class Session:
         returns the domains picked."""
         nb = min(2 * self.batch, len(f), f.cap - len(f))
         if nb > 0:
-            pL, pU, pkey = f.pick_out(nb)
-            lp = self._lp()
-            if self.planes:
-                lb, lA, lbias = self.bound(pL, pU, return_A=True,
-                                          plane_rows=self.p.S if lp else 1)
-            else:
-                lb, lA, lbias = self.bound(pL, pU), None, None
-            lb = torch.maximum(lb, pkey)
-            if lp:
-                self._raise_by_lp(lb, lA, lbias, pL, pU)
-            cL, cU, keys = branch_split_keys(lb, pL, pU, lA, lbias)
-            f.filter_append(keys, self.thr, cL, cU, sync=False)
+            reordered_step(f, self.bound, self.thr, nb, self.p.n, self.p.S, planes=self.planes,
+                           lp=self._raise_by_lp if self._lp() else None, clip=self.clip,
+                           clip_max_rows=self.cfg.clip_max_rows, sync=False)
             self._bounded = nb
         return nb
 
