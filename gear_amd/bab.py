@@ -255,7 +255,14 @@ class QCQPBranchAndBound:
         si = self.cfg.sort_interval
         if si > 0 and self.stats.iterations % si == 0:
             f.sort(self.thr)
-        nb = min(2 * (batch or self.cfg.batch), len(f), f.cap - len(f))
+        want = 2 * (batch or self.cfg.batch)
+        nb = min(want, len(f), f.cap - len(f))
+        if nb < want and f.unsynced:
+            # dead entries of unsynced appends take room: drop them (a sync)
+            f.compact_order()
+            if si > 0:
+                f.sort(self.thr)
+            nb = min(want, len(f), f.cap - len(f))
         if nb <= 0:
             return 0
         lp = self.raise_by_lp if (self.cfg.dual_lp_iters > 0 and self.P < float("inf")) else None

@@ -250,7 +250,15 @@ class DeviceFrontier:
     ids, so re-sorting the frontier moves 8 bytes per domain, not the
     record.  Appends go to a physical high-water mark; when the slack above
     it runs out, the live records are compacted in logical order.  `L`,
-    `U`, `lb`, `pay` and `rows()` give logical-order copies."""
+    `U`, `lb`, `pay` and `rows()` give logical-order copies.
+
+    An append without a host sync (filter_append(sync=False)) gives all B
+    children a logical slot: survivors their physical rows, the others the
+    dead record (lb = +inf, an empty box), so the next step can be issued
+    before the survivor count is read.  Dead entries sort to the front,
+    never lower the global bound and are pruned if picked; `count` is then
+    an upper bound on the live rows until compact_order() (a sync) drops
+    them."""
 
     def __init__(self, n, S, cap, device="cuda", payload=0, slack=None):
         self.n, self.S, self.cap = n, S, int(cap)
@@ -260,6 +268,7 @@ class DeviceFrontier:
         # compaction (children are appended before their parents' rows are
         # reclaimed)
         self.phys = self.cap + int(slack if slack is not None else max(self.cap // 2, 4096))
+        self.dead = self.phys           # the dead record's physical row
         self._alloc(self.phys)
         self.order = torch.empty(self.cap, dtype=torch.int64, device=self.device)
         self._order2 = torch.empty_like(self.order)    # the sort's other buffer
@@ -274,10 +283,18 @@ class DeviceFrontier:
 
     def _alloc(self, rows):
         d = dict(dtype=torch.float64, device=self.device)
-        self._L = torch.empty(rows, self.n, **d)
-        self._U = torch.empty(rows, self.n, **d)
-        self._lb = torch.empty(rows, self.S, **d)
-        self._pay = torch.empty(rows, self.W, **d) if self.W else None
+        self._L = torch.empty(rows + 1, self.n, **d)
+        self._U = torch.empty(rows + 1, self.n, **d)
+        self._lb = torch.empty(rows + 1, self.S, **d)
+        self._pay = torch.empty(rows + 1, self.W, **d) if self.W else None
+        # the dead record: an empty box whose bound is +inf on every row
+        self._L[rows] = 1.0
+        self._U[rows] = 0.0
+        self._lb[rows] = float("inf")
+        if self._pay is not None:
+            self._pay[rows] = 0.0
+            self._pay[rows, self.S * self.n:] = -1.0
+        self.unsynced = 0         # appends since the last exact count
 
     def __len__(self):
         return self.count
@@ -329,7 +346,9 @@ class DeviceFrontier:
             return
         live = self.order[:self.count]
         old = (self._L, self._U, self._lb, self._pay)
+        unsynced = self.unsynced
         self._alloc(self.phys)
+        self.unsynced = unsynced
         for src, dst in zip(old, (self._L, self._U, self._lb, self._pay)):
             if src is not None:
                 torch.index_select(src, 0, live, out=dst[:self.count])
@@ -397,6 +416,7 @@ class DeviceFrontier:
         if self.count + B > self.cap:
             raise RuntimeError("frontier capacity exceeded")
         self._room(B)
+        self._last_B = B
         ws = self.workspace(B)
         check(lib().gh_filter_append_f64(B, self.S, self.n, ptr(lb), ptr(thr), ptr(x_L),
                                          ptr(x_U), ptr(self._L), ptr(self._U), ptr(self._lb),
@@ -410,25 +430,49 @@ class DeviceFrontier:
             check(lib().gh_filter_payload_f64(B, self.W, payload.shape[0], ptr(payload), ptr(ws),
                                               ptr(self._pay), self.hw, self.phys, stream_ptr()),
                   "gh_filter_payload_f64")
-        # survivor r sits in physical row hw + r; slots past the survivor
-        # count are written too and never read
-        torch.add(self._iota[:B], self.hw, out=self.order[self.count:self.count + B])
+        # survivor r sits in physical row hw + r; the other children's slots
+        # name the dead record
+        check(lib().gh_fill_order_i64(B, ptr(self.total), self.hw, self.dead,
+                                      ptr(self.order[self.count:self.count + B]), stream_ptr()),
+              "gh_fill_order_i64")
         self._pending = B
         if sync:
             return self.finish_append()
+        # without the count: every child keeps its slot (dead ones included)
+        self.count += B
+        self.hw += B
+        self.unsynced += 1
+        self._pending = None
         return None
 
     def commit(self, kept):
-        """Account the survivors of the last filter_append (read from
-        `total`)."""
-        self.count += kept
-        self.hw += kept
-        self._pending = None
+        """Account the survivors of the last filter_append(sync=False),
+        read from `total` later, when the caller wants an exact count:
+        drops that append's dead slots (the last B - kept)."""
+        B = self._last_B
+        self.count -= B - kept
+        self.hw -= B - kept
+        self.unsynced -= 1
 
     def finish_append(self):
         kept = int(self.total.item())
-        self.commit(kept)
+        self.count += kept
+        self.hw += kept
+        self._pending = None
         return kept
+
+    def compact_order(self):
+        """Drop the dead entries of unsynced appends from the logical order
+        (one host sync); `count` is exact afterwards."""
+        if self.unsynced == 0:
+            return self.count
+        o = self.order[:self.count]
+        live = o[o != self.dead]
+        c = live.numel()
+        self.order[:c] = live
+        self.count = c
+        self.unsynced = 0
+        return c
 
     def global_lb(self, out=None):
         """min(lb[:, 0] over the frontier, discarded-threshold minimum) as a

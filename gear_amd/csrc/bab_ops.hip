@@ -244,6 +244,15 @@ __global__ void frontier_keys_kernel(int64_t N, int S, const int64_t* __restrict
     keys[i] = nan ? NAN : m;
 }
 
+// Slot ids of the children just appended: child t < *total (a survivor)
+// sits in physical row hw + t; the others get the dead record's slot.
+__global__ void fill_order_kernel(int B, const int64_t* __restrict__ total, int64_t hw,
+                                  int64_t dead, int64_t* __restrict__ out)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < B) out[t] = t < *total ? hw + t : dead;
+}
+
 // Rows idx[i] of up to three row-major arrays (widths w0, w1, w2) into
 // contiguous outputs, one thread per element.
 __global__ void gather_rows_kernel(int64_t N, const int64_t* __restrict__ idx, int w0,
@@ -377,6 +386,16 @@ extern "C" int gh_frontier_keys_f64(int64_t N, int S, const int64_t* order, cons
     if (N == 0) return GH_OK;
     hipLaunchKernelGGL(frontier_keys_kernel, dim3((unsigned)((N + 255) / 256)), dim3(256), 0,
                        stream, N, S, order, lb, thr, keys);
+    return status();
+}
+
+extern "C" int gh_fill_order_i64(int B, const int64_t* total, int64_t hw, int64_t dead,
+                                 int64_t* out, hipStream_t stream)
+{
+    if (B < 0 || (B > 0 && (!total || !out))) return GH_EINVAL;
+    if (B == 0) return GH_OK;
+    hipLaunchKernelGGL(fill_order_kernel, dim3((B + 255) / 256), dim3(256), 0, stream, B, total,
+                       hw, dead, out);
     return status();
 }
 

@@ -350,6 +350,12 @@ int gh_frontier_keys_f64(int64_t N, int S, const int64_t *order, const double *l
                          const double *thr, double *keys, hipStream_t stream);
 int gh_min_rows_f64(int64_t N, int64_t stride, const int64_t *idx, const double *x, double *out,
                     hipStream_t stream);
+/* Slot ids of B children just appended by gh_filter_append_f64 at
+ * physical row hw: out[t] = hw + t for the *total survivors, `dead` (the
+ * frontier's dead record) for the rest, so the host need not read the
+ * survivor count before the next step. */
+int gh_fill_order_i64(int B, const int64_t *total, int64_t hw, int64_t dead, int64_t *out,
+                      hipStream_t stream);
 /* Rows idx[i] (i < N) of up to three row-major arrays s_k of widths w_k
  * (a width 0 skips the array) into contiguous d_k (N, w_k): the picked
  * records (x_L, x_U, lb) in one launch (pick_out_batch,

@@ -274,3 +274,28 @@ def test_branch_split_keys_matches_oracle(cuda, K, n, m):
     # without planes the keys are the parents' bounds
     _, _, k0 = branch_split_keys(_dev(plb, cuda), _dev(L, cuda), _dev(U, cuda))
     assert _bits_equal(k0.cpu().numpy(), np.vstack([plb, plb]))
+
+
+def test_unsynced_steps_equal_synced(cuda):
+    """Steps issued without reading the survivor count (dead slots for the
+    pruned children) leave, after compact_order, the same frontier and
+    global bound as steps that read it, bit for bit."""
+    from gear_amd.bab import BaBConfig, QCQPBranchAndBound
+    p = qcqp_dense(12, 4, seed=2)
+    out = []
+    for sync in (True, False):
+        bb = QCQPBranchAndBound(p, BaBConfig(batch=64, cap=1 << 14), device=cuda)
+        bb.set_incumbent(-0.8, None)      # prunes part of every step's children
+        bb.presplit(256, alm=False)       # every step picks a full 128
+        for _ in range(8):
+            bb.step(sync=sync)
+        if not sync:
+            assert bb.frontier.unsynced > 0
+        c = bb.frontier.compact_order()
+        assert bb.frontier.unsynced == 0
+        L, U, lb = bb.frontier.rows()
+        out.append((c, L.cpu().numpy(), U.cpu().numpy(), lb.cpu().numpy(),
+                    float(bb.frontier.global_lb().item()), bb.stats.boxes_bounded))
+    (c0, L0, U0, lb0, g0, n0), (c1, L1, U1, lb1, g1, n1) = out
+    assert c0 == c1 and n0 == n1 and g0 == g1
+    assert _bits_equal(L0, L1) and _bits_equal(U0, U1) and _bits_equal(lb0, lb1)
