@@ -70,14 +70,44 @@ __global__ void qcqp_pack_kernel(int n, int S, int nch, int kt, const double* __
     QA[u] = (j2 < n && i2 < n) ? Qs[(size_t)j2 * n + i2] : 0.0;
 }
 
+// Direct global -> LDS copy of one row's packed Q tiles (rowstride doubles
+// of QB, then of QA) by the W waves of a workgroup: gfx950's
+// global_load_lds writes lane l's 16 bytes at the wave's LDS base + 16 l,
+// so no VGPR holds the data.  Asynchronous: the caller waits (vmcnt) and
+// synchronizes before reading the buffer.
+template <int W>
+__device__ __forceinline__ void stage_row(const double* __restrict__ QB,
+                                          const double* __restrict__ QA, size_t rowstride, int s,
+                                          double* dst, int wv, int lane)
+{
+    const int chunks = (int)(rowstride / 2);   // 16-byte pieces per half
+    const double* qb = QB + (size_t)s * rowstride;
+    const double* qa = QA + (size_t)s * rowstride;
+    for (int g = wv; g * 64 < chunks; g += W) {
+        const int ch = g * 64 + lane;
+        if (ch < chunks) {
+            __builtin_amdgcn_global_load_lds(
+                (const void*)(qb + 2 * ch),
+                (__attribute__((address_space(3))) void*)(dst + (size_t)g * 128), 16, 0, 0);
+            __builtin_amdgcn_global_load_lds(
+                (const void*)(qa + 2 * ch),
+                (__attribute__((address_space(3))) void*)(dst + rowstride + (size_t)g * 128), 16,
+                0, 0);
+        }
+    }
+}
+
 // One wavefront per 16 boxes, one box per lane column: lane (q, r) = lane
 // 16q + r owns box r and the variable indices q + 4m, m < KT = ceil(n/4).
 // The MFMAs take Q tiles as the A operand and the lane's box vectors as the
 // B operand, so every per-box vector (box rows, Cm + alpha, beta) stays in
-// the lane's registers and no LDS is used.  IBP tiles are processed in
-// groups of two to bound the live accumulators.
-template <int KT>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void qcqp_crown_f64_kernel(
+// the lane's registers.  IBP tiles are processed in groups of two to bound
+// the live accumulators.  STAGED: 8 wavefronts (128 boxes) share a
+// workgroup whose LDS holds the current row's Q tiles while the next row's
+// are copied in (double buffer), so the MFMA operands come from LDS rather
+// than from L2; the arithmetic and its order are the same either way.
+template <int KT, bool STAGED>
+__global__ __launch_bounds__(STAGED ? 512 : 64) __attribute__((amdgpu_waves_per_eu(2))) void qcqp_crown_f64_kernel(
     int B, int n, int S, const double* __restrict__ QB, const double* __restrict__ QA,
     const double* __restrict__ Cm, const double* __restrict__ d, const double* __restrict__ xL,
     const double* __restrict__ xU, double* __restrict__ lb, double* __restrict__ lA,
@@ -86,9 +116,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void qc
     constexpr int NCH = (KT + 3) / 4;    // 16-wide tiles of j (and of i)
     constexpr int NM = KT;               // per-lane indices q + 4m
     constexpr int G = (KT > 12) ? 1 : 2; // IBP tiles per group
-    const int l = threadIdx.x;
+    constexpr int W = STAGED ? 8 : 1;     // wavefronts per workgroup
+    extern __shared__ double qsm[];       // STAGED: 2 x [QB row | QA row]
+    const int l = threadIdx.x & 63;
+    const int wv = threadIdx.x >> 6;
     const int lr = l & 15, lq = l >> 4;
-    const int box = blockIdx.x * 16 + lr;
+    const int box = (blockIdx.x * W + wv) * 16 + lr;
     const int bo = min(box, B - 1);      // rows past B compute on box B-1, never stored
     const bool live = box < B;
 
@@ -101,9 +134,24 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void qc
     }
 
     const size_t rowstride = (size_t)NCH * KT * 64;
+    if constexpr (STAGED) {
+        stage_row<W>(QB, QA, rowstride, 0, qsm, wv, l);
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+    }
     for (int s = 0; s < S; ++s) {
-        const double* qb = QB + (size_t)s * rowstride + l;
-        const double* qa = QA + (size_t)s * rowstride + l;
+        const double* qb;
+        const double* qa;
+        if constexpr (STAGED) {
+            if (s + 1 < S) stage_row<W>(QB, QA, rowstride, s + 1, qsm + ((s + 1) & 1) * 2 * rowstride,
+                                        wv, l);
+            const double* buf = qsm + (s & 1) * 2 * rowstride;
+            qb = buf + l;
+            qa = buf + rowstride + l;
+        } else {
+            qb = QB + (size_t)s * rowstride + l;
+            qa = QA + (size_t)s * rowstride + l;
+        }
         const double* cs = Cm + (size_t)s * n;
 
         double acm[NM], beta[NM];
@@ -275,17 +323,43 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void qc
             lb[(size_t)box * S + s] = gh::add_enclosed(bias, root, -1.0, k.k1);
             if (lbias && s < PR) lbias[(size_t)box * PR + s] = a;
         }
+        if constexpr (STAGED) {
+            // the next row's tiles have landed, and every wave is done with
+            // this row's buffer before it is refilled
+            __builtin_amdgcn_s_waitcnt(0);
+            __syncthreads();
+        }
     }
 }
+
+// Batches of at least this many boxes take the LDS-staged form (128 boxes
+// per workgroup; one workgroup per CU at 32,768 boxes).
+constexpr int kStagedMinBoxes = 16384;
 
 template <int KT>
 int launch_f64(int B, int n, int S, const double* QB, const double* QA, const double* Cm,
                const double* d, const double* xL, const double* xU, double* lb, double* lA,
                double* lbias, int PR, const QcqpConsts& k, hipStream_t stream)
 {
-    dim3 grid((B + 15) / 16);
-    hipLaunchKernelGGL(qcqp_crown_f64_kernel<KT>, grid, dim3(64), 0, stream, B, n, S, QB, QA,
-                       Cm, d, xL, xU, lb, lA, lbias, PR, k);
+    if (B >= kStagedMinBoxes) {
+        const size_t rowstride = (size_t)((KT + 3) / 4) * KT * 64;
+        const size_t lds = 4 * rowstride * sizeof(double);
+        static bool attr = false;   // once per instantiation (same value every call)
+        if (!attr) {
+            if (hipFuncSetAttribute((const void*)qcqp_crown_f64_kernel<KT, true>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)lds) != hipSuccess)
+                return GH_ELAUNCH;
+            attr = true;
+        }
+        dim3 grid((B + 127) / 128);
+        hipLaunchKernelGGL((qcqp_crown_f64_kernel<KT, true>), grid, dim3(512), lds, stream, B, n,
+                           S, QB, QA, Cm, d, xL, xU, lb, lA, lbias, PR, k);
+    } else {
+        dim3 grid((B + 15) / 16);
+        hipLaunchKernelGGL((qcqp_crown_f64_kernel<KT, false>), grid, dim3(64), 0, stream, B, n, S,
+                           QB, QA, Cm, d, xL, xU, lb, lA, lbias, PR, k);
+    }
     return hipGetLastError() == hipSuccess ? GH_OK : GH_ELAUNCH;
 }
 

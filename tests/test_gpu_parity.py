@@ -299,3 +299,23 @@ def test_unsynced_steps_equal_synced(cuda):
     (c0, L0, U0, lb0, g0, n0), (c1, L1, U1, lb1, g1, n1) = out
     assert c0 == c1 and n0 == n1 and g0 == g1
     assert _bits_equal(L0, L1) and _bits_equal(U0, U1) and _bits_equal(lb0, lb1)
+
+
+@pytest.mark.parametrize("n,m,B", [(50, 20, 16384 + 37), (17, 4, 20000)])
+def test_qcqp_crown_f64_staged_bit_exact(cuda, n, m, B):
+    """Batches of 16,384 boxes or more take the LDS-staged kernel (128 boxes
+    per workgroup, Q tiles double-buffered in LDS); bounds and planes stay
+    bit-identical to the oracle, including the partial last workgroup."""
+    import os
+    from gear_amd.crown import BoundedQCQP
+    print(f"case n={n} m={m} B={B}", flush=True)
+    p = qcqp_dense(n, m, seed=n * m)
+    L, U = random_bisection_boxes(p.xl, p.xu, B, seed=B)
+    lb_o, lA_o, lbias_o = O.qcqp_crown(p.Q, p.c, p.d, L, U, return_A=True,
+                                       threads=min(16, os.cpu_count() or 1))
+    net = BoundedQCQP(p, cuda)
+    lb, lA, lbias = net.bound(_dev(L, cuda), _dev(U, cuda), return_A=True)
+    torch.cuda.synchronize()
+    assert _bits_equal(lb.cpu().numpy(), lb_o)
+    assert _bits_equal(lA.cpu().numpy(), lA_o)
+    assert _bits_equal(lbias.cpu().numpy(), lbias_o)
