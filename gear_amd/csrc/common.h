@@ -107,4 +107,32 @@ __device__ __forceinline__ double tree4x4(const double p[4])
     return r;
 }
 
+// Direct global -> LDS copy of one row's packed tiles (rowstride elements of
+// A, then rowstride of B) by the W wavefronts of a workgroup: gfx950's
+// global_load_lds writes lane l's 16 bytes at the wave's LDS base + 16 l,
+// so no VGPR holds the data.  Asynchronous: the caller waits (vmcnt) and
+// synchronizes the workgroup before reading dst.  rowstride * sizeof(T)
+// must be a multiple of 16; the row must lie inside A and B.
+template <typename T, int W>
+__device__ __forceinline__ void stage_row(const T* __restrict__ A, const T* __restrict__ B,
+                                          size_t rowstride, int s, T* dst, int wv, int lane)
+{
+    constexpr int per = 16 / (int)sizeof(T);      // elements per 16-byte piece
+    const int chunks = (int)(rowstride / per);
+    const T* a = A + (size_t)s * rowstride;
+    const T* b = B + (size_t)s * rowstride;
+    for (int g = wv; g * 64 < chunks; g += W) {
+        const int ch = g * 64 + lane;
+        if (ch < chunks) {
+            __builtin_amdgcn_global_load_lds(
+                (const void*)(a + (size_t)per * ch),
+                (__attribute__((address_space(3))) void*)(dst + (size_t)g * 64 * per), 16, 0, 0);
+            __builtin_amdgcn_global_load_lds(
+                (const void*)(b + (size_t)per * ch),
+                (__attribute__((address_space(3))) void*)(dst + rowstride + (size_t)g * 64 * per),
+                16, 0, 0);
+        }
+    }
+}
+
 }  // namespace gh

@@ -70,33 +70,6 @@ __global__ void qcqp_pack_kernel(int n, int S, int nch, int kt, const double* __
     QA[u] = (j2 < n && i2 < n) ? Qs[(size_t)j2 * n + i2] : 0.0;
 }
 
-// Direct global -> LDS copy of one row's packed Q tiles (rowstride doubles
-// of QB, then of QA) by the W waves of a workgroup: gfx950's
-// global_load_lds writes lane l's 16 bytes at the wave's LDS base + 16 l,
-// so no VGPR holds the data.  Asynchronous: the caller waits (vmcnt) and
-// synchronizes before reading the buffer.
-template <int W>
-__device__ __forceinline__ void stage_row(const double* __restrict__ QB,
-                                          const double* __restrict__ QA, size_t rowstride, int s,
-                                          double* dst, int wv, int lane)
-{
-    const int chunks = (int)(rowstride / 2);   // 16-byte pieces per half
-    const double* qb = QB + (size_t)s * rowstride;
-    const double* qa = QA + (size_t)s * rowstride;
-    for (int g = wv; g * 64 < chunks; g += W) {
-        const int ch = g * 64 + lane;
-        if (ch < chunks) {
-            __builtin_amdgcn_global_load_lds(
-                (const void*)(qb + 2 * ch),
-                (__attribute__((address_space(3))) void*)(dst + (size_t)g * 128), 16, 0, 0);
-            __builtin_amdgcn_global_load_lds(
-                (const void*)(qa + 2 * ch),
-                (__attribute__((address_space(3))) void*)(dst + rowstride + (size_t)g * 128), 16,
-                0, 0);
-        }
-    }
-}
-
 // One wavefront per 16 boxes, one box per lane column: lane (q, r) = lane
 // 16q + r owns box r and the variable indices q + 4m, m < KT = ceil(n/4).
 // The MFMAs take Q tiles as the A operand and the lane's box vectors as the
@@ -135,7 +108,7 @@ __global__ __launch_bounds__(STAGED ? 512 : 64) __attribute__((amdgpu_waves_per_
 
     const size_t rowstride = (size_t)NCH * KT * 64;
     if constexpr (STAGED) {
-        stage_row<W>(QB, QA, rowstride, 0, qsm, wv, l);
+        gh::stage_row<double, W>(QB, QA, rowstride, 0, qsm, wv, l);
         __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
     }
@@ -143,8 +116,9 @@ __global__ __launch_bounds__(STAGED ? 512 : 64) __attribute__((amdgpu_waves_per_
         const double* qb;
         const double* qa;
         if constexpr (STAGED) {
-            if (s + 1 < S) stage_row<W>(QB, QA, rowstride, s + 1, qsm + ((s + 1) & 1) * 2 * rowstride,
-                                        wv, l);
+            if (s + 1 < S)
+                gh::stage_row<double, W>(QB, QA, rowstride, s + 1,
+                                         qsm + ((s + 1) & 1) * 2 * rowstride, wv, l);
             const double* buf = qsm + (s & 1) * 2 * rowstride;
             qb = buf + l;
             qa = buf + rowstride + l;

@@ -107,8 +107,11 @@ __device__ __forceinline__ float tree4x4f(const float p[4])
     return r;
 }
 
-template <int KT>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void qcqp_crown_f32s_kernel(
+// STAGED: 8 wavefronts (128 boxes) per workgroup share the current row's
+// FP32 Q tiles in LDS while the next row's are copied in (double buffer);
+// the arithmetic is the same either way.
+template <int KT, bool STAGED>
+__global__ __launch_bounds__(STAGED ? 512 : 64) __attribute__((amdgpu_waves_per_eu(2))) void qcqp_crown_f32s_kernel(
     int B, int n, int S, const float* __restrict__ QB, const float* __restrict__ QA,
     const double* __restrict__ Q64, const double* __restrict__ Cm, const double* __restrict__ d,
     const double* __restrict__ xL, const double* __restrict__ xU, double* __restrict__ lb,
@@ -118,9 +121,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void qc
     constexpr int NCH = (KT + 3) / 4;
     constexpr int NM = KT;
     constexpr int G = 2;
-    const int l = threadIdx.x;
+    constexpr int W = STAGED ? 8 : 1;
+    extern __shared__ float qsf[];       // STAGED: 2 x [QB row | QA row]
+    const int l = threadIdx.x & 63;
+    const int wv = threadIdx.x >> 6;
     const int lr = l & 15, lq = l >> 4;
-    const int box = blockIdx.x * 16 + lr;
+    const int box = (blockIdx.x * W + wv) * 16 + lr;
     const int bo = min(box, B - 1);
     const bool live = box < B;
 
@@ -134,9 +140,25 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void qc
     int ambiguous = 0;
 
     const size_t rowstride = (size_t)NCH * KT * 64;
+    if constexpr (STAGED) {
+        gh::stage_row<float, W>(QB, QA, rowstride, 0, qsf, wv, l);
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+    }
     for (int s = 0; s < S; ++s) {
-        const float* qb = QB + (size_t)s * rowstride + l;
-        const float* qa = QA + (size_t)s * rowstride + l;
+        const float* qb;
+        const float* qa;
+        if constexpr (STAGED) {
+            if (s + 1 < S)
+                gh::stage_row<float, W>(QB, QA, rowstride, s + 1,
+                                        qsf + ((s + 1) & 1) * 2 * rowstride, wv, l);
+            const float* buf = qsf + (s & 1) * 2 * rowstride;
+            qb = buf + l;
+            qa = buf + rowstride + l;
+        } else {
+            qb = QB + (size_t)s * rowstride + l;
+            qa = QA + (size_t)s * rowstride + l;
+        }
         const double* cs = Cm + (size_t)s * n;
 
         float acm[NM], beta[NM];
@@ -294,6 +316,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void qc
                 lbias[(size_t)box * PR + s] = isfinite(pb) ? pb : -INFINITY;
             }
         }
+        if constexpr (STAGED) {
+            __builtin_amdgcn_s_waitcnt(0);
+            __syncthreads();
+        }
     }
     // one flag per box: some plane choice was near a tie and was taken
     // from FP64 (diagnostic); box r's lanes are r, r + 16, r + 32, r + 48
@@ -307,9 +333,25 @@ int launch_f32(int B, int n, int S, const float* QB, const float* QA, const doub
                int* flag, double* lA, double* lbias, int PR, const F32Consts& k,
                hipStream_t stream)
 {
-    dim3 grid((B + 15) / 16);
-    hipLaunchKernelGGL(qcqp_crown_f32s_kernel<KT>, grid, dim3(64), 0, stream, B, n, S, QB, QA, Q64,
-                       Cm, d, xL, xU, lb, flag, lA, lbias, PR, k);
+    if (B >= 16384) {   // the LDS-staged form, 128 boxes per workgroup
+        const size_t rowstride = (size_t)((KT + 3) / 4) * KT * 64;
+        const size_t lds = 4 * rowstride * sizeof(float);
+        static bool attr = false;   // once per instantiation (same value every call)
+        if (!attr) {
+            if (hipFuncSetAttribute((const void*)qcqp_crown_f32s_kernel<KT, true>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)lds) != hipSuccess)
+                return GH_ELAUNCH;
+            attr = true;
+        }
+        hipLaunchKernelGGL((qcqp_crown_f32s_kernel<KT, true>), dim3((B + 127) / 128), dim3(512),
+                           lds, stream, B, n, S, QB, QA, Q64, Cm, d, xL, xU, lb, flag, lA, lbias,
+                           PR, k);
+    } else {
+        hipLaunchKernelGGL((qcqp_crown_f32s_kernel<KT, false>), dim3((B + 15) / 16), dim3(64), 0,
+                           stream, B, n, S, QB, QA, Q64, Cm, d, xL, xU, lb, flag, lA, lbias, PR,
+                           k);
+    }
     return hipGetLastError() == hipSuccess ? GH_OK : GH_ELAUNCH;
 }
 

@@ -91,3 +91,22 @@ def test_fp32s_planes_are_valid_lower_planes(cuda):
                                return_A=True, plane_rows=1)
     assert np.array_equal(lA1.cpu().numpy(), lA[:, :1])
     assert np.array_equal(lbias1.cpu().numpy(), lbias[:, :1])
+
+
+def test_fp32s_staged_equals_unstaged(cuda):
+    """The LDS-staged form (batches >= 16,384 boxes) computes the same bits
+    as the one-wavefront form on the same boxes (bounds, planes, flags)."""
+    from gear_amd.crown import BoundedQCQP
+    from gear_amd.problems import qcqp_dense, random_bisection_boxes
+    p = qcqp_dense(50, 20, seed=7)
+    L, U = random_bisection_boxes(p.xl, p.xu, 20000, seed=4)
+    tL, tU = torch.as_tensor(L, device=cuda), torch.as_tensor(U, device=cuda)
+    net = BoundedQCQP(p, cuda, precision="fp32s")
+    lb, lA, lbias = net.bound(tL, tU, return_A=True)
+    f = net.last_flag.clone()
+    parts = [net.bound(tL[a:a + 10000], tU[a:a + 10000], return_A=True) + (net.last_flag.clone(),)
+             for a in (0, 10000)]
+    for got, k in ((lb, 0), (lA, 1), (lbias, 2), (f, 3)):
+        ref = torch.cat([parts[0][k], parts[1][k]])
+        assert torch.equal(got.view(torch.int64) if got.dtype == torch.float64 else got,
+                           ref.view(torch.int64) if ref.dtype == torch.float64 else ref)
