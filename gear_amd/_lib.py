@@ -52,6 +52,7 @@ SIGNATURES = {
     "gh_min_rows_f64": [ctypes.c_int64, ctypes.c_int64, _P, _P, _P, _P],
     "gh_gather_rows_f64": [ctypes.c_int64, _P, _I, _P, _P, _I, _P, _P, _I, _P, _P, _P],
     "gh_fill_order_i64": [_I, _P, ctypes.c_int64, ctypes.c_int64, _P, _P],
+    "gh_merge_desc_f64": [ctypes.c_int64, _P, _P, ctypes.c_int64, _P, _P, _P, _P, _P],
     "gh_dag_plan_create": [_I, _P, ctypes.POINTER(_P)],
     "gh_dag_plan_destroy": [_P],
     "gh_dag_plan_info": [_P, _P, _P, _P],

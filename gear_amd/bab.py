@@ -227,7 +227,7 @@ class QCQPBranchAndBound:
         f = self.frontier
         si = self.cfg.sort_interval
         if si > 0 and self.stats.iterations % si == 0:
-            f.sort(self.thr)
+            f.sort(self.thr, tag=self.P)
         # children (2 nb) go where the nb parents were plus free rows
         nb = min(batch or self.cfg.batch, len(f), f.cap - len(f))
         if nb <= 0:
@@ -254,14 +254,14 @@ class QCQPBranchAndBound:
         f = self.frontier
         si = self.cfg.sort_interval
         if si > 0 and self.stats.iterations % si == 0:
-            f.sort(self.thr)
+            f.sort(self.thr, tag=self.P)
         want = 2 * (batch or self.cfg.batch)
         nb = min(want, len(f), f.cap - len(f))
         if nb < want and f.unsynced:
             # dead entries of unsynced appends take room: drop them (a sync)
             f.compact_order()
             if si > 0:
-                f.sort(self.thr)
+                f.sort(self.thr, tag=self.P)
             nb = min(want, len(f), f.cap - len(f))
         if nb <= 0:
             return 0

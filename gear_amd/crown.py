@@ -272,13 +272,18 @@ class DeviceFrontier:
         self._alloc(self.phys)
         self.order = torch.empty(self.cap, dtype=torch.int64, device=self.device)
         self._order2 = torch.empty_like(self.order)    # the sort's other buffer
+        # sort keys aligned with the order; order[:_sorted] is sorted under
+        # the thresholds named by _tag (appends and picks keep that prefix)
+        self._okeys = torch.empty(self.cap, dtype=torch.float64, device=self.device)
+        self._okeys2 = torch.empty_like(self._okeys)
+        self._sorted = 0
+        self._tag = None
         self._iota = torch.arange(self.cap, dtype=torch.int64, device=self.device)
         self.count = 0          # logical rows
         self.hw = 0             # physical rows in use (live or picked)
         self.total = torch.zeros(1, dtype=torch.int64, device=self.device)
         self.disc_min = torch.full((1,), float("inf"), dtype=torch.float64, device=self.device)
         self._ws = None
-        self._keys = None
         self._pending = None
 
     def _alloc(self, rows):
@@ -358,7 +363,7 @@ class DeviceFrontier:
         if self.hw + B > self.phys:
             raise RuntimeError("frontier capacity exceeded")
 
-    def sort(self, thr):
+    def sort(self, thr, tag=None):
         """Order the rows by their largest margin max_s (lb_s - thr_s),
         descending, so picking from the end takes the least verified domains
         first: UnsortedInputDomainList.sort with sort_heuristic 'lb_rhs',
@@ -366,19 +371,34 @@ class DeviceFrontier:
         (complete_verifier/input_split/branching_domains.py:1171-1241,
         976-981; gear/presets/gear-f0.yaml:409, 550-552).  Ties keep their
         current order (a stable sort; the reference's argsort leaves tie
-        order unspecified).  Only the slot ids move."""
+        order unspecified).  Only the slot ids move.  tag names thr (e.g.
+        the incumbent it holds): when it matches the last sort's, only the
+        rows appended since are sorted and merged in."""
         c = self.count
-        if c <= 1:
+        thr = thr.contiguous()
+        p = self._sorted if (tag is not None and tag == self._tag) else 0
+        if p >= c:
+            self._sorted, self._tag = c, tag
             return
-        if self._keys is None or self._keys.numel() < c:
-            self._keys = torch.empty(self.cap, dtype=torch.float64, device=self.device)
-        keys = self._keys[:c]
-        check(lib().gh_frontier_keys_f64(c, self.S, ptr(self.order), ptr(self._lb),
-                                         ptr(thr.contiguous()), ptr(keys), stream_ptr()),
+        # keys of the unsorted tail (all rows when nothing is known sorted)
+        check(lib().gh_frontier_keys_f64(c - p, self.S, ptr(self.order[p:]), ptr(self._lb),
+                                         ptr(thr), ptr(self._okeys[p:c]), stream_ptr()),
               "gh_frontier_keys_f64")
-        perm = torch.argsort(keys, descending=True, stable=True)
-        torch.index_select(self.order[:c], 0, perm, out=self._order2[:c])
+        perm = torch.argsort(self._okeys[p:c], descending=True, stable=True)
+        if p == 0:
+            torch.index_select(self.order[:c], 0, perm, out=self._order2[:c])
+            torch.index_select(self._okeys[:c], 0, perm, out=self._okeys2[:c])
+        else:
+            # sort the tail, then merge it behind the sorted prefix (ties:
+            # prefix first) -- the stable sort of the whole
+            tk = self._okeys[p:c].index_select(0, perm)
+            tv = self.order[p:c].index_select(0, perm)
+            check(lib().gh_merge_desc_f64(p, ptr(self._okeys), ptr(self.order), c - p, ptr(tk),
+                                          ptr(tv), ptr(self._okeys2), ptr(self._order2),
+                                          stream_ptr()), "gh_merge_desc_f64")
         self.order, self._order2 = self._order2, self.order
+        self._okeys, self._okeys2 = self._okeys2, self._okeys
+        self._sorted, self._tag = c, tag
 
     def pick_out(self, batch):
         """(x_L, x_U, lb) of the last `batch` logical rows (the parents of the
@@ -386,6 +406,7 @@ class DeviceFrontier:
         nb = min(batch, self.count)
         s = self.count - nb
         self.count = s
+        self._sorted = min(self._sorted, s)
         idx = self.order[s:s + nb]
         self._picked = idx
         d = dict(dtype=torch.float64, device=self.device)
@@ -452,6 +473,7 @@ class DeviceFrontier:
         B = self._last_B
         self.count -= B - kept
         self.hw -= B - kept
+        self._sorted = min(self._sorted, self.count)
         self.unsynced -= 1
 
     def finish_append(self):
@@ -467,11 +489,13 @@ class DeviceFrontier:
         if self.unsynced == 0:
             return self.count
         o = self.order[:self.count]
-        live = o[o != self.dead]
+        keep = o != self.dead
+        live = o[keep]
         c = live.numel()
         self.order[:c] = live
         self.count = c
         self.unsynced = 0
+        self._sorted = 0            # keys are recomputed by the next sort
         return c
 
     def global_lb(self, out=None):

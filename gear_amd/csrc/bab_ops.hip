@@ -244,6 +244,49 @@ __global__ void frontier_keys_kernel(int64_t N, int S, const int64_t* __restrict
     keys[i] = nan ? NAN : m;
 }
 
+// Descending-order rank key of a sort key: every NaN first (torch's
+// descending sort), then numbers from +inf down; -0 equals +0.
+__device__ __forceinline__ unsigned long long desc_key(double x)
+{
+    if (isnan(x)) return ~0ull;
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, x + 0.0);
+    return (u >> 63) ? ~u : (u | 0x8000000000000000ull);
+}
+
+// Stable merge of two descending runs (A before B on ties): element i of A
+// goes to i + #{B strictly before it}, element j of B to j + #{A before or
+// tied with it}; both counts by binary search in the other run.
+__global__ void merge_desc_kernel(int64_t nA, const double* __restrict__ kA,
+                                  const int64_t* __restrict__ vA, int64_t nB,
+                                  const double* __restrict__ kB, const int64_t* __restrict__ vB,
+                                  double* __restrict__ kO, int64_t* __restrict__ vO)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nA + nB) return;
+    if (t < nA) {
+        const unsigned long long x = desc_key(kA[t]);
+        int64_t lo = 0, hi = nB;            // count B with key > x
+        while (lo < hi) {
+            const int64_t mid = (lo + hi) >> 1;
+            if (desc_key(kB[mid]) > x) lo = mid + 1;
+            else hi = mid;
+        }
+        kO[t + lo] = kA[t];
+        vO[t + lo] = vA[t];
+    } else {
+        const int64_t j = t - nA;
+        const unsigned long long x = desc_key(kB[j]);
+        int64_t lo = 0, hi = nA;            // count A with key >= x
+        while (lo < hi) {
+            const int64_t mid = (lo + hi) >> 1;
+            if (desc_key(kA[mid]) >= x) lo = mid + 1;
+            else hi = mid;
+        }
+        kO[j + lo] = kB[j];
+        vO[j + lo] = vB[j];
+    }
+}
+
 // Slot ids of the children just appended: child t < *total (a survivor)
 // sits in physical row hw + t; the others get the dead record's slot.
 __global__ void fill_order_kernel(int B, const int64_t* __restrict__ total, int64_t hw,
@@ -386,6 +429,19 @@ extern "C" int gh_frontier_keys_f64(int64_t N, int S, const int64_t* order, cons
     if (N == 0) return GH_OK;
     hipLaunchKernelGGL(frontier_keys_kernel, dim3((unsigned)((N + 255) / 256)), dim3(256), 0,
                        stream, N, S, order, lb, thr, keys);
+    return status();
+}
+
+extern "C" int gh_merge_desc_f64(int64_t nA, const double* kA, const int64_t* vA, int64_t nB,
+                                 const double* kB, const int64_t* vB, double* kO, int64_t* vO,
+                                 hipStream_t stream)
+{
+    if (nA < 0 || nB < 0) return GH_EINVAL;
+    const int64_t N = nA + nB;
+    if (N == 0) return GH_OK;
+    if ((nA && (!kA || !vA)) || (nB && (!kB || !vB)) || !kO || !vO) return GH_EINVAL;
+    hipLaunchKernelGGL(merge_desc_kernel, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, stream,
+                       nA, kA, vA, nB, kB, vB, kO, vO);
     return status();
 }
 

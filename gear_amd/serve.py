@@ -267,7 +267,7 @@ class Session:
         cfg = self.cfg
         with torch.cuda.stream(self.stream):
             f.global_lb(self._glb)
-            f.sort(self.thr)
+            f.sort(self.thr, tag=self.P)
             self._host[0:1].copy_(self._glb, non_blocking=True)
             self._have_glb = True
             if cfg.reorder:

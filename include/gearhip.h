@@ -350,6 +350,14 @@ int gh_frontier_keys_f64(int64_t N, int S, const int64_t *order, const double *l
                          const double *thr, double *keys, hipStream_t stream);
 int gh_min_rows_f64(int64_t N, int64_t stride, const int64_t *idx, const double *x, double *out,
                     hipStream_t stream);
+/* Stable merge of two runs sorted descending by key (every NaN first, as
+ * torch's descending sort; -0 equals +0), A before B on equal keys: the
+ * frontier's re-sort when only the appended children are unsorted, equal
+ * to the stable sort of the whole (UnsortedInputDomainList.sort,
+ * branching_domains.py:1171-1241).  kO, vO (nA + nB) out. */
+int gh_merge_desc_f64(int64_t nA, const double *kA, const int64_t *vA, int64_t nB,
+                      const double *kB, const int64_t *vB, double *kO, int64_t *vO,
+                      hipStream_t stream);
 /* Slot ids of B children just appended by gh_filter_append_f64 at
  * physical row hw: out[t] = hw + t for the *total survivors, `dead` (the
  * frontier's dead record) for the rest, so the host need not read the

@@ -319,3 +319,31 @@ def test_qcqp_crown_f64_staged_bit_exact(cuda, n, m, B):
     assert _bits_equal(lb.cpu().numpy(), lb_o)
     assert _bits_equal(lA.cpu().numpy(), lA_o)
     assert _bits_equal(lbias.cpu().numpy(), lbias_o)
+
+
+def test_merge_desc_equals_stable_sort(cuda):
+    """gh_merge_desc_f64 of a sorted prefix and a sorted tail equals torch's
+    stable descending argsort of the concatenation (ties, NaN, +-inf, -0)."""
+    from gear_amd._lib import lib, ptr, stream_ptr
+    rng = np.random.default_rng(3)
+    for nA, nB in ((1000, 300), (0, 50), (70, 0), (1, 1), (5000, 4096)):
+        k = np.round(rng.standard_normal(nA + nB) * 3) / 2          # many ties
+        k[rng.random(nA + nB) < 0.03] = np.nan
+        k[rng.random(nA + nB) < 0.02] = np.inf
+        k[rng.random(nA + nB) < 0.02] = -np.inf
+        k[rng.random(nA + nB) < 0.05] = -0.0
+        keys = torch.as_tensor(k, device=cuda)
+        vals = torch.arange(nA + nB, dtype=torch.int64, device=cuda)
+        pa = torch.argsort(keys[:nA], descending=True, stable=True)
+        pb = torch.argsort(keys[nA:], descending=True, stable=True)
+        kA, vA = keys[:nA][pa].contiguous(), vals[:nA][pa].contiguous()
+        kB, vB = keys[nA:][pb].contiguous(), vals[nA:][pb].contiguous()
+        kO = torch.empty(nA + nB, dtype=torch.float64, device=cuda)
+        vO = torch.empty(nA + nB, dtype=torch.int64, device=cuda)
+        # the prefix as the frontier holds it: already in the order of a stable sort
+        ref = torch.argsort(torch.cat([kA, kB]), descending=True, stable=True)
+        refv = torch.cat([vA, vB])[ref]
+        assert lib().gh_merge_desc_f64(nA, ptr(kA) if nA else None, ptr(vA) if nA else None, nB,
+                                       ptr(kB) if nB else None, ptr(vB) if nB else None, ptr(kO),
+                                       ptr(vO), stream_ptr()) == 0
+        assert torch.equal(vO, refv), (nA, nB)
