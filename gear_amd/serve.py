@@ -262,9 +262,20 @@ class Session:
 
     def launch(self):
         """Enqueue the global bound of the current frontier, the frontier
-        sort and one step."""
+        sort and one step.  An append not yet accounted (the presplit of a
+        session started in this tick) is committed first: the global bound
+        and the pick must see every survivor, or a frontier that still holds
+        boxes would read as empty and the bound as the incumbent."""
         f = self.frontier
         cfg = self.cfg
+        if self._pending:
+            self.event.synchronize()
+            f.commit(int(self._host_total[0]))
+            self._pending = False
+            if self._nb:
+                self.steps += 1
+                self.boxes += self._bounded
+            self._nb = 0
         with torch.cuda.stream(self.stream):
             f.global_lb(self._glb)
             f.sort(self.thr, tag=self.P)

@@ -84,3 +84,29 @@ def test_plain_sequential_agrees(cuda):
         if a["status"] == b["status"] == "converged":
             scale = max(1.0, abs(a["P"]))
             assert abs(a["D"] - b["D"]) <= 3e-4 * scale
+
+
+def test_served_bound_is_never_above_a_feasible_point(cuda):
+    """A request whose presplit keeps few of its boxes (request 7 of the
+    seed-11 stream, n = 5): the served lower bound D must stay below every
+    feasible objective value (sampled), whether or not the gap closes.  The
+    first step of a session is launched in the tick that starts it, so this
+    also checks that the presplit's survivors are accounted before the
+    global bound is read (reading them late made the frontier look empty
+    and D equal to the incumbent)."""
+    from gear_amd.serve import ServeConfig, poisson_stream, serve
+    r = poisson_stream(24, 8.0, seed=11, n_median=8.0, n_sigma=0.6)[7]
+    p = r.problem
+    rng = np.random.default_rng(0)
+    X = rng.uniform(-1.0, 1.0, size=(200000, p.n))
+    F = p.values(X)
+    feas = (F[:, 1:] <= p.rhs[1:]).all(1)
+    assert feas.any()
+    fmin = float(F[feas, 0].min())
+    req = type(r)(r.rid, 0.0, p)
+    res, _ = serve([req], ServeConfig(time_limit=1.5, cap=1 << 18), device=cuda, realtime=False)
+    x = res[0]
+    print(f"status {x['status']} P {x['P']:.6f} D {x['D']:.6f} sampled feasible min {fmin:.6f}",
+          flush=True)
+    assert x["D"] <= fmin + 1e-9
+    assert x["steps"] > 0
