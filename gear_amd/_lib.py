@@ -53,6 +53,9 @@ SIGNATURES = {
     "gh_gather_rows_f64": [ctypes.c_int64, _P, _I, _P, _P, _I, _P, _P, _I, _P, _P, _P],
     "gh_fill_order_i64": [_I, _P, ctypes.c_int64, ctypes.c_int64, _P, _P],
     "gh_merge_desc_f64": [ctypes.c_int64, _P, _P, ctypes.c_int64, _P, _P, _P, _P, _P],
+    "gh_split_append_workspace_bytes": [_I],
+    "gh_split_append_f64": [_I, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _L, _L, _P, _L,
+                            _P, _P, _P, _P],
     "gh_dag_plan_create": [_I, _P, ctypes.POINTER(_P)],
     "gh_dag_plan_destroy": [_P],
     "gh_dag_plan_info": [_P, _P, _P, _P],
@@ -60,7 +63,8 @@ SIGNATURES = {
     "gh_dag_workspace_bytes": [_P, _I],
     "gh_dag_crown_f64": [_P, _P, _P, ctypes.c_size_t, _I, _P],
 }
-_RESTYPES = {"gh_filter_workspace_bytes": ctypes.c_size_t, "gh_qcqp_pack_bytes": ctypes.c_size_t,
+_RESTYPES = {"gh_filter_workspace_bytes": ctypes.c_size_t,
+             "gh_split_append_workspace_bytes": ctypes.c_size_t, "gh_qcqp_pack_bytes": ctypes.c_size_t,
              "gh_qcqp_pack_f32_bytes": ctypes.c_size_t, "gh_dag_workspace_bytes": ctypes.c_size_t,
              "gh_qcqp_large_plan_bytes": ctypes.c_size_t,
              "gh_alm_plan_bytes": ctypes.c_size_t,

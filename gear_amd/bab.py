@@ -20,8 +20,8 @@ from dataclasses import dataclass, field
 import torch
 
 from .alm import ALMConfig, ALMPGDSolver, replay_float64
-from .crown import (BoundedQCQP, DeviceFrontier, branch_split_keys, child_keys,
-                    input_split_branching, input_split_parallel)
+from .crown import (BoundedQCQP, DeviceFrontier, child_keys, input_split_branching,
+                    input_split_parallel)
 
 
 def is_primal_dual_converged(P, D, abs_tol=1e-5, rel_tol=0.0):
@@ -88,14 +88,15 @@ def reordered_step(f, bound, thr, nb, n, S, planes=True, lp=None, clip=False,
     lb = torch.maximum(lb, pkey)
     if lp is not None and lA is not None:
         lp(lb, lA, lbias, pL, pU)
-    if clip:
-        from .concretize import clip_domains
-        idx = input_split_branching(None, None, pL, pU, None, None, "naive")
-        cL, cU, _ = input_split_parallel(pL, pU, i_idx=idx)
-        clip_domains(lA, lbias, thr, cL, cU, out=(cL, cU))
-        keys = child_keys(lb, lA, lbias, cL, cU)
-    else:
-        cL, cU, keys = branch_split_keys(lb, pL, pU, lA, lbias)
+    if not clip:
+        # branch, bisect, key, prune and append without materializing the
+        # children (the same frontier as branch_split_keys + filter_append)
+        return f.split_append(lb, pL, pU, thr, lA, lbias, sync=sync)
+    from .concretize import clip_domains
+    idx = input_split_branching(None, None, pL, pU, None, None, "naive")
+    cL, cU, _ = input_split_parallel(pL, pU, i_idx=idx)
+    clip_domains(lA, lbias, thr, cL, cU, out=(cL, cU))
+    keys = child_keys(lb, lA, lbias, cL, cU)
     return f.filter_append(keys, thr, cL, cU, sync=sync, payload=cuts)
 
 

@@ -422,11 +422,52 @@ class DeviceFrontier:
         """The payload rows of the domains the last pick_out(nb) returned."""
         return self._pay.index_select(0, self._picked[:nb])
 
-    def workspace(self, B):
-        need = int(lib().gh_filter_workspace_bytes(B))
+    def workspace(self, B, nbytes=None):
+        need = int(lib().gh_filter_workspace_bytes(B)) if nbytes is None else int(nbytes)
         if self._ws is None or self._ws.numel() < need:
             self._ws = torch.empty(need, dtype=torch.uint8, device=self.device)
         return self._ws
+
+    def split_append(self, plb, pL, pU, thr, lA=None, lbias=None, sync=True):
+        """Widest branching, bisection, child keys from the parents'
+        objective planes, prune and append for K parents in one call
+        (gh_split_append_f64): the frontier branch_split_keys followed by
+        filter_append of the 2K children would give, without materializing
+        the children.  lA (K, R, n) / lbias (K, R) may be None (the keys are
+        the parents' bounds).  Returns the survivor count when sync is set,
+        else None (as filter_append)."""
+        if self._pay is not None:
+            raise ValueError("a frontier with a payload appends through filter_append")
+        K = pL.shape[0]
+        B = 2 * K
+        R = 1
+        if lA is not None:
+            R = lA.shape[1]
+            if lA.shape != (K, R, self.n) or lbias.shape != (K, R) or not lA.is_contiguous() \
+                    or not lbias.is_contiguous():
+                raise ValueError("planes must be contiguous (K, R, n) and (K, R)")
+        if pL.shape != (K, self.n) or pU.shape != (K, self.n) or plb.shape != (K, self.S):
+            raise ValueError("parents must be (K, n) boxes with (K, S) bounds")
+        if self.count + B > self.cap:
+            raise RuntimeError("frontier capacity exceeded")
+        self._room(B)
+        self._last_B = B
+        ws = self.workspace(B, lib().gh_split_append_workspace_bytes(K))
+        check(lib().gh_split_append_f64(K, self.n, self.S, R, ptr(pL.contiguous()),
+                                        ptr(pU.contiguous()), ptr(plb.contiguous()), ptr(lA),
+                                        ptr(lbias), ptr(thr.contiguous()), ptr(self._L),
+                                        ptr(self._U), ptr(self._lb), self.hw, self.phys,
+                                        ptr(self.order[self.count:self.count + B]), self.dead,
+                                        ptr(self.total), ptr(self.disc_min), ptr(ws),
+                                        stream_ptr()), "gh_split_append_f64")
+        self._pending = B
+        if sync:
+            return self.finish_append()
+        self.count += B
+        self.hw += B
+        self.unsynced += 1
+        self._pending = None
+        return None
 
     def filter_append(self, lb, thr, x_L, x_U, sync=True, payload=None):
         """Prune boxes verified against thresholds thr (S,) and append the

@@ -48,23 +48,52 @@ __global__ void split_kernel(int B, int n, const double* __restrict__ xL,
 }
 
 // Verification predicate of get_remaining_index (solving mode) plus the
-// invalid-box check.  Returns keep; *verified_thr gets thr[0] when the box
-// is verified with a finite objective threshold, +inf otherwise.
+// invalid-box check on a bound row whose entry 0 is L0 and entries s >= 1
+// are L[s].  Returns keep; *verified_thr gets thr[0] when the box is
+// verified with a finite objective threshold, +inf otherwise.
 // box_flags: bit 0 some coordinate non-finite, bit 1 some l > u.
+__device__ __forceinline__ bool keep_row(double L0, const double* __restrict__ L, int S,
+                                         const double* __restrict__ thr, int box_flags,
+                                         double* verified_thr)
+{
+    bool ver = false;
+    for (int s = 0; s < S; ++s) {
+        double Ls = s == 0 ? L0 : L[s], Ts = thr[s];
+        bool gt = isfinite(Ls) && isfinite(Ts) && (Ls > Ts);
+        if (s == 0) gt = gt || (isfinite(Ls) && isfinite(Ts) && (Ls >= Ts));
+        ver = ver || gt;
+    }
+    ver = ver || (isinf(L0) && L0 > 0 && isfinite(thr[0]));
+    *verified_thr = (ver && isfinite(thr[0])) ? thr[0] : INFINITY;
+    const bool nonfinite = box_flags & 1, invalid = box_flags & 2;
+    return !ver && !(invalid && !nonfinite);
+}
+
 __device__ __forceinline__ bool keep_box(int b, int S, const double* __restrict__ lb,
                                          const double* __restrict__ thr, int box_flags,
                                          double* verified_thr)
 {
     const double* L = lb + (size_t)b * S;
-    bool ver = false;
-    for (int s = 0; s < S; ++s) {
-        double Ls = L[s], Ts = thr[s];
+    return keep_row(L[0], L, S, thr, box_flags, verified_thr);
+}
+
+// keep_row evaluated by a whole wave, lanes over the rows s = lane,
+// lane + 64, ... (the predicate is an OR over rows, so the split changes
+// nothing).  Every lane returns the result.
+__device__ __forceinline__ bool keep_row_wave(int lane, double L0, const double* __restrict__ L,
+                                              int S, const double* __restrict__ thr,
+                                              int box_flags, double* verified_thr)
+{
+    bool any = false;
+    for (int s = lane; s < S; s += 64) {
+        const double Ls = s == 0 ? L0 : L[s], Ts = thr[s];
         bool gt = isfinite(Ls) && isfinite(Ts) && (Ls > Ts);
         if (s == 0) gt = gt || (isfinite(Ls) && isfinite(Ts) && (Ls >= Ts));
-        ver = ver || gt;
+        any = any || gt;
     }
-    ver = ver || (isinf(L[0]) && L[0] > 0 && isfinite(thr[0]));
-    *verified_thr = (ver && isfinite(thr[0])) ? thr[0] : INFINITY;
+    const double T0 = thr[0];
+    const bool ver = __ballot(any) != 0 || (isinf(L0) && L0 > 0 && isfinite(T0));
+    *verified_thr = (ver && isfinite(T0)) ? T0 : INFINITY;
     const bool nonfinite = box_flags & 1, invalid = box_flags & 2;
     return !ver && !(invalid && !nonfinite);
 }
@@ -139,31 +168,47 @@ __global__ __launch_bounds__(256) void filter_count_kernel(
     }
 }
 
-// Pass 2: exclusive scan of the block counts (one block), total survivors.
-__global__ __launch_bounds__(1024) void filter_scan_kernel(int nblk, int32_t* __restrict__ counts,
+// Pass 2: exclusive scan of the N block counts in place (one workgroup of
+// 16 waves, 4096 counts per round: 4 per thread, wave shuffles, then the
+// wave totals), total survivors.  Counts and their sums fit in int32 (at
+// most 2^31 - 1 boxes).
+__global__ __launch_bounds__(1024) void filter_scan_kernel(int N, int32_t* __restrict__ counts,
                                                            int64_t* __restrict__ total)
 {
-    __shared__ int64_t part[1024];
-    int t = threadIdx.x;
-    int per = (nblk + 1023) / 1024;
-    int lo = t * per, hi = min(nblk, lo + per);
-    int64_t sum = 0;
-    for (int i = lo; i < hi; ++i) sum += counts[i];
-    part[t] = sum;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        int64_t v = (t >= off) ? part[t - off] : 0;
+    __shared__ int wsum[16];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    int carry = 0;
+    for (int base = 0; base < N; base += 4096) {
+        const int i0 = base + 4 * t;
+        int v[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) v[k] = (i0 + k < N) ? counts[i0 + k] : 0;
+        const int s4 = v[0] + v[1] + v[2] + v[3];
+        int x = s4;    // inclusive scan over the wave
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int y = __shfl_up(x, o);
+            if (lane >= o) x += y;
+        }
+        if (lane == 63) wsum[w] = x;
         __syncthreads();
-        part[t] += v;
-        __syncthreads();
+        int wpre = 0, tot = 0;
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            const int ws = wsum[j];
+            wpre += j < w ? ws : 0;
+            tot += ws;
+        }
+        int run = carry + wpre + x - s4;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (i0 + k < N) counts[i0 + k] = run;
+            run += v[k];
+        }
+        carry += tot;
+        __syncthreads();   // wsum is rewritten next round
     }
-    int64_t run = part[t] - sum;  // exclusive prefix of this thread's chunk
-    for (int i = lo; i < hi; ++i) {
-        int32_t c = counts[i];
-        counts[i] = (int32_t)run;
-        run += c;
-    }
-    if (t == 1023) *total = part[1023];
+    if (t == 0) *total = carry;
 }
 
 // Pass 3: rank of each survivor inside its block -> destination row
@@ -297,29 +342,38 @@ __global__ void fill_order_kernel(int B, const int64_t* __restrict__ total, int6
 }
 
 // Rows idx[i] of up to three row-major arrays (widths w0, w1, w2) into
-// contiguous outputs, one thread per element.
-__global__ void gather_rows_kernel(int64_t N, const int64_t* __restrict__ idx, int w0,
-                                   const double* __restrict__ s0, double* __restrict__ d0, int w1,
-                                   const double* __restrict__ s1, double* __restrict__ d1, int w2,
-                                   const double* __restrict__ s2, double* __restrict__ d2)
+// contiguous outputs: one wave per row, lanes over the row's w0 + w1 + w2
+// entries, up to four per lane loaded before any is stored (no per-element
+// index division).
+__global__ void __launch_bounds__(256)
+gather_rows_kernel(int64_t N, const int64_t* __restrict__ idx, int w0,
+                   const double* __restrict__ s0, double* __restrict__ d0, int w1,
+                   const double* __restrict__ s1, double* __restrict__ d1, int w2,
+                   const double* __restrict__ s2, double* __restrict__ d2)
 {
-    const int64_t W = (int64_t)w0 + w1 + w2;
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= N * W) return;
-    const int64_t i = t / W;
-    int c = (int)(t - i * W);
+    const int64_t i = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (i >= N) return;
     const int64_t r = idx[i];
-    if (c < w0) {
-        d0[i * w0 + c] = s0[r * w0 + c];
-        return;
+    const int W = w0 + w1 + w2;
+    for (int c0 = 0; c0 < W; c0 += 256) {
+        double v[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int c = c0 + 64 * k + lane;
+            v[k] = 0.0;
+            if (c < w0) v[k] = s0[r * w0 + c];
+            else if (c < w0 + w1) v[k] = s1[r * w1 + (c - w0)];
+            else if (c < W) v[k] = s2[r * w2 + (c - w0 - w1)];
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int c = c0 + 64 * k + lane;
+            if (c < w0) d0[i * w0 + c] = v[k];
+            else if (c < w0 + w1) d1[i * w1 + (c - w0)] = v[k];
+            else if (c < W) d2[i * w2 + (c - w0 - w1)] = v[k];
+        }
     }
-    c -= w0;
-    if (c < w1) {
-        d1[i * w1 + c] = s1[r * w1 + c];
-        return;
-    }
-    c -= w1;
-    d2[i * w2 + c] = s2[r * w2 + c];
 }
 
 // min over i < N of x[idx[i] * stride] into *out (atomic min).
@@ -463,9 +517,8 @@ extern "C" int gh_gather_rows_f64(int64_t N, const int64_t* idx, int w0, const d
     if (N == 0 || w0 + w1 + w2 == 0) return GH_OK;
     if (!idx || (w0 && (!s0 || !d0)) || (w1 && (!s1 || !d1)) || (w2 && (!s2 || !d2)))
         return GH_EINVAL;
-    const int64_t items = N * ((int64_t)w0 + w1 + w2);
-    hipLaunchKernelGGL(gather_rows_kernel, dim3((unsigned)((items + 255) / 256)), dim3(256), 0,
-                       stream, N, idx, w0, s0, d0, w1, s1, d1, w2, s2, d2);
+    hipLaunchKernelGGL(gather_rows_kernel, dim3((unsigned)((N + 3) / 4)), dim3(256), 0, stream, N,
+                       idx, w0, s0, d0, w1, s1, d1, w2, s2, d2);
     return status();
 }
 
@@ -508,6 +561,20 @@ __device__ __forceinline__ double red64x(double v)
 
 constexpr double kU53 = 1.1102230246251565404236316680908203125e-16;
 
+// Wave-wide tail of a child key: the red64s sums of the per-lane partials
+// pm (plane value at the centre minus its radius term) and mg (magnitude),
+// the margin and the step toward -inf.
+__device__ __forceinline__ double child_key_tail(int n, double pm, double mg, bool bad, double b,
+                                                 double p0)
+{
+    pm = red64x(pm) + b;
+    mg = red64x(mg);
+    bad = __ballot(bad) != 0;
+    const double margin = (n + 4) * kU53 * (mg + fabs(b));
+    const double key = nextafter(pm - margin, -INFINITY);
+    return (!bad && isfinite(key)) ? fmax(p0, key) : p0;
+}
+
 // Key of one child box [xl, xu] from its parent's objective plane (a, b)
 // and bound row plb: max(plb[0], the plane's minimum over the box lowered
 // by (n + 4) u (sum |a| (|c| + r) + |b|) and one step toward -inf);
@@ -528,12 +595,76 @@ __device__ __forceinline__ double child_key(int n, int lane, const double* __res
         mg = mg + fabs(ai) * (fabs(c) + r);
         bad = bad || !(l <= u);
     }
-    pm = red64x(pm) + b;
-    mg = red64x(mg);
-    bad = __ballot(bad) != 0;
-    const double margin = (n + 4) * kU53 * (mg + fabs(b));
-    const double key = nextafter(pm - margin, -INFINITY);
-    return (!bad && isfinite(key)) ? fmax(p0, key) : p0;
+    return child_key_tail(n, pm, mg, bad, b, p0);
+}
+
+// First of the widest coordinates of [l, u] as branch_widest_kernel's
+// sequential scan finds it (a NaN width at 0 keeps 0, later NaN widths
+// never win).  Wave-wide; every lane returns the index.
+__device__ __forceinline__ int first_widest(int n, int lane, const double* __restrict__ l,
+                                            const double* __restrict__ u)
+{
+    // per-lane candidates in index order, then the largest width with the
+    // lowest index
+    double bw = -INFINITY;
+    int bi = -1;
+    const double w0 = u[0] - l[0];
+    for (int i = lane; i < n; i += 64) {
+        const double w = u[i] - l[i];
+        if (bi < 0 ? !isnan(w) : w > bw) {
+            bw = w;
+            bi = i;
+        }
+    }
+    for (int o = 32; o >= 1; o >>= 1) {
+        const double w2 = __shfl_xor(bw, o);
+        const int i2 = __shfl_xor(bi, o);
+        if (i2 >= 0 && (bi < 0 || w2 > bw || (w2 == bw && i2 < bi))) {
+            bw = w2;
+            bi = i2;
+        }
+    }
+    return (isnan(w0) || bi < 0) ? 0 : bi;
+}
+
+// Coordinate i of child c of a box split at idx: c = 0 the upper half,
+// c = 1 the lower half (split.py:74-107 with P = 2, split_kernel's).
+__device__ __forceinline__ void split_coord(double& lo, double& hi, int i, int idx, int c)
+{
+    if (i == idx) {
+        const double a = (c + 1) * lo + (2 - c - 1) * hi;
+        const double b = c * lo + (2 - c) * hi;
+        lo = a / 2.0;
+        hi = b / 2.0;
+    }
+}
+
+// child_key of child c of the parent box [l, u] split at idx, the child's
+// coordinates formed in registers (the same values split_coord writes);
+// a == NULL gives p0.  *flags gets the child box's bits (bit 0 some
+// coordinate non-finite, bit 1 some l > u).  Wave-wide.
+__device__ __forceinline__ double split_child_key(int n, int lane, const double* __restrict__ a,
+                                                  double b, const double* __restrict__ l,
+                                                  const double* __restrict__ u, int idx, int c,
+                                                  double p0, int* flags)
+{
+    double pm = 0.0, mg = 0.0;
+    bool bad = false, nonfin = false, inval = false;
+    for (int i = lane; i < n; i += 64) {
+        double lo = l[i], hi = u[i];
+        split_coord(lo, hi, i, idx, c);
+        nonfin = nonfin || !isfinite(lo) || !isfinite(hi);
+        inval = inval || lo > hi;
+        if (a) {
+            const double cc = (lo + hi) / 2, r = (hi - lo) / 2;
+            const double ai = a[i];
+            pm = pm + (ai * cc - fabs(ai) * r);
+            mg = mg + fabs(ai) * (fabs(cc) + r);
+            bad = bad || !(lo <= hi);
+        }
+    }
+    *flags = (__ballot(nonfin) ? 1 : 0) | (__ballot(inval) ? 2 : 0);
+    return a ? child_key_tail(n, pm, mg, bad, b, p0) : p0;
 }
 
 // One wave per child k of parent k % K (children [upper halves; lower
@@ -571,38 +702,13 @@ branch_split_keys_kernel(int K, int n, int S, int R, const double *__restrict__ 
     const int lane = threadIdx.x & 63;
     if (p >= K) return;
     const double *l = pL + (size_t)p * n, *u = pU + (size_t)p * n;
-    // first widest coordinate: per-lane candidates in index order, then the
-    // largest width with the lowest index
-    double bw = -INFINITY;
-    int bi = -1;
-    const double w0 = u[0] - l[0];
-    for (int i = lane; i < n; i += 64) {
-        const double w = u[i] - l[i];
-        if (bi < 0 ? !isnan(w) : w > bw) {
-            bw = w;
-            bi = i;
-        }
-    }
-    for (int o = 32; o >= 1; o >>= 1) {
-        const double w2 = __shfl_xor(bw, o);
-        const int i2 = __shfl_xor(bi, o);
-        if (i2 >= 0 && (bi < 0 || w2 > bw || (w2 == bw && i2 < bi))) {
-            bw = w2;
-            bi = i2;
-        }
-    }
-    const int idx = (isnan(w0) || bi < 0) ? 0 : bi;
+    const int idx = first_widest(n, lane, l, u);
     // children: c = 0 the upper half (row p), c = 1 the lower half (row p + K)
     for (int c = 0; c < 2; ++c) {
         double *ol = cL + (size_t)(p + c * K) * n, *ou = cU + (size_t)(p + c * K) * n;
         for (int i = lane; i < n; i += 64) {
             double lo = l[i], hi = u[i];
-            if (i == idx) {
-                const double a = (c + 1) * lo + (2 - c - 1) * hi;
-                const double b = c * lo + (2 - c) * hi;
-                lo = a / 2.0;
-                hi = b / 2.0;
-            }
+            split_coord(lo, hi, i, idx, c);
             ol[i] = lo;
             ou[i] = hi;
         }
@@ -616,6 +722,203 @@ branch_split_keys_kernel(int K, int n, int S, int R, const double *__restrict__ 
                                           cL + k * n, cU + k * n, p0)
                               : p0;
         for (int s = lane; s < S; s += 64) keys[k * S + s] = s == 0 ? key : plb[(size_t)p * S + s];
+    }
+}
+
+// Register forms of first_widest, split_child_key and keep_row_wave when
+// n <= 64 and S <= 64: lane i holds coordinate i (li, ui, ai) and bound
+// row entry i (pr) with its threshold (th).  The same operations on the
+// same values as the pointer forms, whose loops then run once per lane.
+__device__ __forceinline__ int first_widest_lane(int n, int lane, double li, double ui)
+{
+    const double w = ui - li;
+    double bw = -INFINITY;
+    int bi = -1;
+    if (lane < n && !isnan(w)) {
+        bw = w;
+        bi = lane;
+    }
+    for (int o = 32; o >= 1; o >>= 1) {
+        const double w2 = __shfl_xor(bw, o);
+        const int i2 = __shfl_xor(bi, o);
+        if (i2 >= 0 && (bi < 0 || w2 > bw || (w2 == bw && i2 < bi))) {
+            bw = w2;
+            bi = i2;
+        }
+    }
+    const double w0 = __shfl(w, 0);
+    return (isnan(w0) || bi < 0) ? 0 : bi;
+}
+
+__device__ __forceinline__ double split_child_key_lane(int n, int lane, bool planes, double ai,
+                                                       double b, double li, double ui, int idx,
+                                                       int c, double p0, int* flags)
+{
+    const bool in = lane < n;
+    double lo = li, hi = ui;
+    split_coord(lo, hi, lane, idx, c);
+    const bool nonfin = in && (!isfinite(lo) || !isfinite(hi));
+    const bool inval = in && lo > hi;
+    *flags = (__ballot(nonfin) ? 1 : 0) | (__ballot(inval) ? 2 : 0);
+    if (!planes) return p0;
+    double pm = 0.0, mg = 0.0;
+    bool bad = false;
+    if (in) {
+        const double cc = (lo + hi) / 2, r = (hi - lo) / 2;
+        pm = pm + (ai * cc - fabs(ai) * r);
+        mg = mg + fabs(ai) * (fabs(cc) + r);
+        bad = !(lo <= hi);
+    }
+    return child_key_tail(n, pm, mg, bad, b, p0);
+}
+
+__device__ __forceinline__ bool keep_lane(int lane, int S, double L0, double pr, double th,
+                                          double T0, int box_flags, double* verified_thr)
+{
+    bool gt = false;
+    if (lane < S) {
+        const double Ls = lane == 0 ? L0 : pr;
+        gt = isfinite(Ls) && isfinite(th) && (Ls > th);
+        if (lane == 0) gt = gt || (isfinite(Ls) && isfinite(th) && (Ls >= th));
+    }
+    const bool ver = __ballot(gt) != 0 || (isinf(L0) && L0 > 0 && isfinite(T0));
+    *verified_thr = (ver && isfinite(T0)) ? T0 : INFINITY;
+    const bool nonfinite = box_flags & 1, invalid = box_flags & 2;
+    return !ver && !(invalid && !nonfinite);
+}
+
+// The reordered step's decision, split, child keys, prune and append
+// without materializing the children.  Pass 1, one wave per parent p
+// (workgroup = 4 parents): widest index, both children's keys, box flags
+// and keep predicates (branch_split_keys_kernel's and filter_count_kernel's
+// values); per workgroup and side the kept count, and the dropped
+// thresholds' minimum into disc_min.
+template <bool SMALL>
+__global__ void __launch_bounds__(256)
+split_keep_kernel(int K, int n, int S, int R, const double *__restrict__ pL,
+                  const double *__restrict__ pU, const double *__restrict__ plb,
+                  const double *__restrict__ lA, const double *__restrict__ lbias,
+                  const double *__restrict__ thr, double *__restrict__ ckey,
+                  int32_t *__restrict__ sidx, int32_t *__restrict__ keep,
+                  int32_t *__restrict__ cnt, double *__restrict__ disc_min)
+{
+    __shared__ int kc[2][4];
+    __shared__ double red[4];
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int p = blockIdx.x * 4 + wv;
+    int kept[2] = {0, 0};
+    double vt = INFINITY;
+    if (p < K) {
+        const double *l = pL + (size_t)p * n, *u = pU + (size_t)p * n;
+        const double *prow = plb + (size_t)p * S;
+        const double *a = lA ? lA + (size_t)p * R * n : nullptr;
+        // SMALL: every operand in one round of independent loads
+        double li = 0.0, ui = 0.0, ai = 0.0, pr = 0.0, th = 0.0;
+        if constexpr (SMALL) {
+            if (lane < n) {
+                li = l[lane];
+                ui = u[lane];
+                if (a) ai = a[lane];
+            }
+            if (lane < S) {
+                pr = prow[lane];
+                th = thr[lane];
+            }
+        }
+        const double b = lA ? lbias[(size_t)p * R] : 0.0;
+        const double p0 = SMALL ? __shfl(pr, 0) : prow[0];
+        const double T0 = SMALL ? __shfl(th, 0) : thr[0];
+        const int idx = SMALL ? first_widest_lane(n, lane, li, ui) : first_widest(n, lane, l, u);
+        for (int c = 0; c < 2; ++c) {
+            int fl;
+            double v;
+            double key;
+            bool k;
+            if constexpr (SMALL) {
+                key = split_child_key_lane(n, lane, a != nullptr, ai, b, li, ui, idx, c, p0, &fl);
+                k = keep_lane(lane, S, key, pr, th, T0, fl, &v);
+            } else {
+                key = split_child_key(n, lane, a, b, l, u, idx, c, p0, &fl);
+                k = keep_row_wave(lane, key, prow, S, thr, fl, &v);
+            }
+            const size_t ch = (size_t)p + (size_t)c * K;
+            if (lane == 0) {
+                ckey[ch] = key;
+                keep[ch] = k ? 1 : 0;
+            }
+            kept[c] = k ? 1 : 0;
+            vt = fmin(vt, v);
+        }
+        if (lane == 0) sidx[p] = idx;
+    }
+    if (lane == 0) {
+        kc[0][wv] = kept[0];
+        kc[1][wv] = kept[1];
+    }
+    const double m = block_min(vt, red);   // synchronizes the workgroup first
+    if (threadIdx.x == 0) {
+        cnt[blockIdx.x] = kc[0][0] + kc[0][1] + kc[0][2] + kc[0][3];
+        cnt[gridDim.x + blockIdx.x] = kc[1][0] + kc[1][1] + kc[1][2] + kc[1][3];
+        if (m < INFINITY) atomic_min_f64(disc_min, m);
+    }
+}
+
+// Pass 3 (after filter_scan_kernel turned the counts, side 0 blocks then
+// side 1 blocks, into exclusive offsets): one wave per parent p writes each
+// kept child's box, rebuilt from the parent, and bound row [key, plb[p][1:]]
+// to frontier row base + rank, and both children's slot ids (base + t for
+// the first *total positions t, the dead record after them).  Every load
+// is issued before the first store.
+__global__ void __launch_bounds__(256)
+split_copy_kernel(int K, int n, int S, const double *__restrict__ pL,
+                  const double *__restrict__ pU, const double *__restrict__ plb,
+                  const double *__restrict__ ckey, const int32_t *__restrict__ sidx,
+                  const int32_t *__restrict__ keep, const int32_t *__restrict__ offs,
+                  const int64_t *__restrict__ total, double *__restrict__ FL,
+                  double *__restrict__ FU, double *__restrict__ Flb, int64_t base,
+                  int64_t *__restrict__ order_out, int64_t dead)
+{
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int blk = blockIdx.x, nblk = gridDim.x;
+    const int p = blk * 4 + wv;
+    if (p >= K) return;
+    // keep flags of the block's parents: lanes 0..3 side 0, lanes 4..7 side 1
+    const int q = blk * 4 + (lane & 3);
+    const bool kq = lane < 8 && q < K && keep[(size_t)q + (size_t)(lane >> 2) * K];
+    const int32_t off0 = offs[blk], off1 = offs[nblk + blk];
+    const int idx = sidx[p];
+    const int64_t T = *total;
+    const double key0 = ckey[p], key1 = ckey[(size_t)p + K];
+    const double *l = pL + (size_t)p * n, *u = pU + (size_t)p * n;
+    const double *prow = plb + (size_t)p * S;
+    const double li = lane < n ? l[lane] : 0.0, ui = lane < n ? u[lane] : 0.0;
+    const double pb = lane < S ? prow[lane] : 0.0;
+    const unsigned long long km = __ballot(kq);
+    for (int c = 0; c < 2; ++c) {
+        const unsigned long long side = (km >> (4 * c)) & 0xFull;
+        if (!((side >> wv) & 1)) continue;
+        const int64_t row =
+            base + (c ? off1 : off0) + __popcll(side & ((1ull << wv) - 1ull));
+        double *fl = FL + (size_t)row * n, *fu = FU + (size_t)row * n;
+        if (lane < n) {
+            double lo = li, hi = ui;
+            split_coord(lo, hi, lane, idx, c);
+            fl[lane] = lo;
+            fu[lane] = hi;
+        }
+        for (int i = lane + 64; i < n; i += 64) {
+            double lo = l[i], hi = u[i];
+            split_coord(lo, hi, i, idx, c);
+            fl[i] = lo;
+            fu[i] = hi;
+        }
+        double *fb = Flb + (size_t)row * S;
+        if (lane < S) fb[lane] = lane == 0 ? (c ? key1 : key0) : pb;
+        for (int s = lane + 64; s < S; s += 64) fb[s] = prow[s];
+    }
+    if (lane < 2) {
+        const int64_t ch = (int64_t)p + (int64_t)lane * K;
+        order_out[ch] = ch < T ? base + ch : dead;
     }
 }
 
@@ -643,5 +946,43 @@ extern "C" int gh_child_keys_f64(int K, int n, int S, int R, const double *plb,
         return GH_EINVAL;
     hipLaunchKernelGGL(child_keys_kernel, dim3((2 * K + 3) / 4), dim3(256), 0, stream, K, n, S,
                        R, plb, lA, lbias, cL, cU, keys);
+    return hipGetLastError() == hipSuccess ? GH_OK : GH_ELAUNCH;
+}
+
+extern "C" size_t gh_split_append_workspace_bytes(int K)
+{
+    if (K < 0) return 0;
+    const size_t nblk = ((size_t)K + 3) / 4;
+    return sizeof(double) * 2 * (size_t)K + sizeof(int32_t) * (3 * (size_t)K + 2 * nblk) + 256;
+}
+
+extern "C" int gh_split_append_f64(int K, int n, int S, int R, const double *pL, const double *pU,
+                                   const double *plb, const double *lA, const double *lbias,
+                                   const double *thr, double *FL, double *FU, double *Flb,
+                                   int64_t base, int64_t cap, int64_t *order_out, int64_t dead,
+                                   int64_t *total, double *disc_min, void *ws, hipStream_t stream)
+{
+    if (K < 0 || n < 1 || S < 1 || base < 0 || !total || !disc_min) return GH_EINVAL;
+    if (lA && (R < 1 || R > S || !lbias)) return GH_EINVAL;
+    if (K == 0) {
+        hipLaunchKernelGGL(set_i64_kernel, dim3(1), dim3(1), 0, stream, total, (int64_t)0);
+        return hipGetLastError() == hipSuccess ? GH_OK : GH_ELAUNCH;
+    }
+    if (!pL || !pU || !plb || !thr || !FL || !FU || !Flb || !order_out || !ws) return GH_EINVAL;
+    if (base + 2 * (int64_t)K > cap) return GH_EINVAL;
+    const int nblk = (K + 3) / 4;
+    double *ckey = (double *)ws;
+    int32_t *sidx = (int32_t *)(ckey + 2 * (size_t)K);
+    int32_t *keep = sidx + K;
+    int32_t *cnt = keep + 2 * (size_t)K;
+    if (n <= 64 && S <= 64)
+        hipLaunchKernelGGL(split_keep_kernel<true>, dim3(nblk), dim3(256), 0, stream, K, n, S, R,
+                           pL, pU, plb, lA, lbias, thr, ckey, sidx, keep, cnt, disc_min);
+    else
+        hipLaunchKernelGGL(split_keep_kernel<false>, dim3(nblk), dim3(256), 0, stream, K, n, S,
+                           R, pL, pU, plb, lA, lbias, thr, ckey, sidx, keep, cnt, disc_min);
+    hipLaunchKernelGGL(filter_scan_kernel, dim3(1), dim3(1024), 0, stream, 2 * nblk, cnt, total);
+    hipLaunchKernelGGL(split_copy_kernel, dim3(nblk), dim3(256), 0, stream, K, n, S, pL, pU,
+                       plb, ckey, sidx, keep, cnt, total, FL, FU, Flb, base, order_out, dead);
     return hipGetLastError() == hipSuccess ? GH_OK : GH_ELAUNCH;
 }

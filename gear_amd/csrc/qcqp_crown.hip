@@ -70,27 +70,240 @@ __global__ void qcqp_pack_kernel(int n, int S, int nch, int kt, const double* __
     QA[u] = (j2 < n && i2 < n) ? Qs[(size_t)j2 * n + i2] : 0.0;
 }
 
+// The lane's box values: endpoints xl, xu at indices q + 4m; the centre,
+// radius and magnitude of an index are recomputed where used (registers
+// are the kernel's scarce resource).
+template <int KT>
+struct BoxVals {
+    double xl[KT], xu[KT];
+    __device__ __forceinline__ double x0(int m) const { return (xu[m] + xl[m]) / 2.0; }
+    __device__ __forceinline__ double ep(int m) const { return (xu[m] - xl[m]) / 2.0; }
+    __device__ __forceinline__ double mx(int m) const
+    {
+        return gh::finite_abs(fmax(-xl[m], xu[m]));
+    }
+};
+
+// Per-box state of one row carried from its forward half (IBP and the plane
+// choice) to its backward half (backward product, merges, concretization).
+template <int KT>
+struct RowState {
+    double acm[KT], beta[KT];
+    double ql, qu, gsum, own;
+};
+
+// Forward half of row s.  qb points at the row's packed IBP tiles (+ lane);
+// G IBP tiles are accumulated at a time.
+template <int KT, int G>
+__device__ __forceinline__ void row_forward(int n, const double* qb, const double* cs,
+                                            const BoxVals<KT>& bx, int lq,
+                                            const QcqpConsts& k, RowState<KT>& st)
+{
+    constexpr int NCH = (KT + 3) / 4;    // 16-wide tiles of j (and of i)
+    constexpr int NM = KT;               // per-lane indices q + 4m
+    double p_ql[4], p_qu[4], p_g[4], p_own[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) p_ql[v] = p_qu[v] = p_g[v] = p_own[v] = 0.0;
+
+#pragma unroll
+    for (int c0 = 0; c0 < NCH; c0 += G) {
+        // ---- 1. IBP of y_s for tiles c0..c0+G-1: D[j][box] = sum_i
+        // Q[j][i] x[box][i] on MFMA (A = Q tile, B = the lane's x at
+        // index q + 4t); centre, radius and magnitude as in
+        // concretize_func.py:548-551, linear.py:607-653,
+        // backward_bound.py:739-741.
+        d4 Dc[G], Dd[G], Dm[G];
+#pragma unroll
+        for (int g = 0; g < G; ++g) Dc[g] = Dd[g] = Dm[g] = (d4){0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int t = 0; t < KT; ++t) {
+            const double x0 = bx.x0(t), ep = bx.ep(t), mx = bx.mx(t);
+#pragma unroll
+            for (int g = 0; g < G; ++g) {
+                if (c0 + g < NCH) {
+                    double q = qb[((size_t)(c0 + g) * KT + t) * 64];
+                    double qa_ = fabs(q);
+                    Dc[g] = mfma16(q, x0, Dc[g]);
+                    Dd[g] = mfma16(qa_, ep, Dd[g]);
+                    Dm[g] = mfma16(qa_, mx, Dm[g]);
+                }
+            }
+            if (t % 4 == 3) __builtin_amdgcn_sched_barrier(0);
+        }
+        // ---- 2. Elementwise: item v of tile c is index j = q + 4m,
+        // m = 4c + v, of box r.
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int c = c0 + g, m = 4 * c + v;
+                if (c < NCH && m < NM) {
+                    const int j = lq + 4 * m;
+                    st.acm[m] = 0.0;
+                    st.beta[m] = 0.0;
+                    if (j < n) {
+                        double cc = Dc[g][v], dv = Dd[g][v], mg = Dm[g][v];
+                        double e = mg * k.kn;
+                        double yl = gh::widen((cc - dv) + 0.0, e, -1.0);
+                        double yu = gh::widen((cc + dv) + 0.0, e, 1.0);
+                        double xlj = bx.xl[m], xuj = bx.xu[m];
+                        // interval of z_j (bivariate.py:563-597)
+                        double r0 = xlj * yl, r1 = xlj * yu, r2 = xuj * yl, r3 = xuj * yu;
+                        p_ql[v] = p_ql[v] + fmin(fmin(r0, r1), fmin(r2, r3));
+                        p_qu[v] = p_qu[v] + fmax(fmax(r0, r1), fmax(r2, r3));
+                        // coeff-aware McCormick lower plane
+                        // (bivariate.py:190-261, 40-75)
+                        double a1 = yl, b1 = xlj, c1 = (-xlj) * yl;
+                        double a2 = yu, b2 = xuj, c2 = (-xuj) * yu;
+                        double m1 = a1 * (a1 >= 0 ? xlj : xuj);
+                        m1 = m1 + b1 * (b1 >= 0 ? yl : yu);
+                        m1 = m1 + c1;
+                        double m2 = a2 * (a2 >= 0 ? xlj : xuj);
+                        m2 = m2 + b2 * (b2 >= 0 ? yl : yu);
+                        m2 = m2 + c2;
+                        double rr = (m1 >= m2) ? 1.0 : 0.0;
+                        double t = (yl - yu) * rr;
+                        double al = t + yu;
+                        t = (xlj - xuj) * rr;
+                        double be = t + xuj;
+                        double pp = yu * xuj, qq = yl * xlj;
+                        t = (pp - qq) * rr;
+                        double ga = t - pp;
+                        st.acm[m] = cs[j] + al;
+                        st.beta[m] = be;
+                        p_g[v] = p_g[v] + ga;
+                        p_own[v] = fma(fabs(be), k.hsn * mg, p_own[v]);
+                    }
+                }
+            }
+        }
+    }
+    st.ql = gh::tree4x4(p_ql);
+    st.qu = gh::tree4x4(p_qu);
+    st.gsum = gh::tree4x4(p_g);
+    st.own = gh::tree4x4(p_own);
+}
+
+// Backward half of row s.  qa points at the row's packed backward tiles
+// (+ lane).
+template <int KT>
+__device__ __forceinline__ void row_backward(int n, int S, int s, const double* qa,
+                                             const double* cs, double ds,
+                                             const BoxVals<KT>& bx, int lq, int box,
+                                             bool live, double* lb, double* lA, double* lbias,
+                                             int PR, const QcqpConsts& k, const RowState<KT>& st)
+{
+    constexpr int NCH = (KT + 3) / 4;
+    constexpr int NM = KT;
+    // ---- 3. Backward product: D[i][box] = sum_j Q[j][i] beta[box][j]
+    // (A = Q^T tile c, B = the lane's beta at index q + 4t).
+    d4 Da[NCH];
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) Da[c] = (d4){0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int t = 0; t < KT; ++t) {
+#pragma unroll
+        for (int c = 0; c < NCH; ++c)
+            Da[c] = mfma16(qa[((size_t)c * KT + t) * 64], st.beta[t], Da[c]);
+        // keep the operand loads of later steps from being hoisted here
+        // (they would hold KT * NCH registers)
+        if (t % 4 == 3) __builtin_amdgcn_sched_barrier(0);
+    }
+
+    // ---- 4. Merges, charges and concretization of box r, in two passes
+    // over the lane's indices to bound the live partial sums.
+    double p_lc[4], p_ldv[4], p_lmg[4], p_m1[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) p_lc[v] = p_ldv[v] = p_lmg[v] = p_m1[v] = 0.0;
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int m = 4 * c + v, i = lq + 4 * m;
+            if (m < NM && i < n) {
+                const double x0 = bx.x0(m), ep = bx.ep(m), mx = bx.mx(m);
+                double csi = cs[i];
+                double ca = fabs(csi);
+                p_lc[v] = fma(csi, x0, p_lc[v]);
+                p_ldv[v] = fma(ca, ep, p_ldv[v]);
+                p_lmg[v] = fma(ca, mx, p_lmg[v]);
+                p_m1[v] = fma(fabs(st.acm[m]), mx, p_m1[v]);
+            }
+        }
+    }
+    const double lc = gh::tree4x4(p_lc), ldv = gh::tree4x4(p_ldv), lmg = gh::tree4x4(p_lmg);
+    const double merge1 = gh::tree4x4(p_m1);
+    double p_m2[4], p_base[4], p_et[4], p_ma[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) p_m2[v] = p_base[v] = p_et[v] = p_ma[v] = 0.0;
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int m = 4 * c + v, i = lq + 4 * m;
+            if (m < NM && i < n) {
+                const double x0 = bx.x0(m), ep = bx.ep(m), mx = bx.mx(m);
+                double a = st.acm[m] + Da[c][v];
+                p_m2[v] = fma(fabs(a), mx, p_m2[v]);
+                p_base[v] = fma(a, x0, p_base[v]);
+                p_et[v] = fma(fabs(a), ep, p_et[v]);
+                p_ma[v] = fma(fabs(a), mx, p_ma[v]);
+                if (lA && live && s < PR) lA[((size_t)box * PR + s) * n + i] = a;
+            }
+        }
+    }
+    const double merge2 = gh::tree4x4(p_m2);
+    const double base = gh::tree4x4(p_base), et = gh::tree4x4(p_et), ma = gh::tree4x4(p_ma);
+    if (live && lq == 0) {
+        double le = (lmg + fabs(ds)) * k.kn;
+        double ll = gh::widen((lc - ldv) + ds, le, -1.0);
+        double lu = gh::widen((lc + ldv) + ds, le, 1.0);
+        double mag_add = (0.0 + gh::finite_abs(fmax(fabs(st.ql), fabs(st.qu))))
+                         + gh::finite_abs(fmax(fabs(ll), fabs(lu)));
+        double add_err = mag_add * k.k2;
+        double a = 0.0;
+        a = gh::add_enclosed(a, 0.0 - add_err, -1.0, k.k1);                    // f
+        a = gh::add_enclosed(a, 0.0, -1.0, k.k1);                              // q
+        double vec = (0.0 + k.hs * lmg) + k.hs * fabs(ds);
+        a = gh::add_enclosed(a, ds - vec, -1.0, k.k1);                         // l
+        a = gh::add_enclosed(a, st.gsum, -1.0, k.k1);                          // z
+        a = gh::add_enclosed(a, 0.0 - k.h1 * merge1, -1.0, k.k1);              // u
+        a = gh::add_enclosed(a, 0.0, -1.0, k.k1);                              // v
+        a = gh::add_enclosed(a, 0.0 - (st.own + k.h1 * merge2), -1.0, k.k1);   // y
+        double bias = gh::widen(a, gh::finite_abs(a) * k.kn, -1.0);
+        double root = gh::widen(base - et, ma * k.kn, -1.0);
+        if (!isfinite(root)) root = -INFINITY;
+        lb[(size_t)box * S + s] = gh::add_enclosed(bias, root, -1.0, k.k1);
+        if (lbias && s < PR) lbias[(size_t)box * PR + s] = a;
+    }
+}
+
 // One wavefront per 16 boxes, one box per lane column: lane (q, r) = lane
 // 16q + r owns box r and the variable indices q + 4m, m < KT = ceil(n/4).
 // The MFMAs take Q tiles as the A operand and the lane's box vectors as the
 // B operand, so every per-box vector (box rows, Cm + alpha, beta) stays in
-// the lane's registers.  IBP tiles are processed in groups of two to bound
-// the live accumulators.  STAGED: 8 wavefronts (128 boxes) share a
-// workgroup whose LDS holds the current row's Q tiles while the next row's
-// are copied in (double buffer), so the MFMA operands come from LDS rather
-// than from L2; the arithmetic and its order are the same either way.
-template <int KT, bool STAGED>
-__global__ __launch_bounds__(STAGED ? 512 : 64) __attribute__((amdgpu_waves_per_eu(2))) void qcqp_crown_f64_kernel(
+// the lane's registers.
+//
+// W wavefronts share a workgroup.  W = 1: Q tiles are read from L2.  W = 8:
+// the workgroup's LDS holds the current row's Q tiles while the next row's
+// are copied in (double buffer, direct global -> LDS copies), two waves per
+// SIMD.  FP64 MFMA and FP64 VALU work of one SIMD do not overlap on gfx950
+// (a wave running MFMAs and a wave running FMAs on one SIMD take the sum of
+// their separate times), so the second wave hides latency only; the row's
+// time is its MFMA cycles plus its VALU cycles plus stalls.  IBP tiles are
+// accumulated one or two at a time to bound the live registers.  Each box
+// row's arithmetic and its order are the same in every form.
+template <int KT, int W>
+__global__ __launch_bounds__(W * 64) __attribute__((amdgpu_waves_per_eu(2))) void qcqp_crown_f64_kernel(
     int B, int n, int S, const double* __restrict__ QB, const double* __restrict__ QA,
     const double* __restrict__ Cm, const double* __restrict__ d, const double* __restrict__ xL,
     const double* __restrict__ xU, double* __restrict__ lb, double* __restrict__ lA,
     double* __restrict__ lbias, int PR, QcqpConsts k)
 {
-    constexpr int NCH = (KT + 3) / 4;    // 16-wide tiles of j (and of i)
-    constexpr int NM = KT;               // per-lane indices q + 4m
-    constexpr int G = (KT > 12) ? 1 : 2; // IBP tiles per group
-    constexpr int W = STAGED ? 8 : 1;     // wavefronts per workgroup
-    extern __shared__ double qsm[];       // STAGED: 2 x [QB row | QA row]
+    constexpr int NCH = (KT + 3) / 4;
+    constexpr int NM = KT;
+    constexpr int G = (KT > 12) ? 1 : 2;  // IBP tiles per group
+    extern __shared__ double qsm[];       // W > 1: 2 x [QB row | QA row]
     const int l = threadIdx.x & 63;
     const int wv = threadIdx.x >> 6;
     const int lr = l & 15, lq = l >> 4;
@@ -98,16 +311,17 @@ __global__ __launch_bounds__(STAGED ? 512 : 64) __attribute__((amdgpu_waves_per_
     const int bo = min(box, B - 1);      // rows past B compute on box B-1, never stored
     const bool live = box < B;
 
-    double xl[NM], xu[NM];
+    BoxVals<KT> bx;
 #pragma unroll
     for (int m = 0; m < NM; ++m) {
         int i = lq + 4 * m;
-        xl[m] = (i < n) ? xL[(size_t)bo * n + i] : 0.0;
-        xu[m] = (i < n) ? xU[(size_t)bo * n + i] : 0.0;
+        bx.xl[m] = (i < n) ? xL[(size_t)bo * n + i] : 0.0;
+        bx.xu[m] = (i < n) ? xU[(size_t)bo * n + i] : 0.0;
     }
 
     const size_t rowstride = (size_t)NCH * KT * 64;
-    if constexpr (STAGED) {
+    RowState<KT> st;
+    if constexpr (W > 1) {
         gh::stage_row<double, W>(QB, QA, rowstride, 0, qsm, wv, l);
         __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
@@ -115,7 +329,7 @@ __global__ __launch_bounds__(STAGED ? 512 : 64) __attribute__((amdgpu_waves_per_
     for (int s = 0; s < S; ++s) {
         const double* qb;
         const double* qa;
-        if constexpr (STAGED) {
+        if constexpr (W > 1) {
             if (s + 1 < S)
                 gh::stage_row<double, W>(QB, QA, rowstride, s + 1,
                                          qsm + ((s + 1) & 1) * 2 * rowstride, wv, l);
@@ -127,177 +341,10 @@ __global__ __launch_bounds__(STAGED ? 512 : 64) __attribute__((amdgpu_waves_per_
             qa = QA + (size_t)s * rowstride + l;
         }
         const double* cs = Cm + (size_t)s * n;
-
-        double acm[NM], beta[NM];
-        double p_ql[4], p_qu[4], p_g[4], p_own[4];
-#pragma unroll
-        for (int v = 0; v < 4; ++v) p_ql[v] = p_qu[v] = p_g[v] = p_own[v] = 0.0;
-
-#pragma unroll
-        for (int c0 = 0; c0 < NCH; c0 += G) {
-            // ---- 1. IBP of y_s for tiles c0..c0+G-1: D[j][box] = sum_i
-            // Q[j][i] x[box][i] on MFMA (A = Q tile, B = the lane's x at
-            // index q + 4t); centre, radius and magnitude as in
-            // concretize_func.py:548-551, linear.py:607-653,
-            // backward_bound.py:739-741.
-            d4 Dc[G], Dd[G], Dm[G];
-#pragma unroll
-            for (int g = 0; g < G; ++g) Dc[g] = Dd[g] = Dm[g] = (d4){0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-            for (int t = 0; t < KT; ++t) {
-                double x0 = (xu[t] + xl[t]) / 2.0;
-                double ep = (xu[t] - xl[t]) / 2.0;
-                double mx = gh::finite_abs(fmax(-xl[t], xu[t]));
-#pragma unroll
-                for (int g = 0; g < G; ++g) {
-                    if (c0 + g < NCH) {
-                        double q = qb[((size_t)(c0 + g) * KT + t) * 64];
-                        double qa_ = fabs(q);
-                        Dc[g] = mfma16(q, x0, Dc[g]);
-                        Dd[g] = mfma16(qa_, ep, Dd[g]);
-                        Dm[g] = mfma16(qa_, mx, Dm[g]);
-                    }
-                }
-                if (t % 4 == 3) __builtin_amdgcn_sched_barrier(0);
-            }
-            // ---- 2. Elementwise: item v of tile c is index j = q + 4m,
-            // m = 4c + v, of box r.
-#pragma unroll
-            for (int g = 0; g < G; ++g) {
-#pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    const int c = c0 + g, m = 4 * c + v;
-                    if (c < NCH && m < NM) {
-                        const int j = lq + 4 * m;
-                        acm[m] = 0.0;
-                        beta[m] = 0.0;
-                        if (j < n) {
-                            double cc = Dc[g][v], dv = Dd[g][v], mg = Dm[g][v];
-                            double e = mg * k.kn;
-                            double yl = gh::widen((cc - dv) + 0.0, e, -1.0);
-                            double yu = gh::widen((cc + dv) + 0.0, e, 1.0);
-                            double xlj = xl[m], xuj = xu[m];
-                            // interval of z_j (bivariate.py:563-597)
-                            double r0 = xlj * yl, r1 = xlj * yu, r2 = xuj * yl, r3 = xuj * yu;
-                            p_ql[v] = p_ql[v] + fmin(fmin(r0, r1), fmin(r2, r3));
-                            p_qu[v] = p_qu[v] + fmax(fmax(r0, r1), fmax(r2, r3));
-                            // coeff-aware McCormick lower plane
-                            // (bivariate.py:190-261, 40-75)
-                            double a1 = yl, b1 = xlj, c1 = (-xlj) * yl;
-                            double a2 = yu, b2 = xuj, c2 = (-xuj) * yu;
-                            double m1 = a1 * (a1 >= 0 ? xlj : xuj);
-                            m1 = m1 + b1 * (b1 >= 0 ? yl : yu);
-                            m1 = m1 + c1;
-                            double m2 = a2 * (a2 >= 0 ? xlj : xuj);
-                            m2 = m2 + b2 * (b2 >= 0 ? yl : yu);
-                            m2 = m2 + c2;
-                            double rr = (m1 >= m2) ? 1.0 : 0.0;
-                            double t = (yl - yu) * rr;
-                            double al = t + yu;
-                            t = (xlj - xuj) * rr;
-                            double be = t + xuj;
-                            double pp = yu * xuj, qq = yl * xlj;
-                            t = (pp - qq) * rr;
-                            double ga = t - pp;
-                            acm[m] = cs[j] + al;
-                            beta[m] = be;
-                            p_g[v] = p_g[v] + ga;
-                            p_own[v] = fma(fabs(be), k.hsn * mg, p_own[v]);
-                        }
-                    }
-                }
-            }
-        }
-        const double ql = gh::tree4x4(p_ql), qu = gh::tree4x4(p_qu);
-        const double gsum = gh::tree4x4(p_g), own = gh::tree4x4(p_own);
-
-        // ---- 3. Backward product: D[i][box] = sum_j Q[j][i] beta[box][j]
-        // (A = Q^T tile c, B = the lane's beta at index q + 4t).
-        d4 Da[NCH];
-#pragma unroll
-        for (int c = 0; c < NCH; ++c) Da[c] = (d4){0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-        for (int t = 0; t < KT; ++t) {
-#pragma unroll
-            for (int c = 0; c < NCH; ++c)
-                Da[c] = mfma16(qa[((size_t)c * KT + t) * 64], beta[t], Da[c]);
-            // keep the operand loads of later steps from being hoisted here
-            // (they would hold KT * NCH registers)
-            if (t % 4 == 3) __builtin_amdgcn_sched_barrier(0);
-        }
-
-        // ---- 4. Merges, charges and concretization of box r, in two passes
-        // over the lane's indices to bound the live partial sums.
-        double p_lc[4], p_ldv[4], p_lmg[4], p_m1[4];
-#pragma unroll
-        for (int v = 0; v < 4; ++v) p_lc[v] = p_ldv[v] = p_lmg[v] = p_m1[v] = 0.0;
-#pragma unroll
-        for (int c = 0; c < NCH; ++c) {
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int m = 4 * c + v, i = lq + 4 * m;
-                if (m < NM && i < n) {
-                    double lo = xl[m], hi = xu[m];
-                    double x0 = (hi + lo) / 2.0, ep = (hi - lo) / 2.0;
-                    double mx = gh::finite_abs(fmax(-lo, hi));
-                    double csi = cs[i];
-                    double ca = fabs(csi);
-                    p_lc[v] = fma(csi, x0, p_lc[v]);
-                    p_ldv[v] = fma(ca, ep, p_ldv[v]);
-                    p_lmg[v] = fma(ca, mx, p_lmg[v]);
-                    p_m1[v] = fma(fabs(acm[m]), mx, p_m1[v]);
-                }
-            }
-        }
-        const double lc = gh::tree4x4(p_lc), ldv = gh::tree4x4(p_ldv), lmg = gh::tree4x4(p_lmg);
-        const double merge1 = gh::tree4x4(p_m1);
-        double p_m2[4], p_base[4], p_et[4], p_ma[4];
-#pragma unroll
-        for (int v = 0; v < 4; ++v) p_m2[v] = p_base[v] = p_et[v] = p_ma[v] = 0.0;
-#pragma unroll
-        for (int c = 0; c < NCH; ++c) {
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int m = 4 * c + v, i = lq + 4 * m;
-                if (m < NM && i < n) {
-                    double lo = xl[m], hi = xu[m];
-                    double x0 = (hi + lo) / 2.0, ep = (hi - lo) / 2.0;
-                    double mx = gh::finite_abs(fmax(-lo, hi));
-                    double a = acm[m] + Da[c][v];
-                    p_m2[v] = fma(fabs(a), mx, p_m2[v]);
-                    p_base[v] = fma(a, x0, p_base[v]);
-                    p_et[v] = fma(fabs(a), ep, p_et[v]);
-                    p_ma[v] = fma(fabs(a), mx, p_ma[v]);
-                    if (lA && live && s < PR) lA[((size_t)box * PR + s) * n + i] = a;
-                }
-            }
-        }
-        const double merge2 = gh::tree4x4(p_m2);
-        const double base = gh::tree4x4(p_base), et = gh::tree4x4(p_et), ma = gh::tree4x4(p_ma);
-        if (live && lq == 0) {
-            const double ds = d[s];
-            double le = (lmg + fabs(ds)) * k.kn;
-            double ll = gh::widen((lc - ldv) + ds, le, -1.0);
-            double lu = gh::widen((lc + ldv) + ds, le, 1.0);
-            double mag_add = (0.0 + gh::finite_abs(fmax(fabs(ql), fabs(qu))))
-                             + gh::finite_abs(fmax(fabs(ll), fabs(lu)));
-            double add_err = mag_add * k.k2;
-            double a = 0.0;
-            a = gh::add_enclosed(a, 0.0 - add_err, -1.0, k.k1);                    // f
-            a = gh::add_enclosed(a, 0.0, -1.0, k.k1);                              // q
-            double vec = (0.0 + k.hs * lmg) + k.hs * fabs(ds);
-            a = gh::add_enclosed(a, ds - vec, -1.0, k.k1);                         // l
-            a = gh::add_enclosed(a, gsum, -1.0, k.k1);                             // z
-            a = gh::add_enclosed(a, 0.0 - k.h1 * merge1, -1.0, k.k1);              // u
-            a = gh::add_enclosed(a, 0.0, -1.0, k.k1);                              // v
-            a = gh::add_enclosed(a, 0.0 - (own + k.h1 * merge2), -1.0, k.k1);      // y
-            double bias = gh::widen(a, gh::finite_abs(a) * k.kn, -1.0);
-            double root = gh::widen(base - et, ma * k.kn, -1.0);
-            if (!isfinite(root)) root = -INFINITY;
-            lb[(size_t)box * S + s] = gh::add_enclosed(bias, root, -1.0, k.k1);
-            if (lbias && s < PR) lbias[(size_t)box * PR + s] = a;
-        }
-        if constexpr (STAGED) {
+        row_forward<KT, G>(n, qb, cs, bx, lq, k, st);
+        row_backward<KT>(n, S, s, qa, cs, d[s], bx, lq, box, live, lb, lA, lbias, PR, k,
+                              st);
+        if constexpr (W > 1) {
             // the next row's tiles have landed, and every wave is done with
             // this row's buffer before it is refilled
             __builtin_amdgcn_s_waitcnt(0);
@@ -309,6 +356,7 @@ __global__ __launch_bounds__(STAGED ? 512 : 64) __attribute__((amdgpu_waves_per_
 // Batches of at least this many boxes take the LDS-staged form (128 boxes
 // per workgroup; one workgroup per CU at 32,768 boxes).
 constexpr int kStagedMinBoxes = 16384;
+constexpr int kStagedWaves = 8;
 
 template <int KT>
 int launch_f64(int B, int n, int S, const double* QB, const double* QA, const double* Cm,
@@ -316,22 +364,23 @@ int launch_f64(int B, int n, int S, const double* QB, const double* QA, const do
                double* lbias, int PR, const QcqpConsts& k, hipStream_t stream)
 {
     if (B >= kStagedMinBoxes) {
+        constexpr int W = kStagedWaves;
         const size_t rowstride = (size_t)((KT + 3) / 4) * KT * 64;
         const size_t lds = 4 * rowstride * sizeof(double);
         static bool attr = false;   // once per instantiation (same value every call)
         if (!attr) {
-            if (hipFuncSetAttribute((const void*)qcqp_crown_f64_kernel<KT, true>,
+            if (hipFuncSetAttribute((const void*)qcqp_crown_f64_kernel<KT, W>,
                                     hipFuncAttributeMaxDynamicSharedMemorySize,
                                     (int)lds) != hipSuccess)
                 return GH_ELAUNCH;
             attr = true;
         }
-        dim3 grid((B + 127) / 128);
-        hipLaunchKernelGGL((qcqp_crown_f64_kernel<KT, true>), grid, dim3(512), lds, stream, B, n,
+        dim3 grid((B + 16 * W - 1) / (16 * W));
+        hipLaunchKernelGGL((qcqp_crown_f64_kernel<KT, W>), grid, dim3(64 * W), lds, stream, B, n,
                            S, QB, QA, Cm, d, xL, xU, lb, lA, lbias, PR, k);
     } else {
         dim3 grid((B + 15) / 16);
-        hipLaunchKernelGGL((qcqp_crown_f64_kernel<KT, false>), grid, dim3(64), 0, stream, B, n, S,
+        hipLaunchKernelGGL((qcqp_crown_f64_kernel<KT, 1>), grid, dim3(64), 0, stream, B, n, S,
                            QB, QA, Cm, d, xL, xU, lb, lA, lbias, PR, k);
     }
     return hipGetLastError() == hipSuccess ? GH_OK : GH_ELAUNCH;

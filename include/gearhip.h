@@ -317,6 +317,27 @@ int gh_filter_append_f64(int B, int S, int n, const double *lb,
                          int64_t *total, double *disc_min, void *ws,
                          hipStream_t stream);
 
+/* Bytes of device workspace gh_split_append_f64 needs for K parents. */
+size_t gh_split_append_workspace_bytes(int K);
+
+/*
+ * The reordered step's decision, split, child keys, prune and append in
+ * one call (batch_branch_and_bound.py:470-534 followed by
+ * UnsortedInputDomainList.add, branching_domains.py:238-830): equal to
+ * gh_branch_split_keys_f64 on K parents, then gh_filter_append_f64 on the
+ * 2K children (base, cap, total, disc_min as there), then
+ * gh_fill_order_i64(2K, total, base, dead, order_out), without writing
+ * the children anywhere but the frontier: each survivor's box is rebuilt
+ * from its parent.  ws: gh_split_append_workspace_bytes(K) bytes.  The
+ * caller guarantees base + 2K <= cap.  Frontiers whose domains carry a
+ * payload use the separate calls.
+ */
+int gh_split_append_f64(int K, int n, int S, int R, const double *pL, const double *pU,
+                        const double *plb, const double *lA, const double *lbias,
+                        const double *thr, double *FL, double *FU, double *Flb, int64_t base,
+                        int64_t cap, int64_t *order_out, int64_t dead, int64_t *total,
+                        double *disc_min, void *ws, hipStream_t stream);
+
 /*
  * Companion of gh_filter_append_f64 for domains that carry a payload (the
  * cut rows of clip_type 'complete', stored with each domain by

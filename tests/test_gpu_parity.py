@@ -276,6 +276,66 @@ def test_branch_split_keys_matches_oracle(cuda, K, n, m):
     assert _bits_equal(k0.cpu().numpy(), np.vstack([plb, plb]))
 
 
+@pytest.mark.parametrize("K,n,m,planes", [(300, 12, 4, True), (257, 50, 20, True),
+                                          (40, 70, 2, True), (9, 1, 0, True), (1, 5, 1, True),
+                                          (123, 50, 20, False)])
+def test_split_append_equals_separate_calls(cuda, K, n, m, planes):
+    """gh_split_append_f64 leaves the frontier (physical rows, slot ids with
+    the dead record for pruned children, survivor count, dropped-threshold
+    minimum) exactly as branch_split_keys + filter_append(sync=False) do,
+    on parents with NaN and tied widths, invalid boxes, an infinite plane,
+    verified and pruned children, after rows already in the frontier."""
+    from gear_amd.crown import DeviceFrontier, branch_split_keys
+    print(f"case K={K} n={n} m={m} planes={planes}", flush=True)
+    rng = np.random.default_rng(K * 7 + n)
+    S = m + 1
+    L = -rng.random((K, n)) - 0.1
+    U = rng.random((K, n)) + 0.1
+    if n > 2:
+        U[::5, 1] = L[::5, 1] + (U[::5, 0] - L[::5, 0])
+        U[3::11, 0] = np.nan
+        U[4::13, 2] = np.nan
+        U[6::17, 1] = L[6::17, 1] - 1.0
+    plb = rng.standard_normal((K, S))
+    lA = rng.standard_normal((K, S, n)) if planes else None
+    lbias = rng.standard_normal((K, S)) if planes else None
+    if planes:
+        lbias[7 % K, 0] = np.inf
+    # thresholds that verify some children on the objective row and some on
+    # a constraint row, and keep the rest
+    thr = np.full(S, np.inf)
+    thr[0] = float(np.median(plb[:, 0])) + 0.5
+    if S > 1:
+        thr[1] = 1.0
+    d = lambda a: None if a is None else _dev(a, cuda)
+    fronts = []
+    for fused in (False, True):
+        f = DeviceFrontier(n, S, 4 * K + 64, device=cuda, slack=64)
+        pre = 13
+        f.push(_dev(np.zeros((pre, n)), cuda), _dev(np.ones((pre, n)), cuda),
+               _dev(np.full((pre, S), -5.0), cuda))
+        if fused:
+            f.split_append(d(plb), d(L), d(U), d(thr), d(lA), d(lbias), sync=False)
+        else:
+            cL, cU, keys = branch_split_keys(d(plb), d(L), d(U), d(lA), d(lbias))
+            f.filter_append(keys, d(thr), cL, cU, sync=False)
+        torch.cuda.synchronize()
+        fronts.append(f)
+    a, b = fronts
+    kept = int(a.total.item())
+    assert kept == int(b.total.item())
+    if K >= 40:
+        assert 0 < kept < 2 * K
+    assert a.count == b.count and a.hw == b.hw
+    assert torch.equal(a.order[:a.count], b.order[:b.count])
+    rows = 13 + kept
+    assert _bits_equal(a._L[:rows].cpu().numpy(), b._L[:rows].cpu().numpy())
+    assert _bits_equal(a._U[:rows].cpu().numpy(), b._U[:rows].cpu().numpy())
+    assert _bits_equal(a._lb[:rows].cpu().numpy(), b._lb[:rows].cpu().numpy())
+    assert _bits_equal(a.disc_min.cpu().numpy(), b.disc_min.cpu().numpy())
+    assert a.compact_order() == b.compact_order() == 13 + kept
+
+
 def test_unsynced_steps_equal_synced(cuda):
     """Steps issued without reading the survivor count (dead slots for the
     pruned children) leave, after compact_order, the same frontier and
