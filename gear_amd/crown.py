@@ -284,6 +284,7 @@ class DeviceFrontier:
         self.total = torch.zeros(1, dtype=torch.int64, device=self.device)
         self.disc_min = torch.full((1,), float("inf"), dtype=torch.float64, device=self.device)
         self._ws = None
+        self._sk = None         # sort buffers, allocated by the first sort
         self._pending = None
 
     def _alloc(self, rows):
@@ -380,21 +381,28 @@ class DeviceFrontier:
         if p >= c:
             self._sorted, self._tag = c, tag
             return
-        # keys of the unsorted tail (all rows when nothing is known sorted)
-        check(lib().gh_frontier_keys_f64(c - p, self.S, ptr(self.order[p:]), ptr(self._lb),
-                                         ptr(thr), ptr(self._okeys[p:c]), stream_ptr()),
-              "gh_frontier_keys_f64")
-        perm = torch.argsort(self._okeys[p:c], descending=True, stable=True)
+        if self._sk is None:
+            # sorted tail and the sort's scratch
+            self._sk = torch.empty(self.cap, dtype=torch.float64, device=self.device)
+            self._sv = torch.empty(self.cap, dtype=torch.int64, device=self.device)
+            self._tk = torch.empty_like(self._sk)
+            self._tv = torch.empty_like(self._sv)
+        n = c - p
         if p == 0:
-            torch.index_select(self.order[:c], 0, perm, out=self._order2[:c])
-            torch.index_select(self._okeys[:c], 0, perm, out=self._okeys2[:c])
+            # every row: keys and the stable sort straight into the other buffers
+            check(lib().gh_frontier_sort_f64(n, self.S, ptr(self.order), ptr(self._lb),
+                                             ptr(thr), ptr(self._okeys2), ptr(self._order2),
+                                             ptr(self._tk), ptr(self._tv), stream_ptr()),
+                  "gh_frontier_sort_f64")
         else:
             # sort the tail, then merge it behind the sorted prefix (ties:
             # prefix first) -- the stable sort of the whole
-            tk = self._okeys[p:c].index_select(0, perm)
-            tv = self.order[p:c].index_select(0, perm)
-            check(lib().gh_merge_desc_f64(p, ptr(self._okeys), ptr(self.order), c - p, ptr(tk),
-                                          ptr(tv), ptr(self._okeys2), ptr(self._order2),
+            check(lib().gh_frontier_sort_f64(n, self.S, ptr(self.order[p:]), ptr(self._lb),
+                                             ptr(thr), ptr(self._sk), ptr(self._sv),
+                                             ptr(self._tk), ptr(self._tv), stream_ptr()),
+                  "gh_frontier_sort_f64")
+            check(lib().gh_merge_desc_f64(p, ptr(self._okeys), ptr(self.order), n, ptr(self._sk),
+                                          ptr(self._sv), ptr(self._okeys2), ptr(self._order2),
                                           stream_ptr()), "gh_merge_desc_f64")
         self.order, self._order2 = self._order2, self.order
         self._okeys, self._okeys2 = self._okeys2, self._okeys

@@ -270,15 +270,11 @@ __global__ __launch_bounds__(256) void min_strided_kernel(int64_t N, int64_t str
 
 __global__ void set_i64_kernel(int64_t* p, int64_t v) { *p = v; }
 
-// Sort key of logical row i: max_s (lb[order[i]][s] - thr[s]), NaN if any
-// term is NaN (torch's amax).
-__global__ void frontier_keys_kernel(int64_t N, int S, const int64_t* __restrict__ order,
-                                     const double* __restrict__ lb, const double* __restrict__ thr,
-                                     double* __restrict__ keys)
+// Sort key of a bound row: max_s (r[s] - thr[s]), NaN if any term is NaN
+// (torch's amax).
+__device__ __forceinline__ double frontier_key(const double* __restrict__ r, int S,
+                                               const double* __restrict__ thr)
 {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= N) return;
-    const double* r = lb + order[i] * S;
     double m = -INFINITY;
     bool nan = false;
     for (int s = 0; s < S; ++s) {
@@ -286,7 +282,17 @@ __global__ void frontier_keys_kernel(int64_t N, int S, const int64_t* __restrict
         nan = nan || isnan(v);
         m = v > m ? v : m;
     }
-    keys[i] = nan ? NAN : m;
+    return nan ? NAN : m;
+}
+
+// Sort key of logical row i.
+__global__ void frontier_keys_kernel(int64_t N, int S, const int64_t* __restrict__ order,
+                                     const double* __restrict__ lb, const double* __restrict__ thr,
+                                     double* __restrict__ keys)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    keys[i] = frontier_key(lb + order[i] * S, S, thr);
 }
 
 // Descending-order rank key of a sort key: every NaN first (torch's
@@ -330,6 +336,103 @@ __global__ void merge_desc_kernel(int64_t nA, const double* __restrict__ kA,
         kO[j + lo] = kB[j];
         vO[j + lo] = vB[j];
     }
+}
+
+// Inverse of desc_key on its image: the key (every NaN as one NaN, -0 as
+// +0 -- the order desc_key gives is unchanged).
+__device__ __forceinline__ double from_desc_key(unsigned long long dk)
+{
+    const unsigned long long u = (dk >> 63) ? (dk & 0x7FFFFFFFFFFFFFFFull) : ~dk;
+    return __builtin_bit_cast(double, u);
+}
+
+constexpr int kSortTile = 1024;     // rows per tile-sort workgroup
+constexpr int kSortThreads = 256;   // its threads
+constexpr int kSortTileLog = 10;
+
+// Stable descending sort of one tile of kSortTile keys (keys[i] the key of
+// slot order[i]): a bitonic network in LDS on (~desc_key, position)
+// ascending -- descending key, ties in input order; positions past N pad
+// the tile after every key.  Out: the tile's keys and slot ids in sorted
+// order.
+__global__ __launch_bounds__(kSortThreads) void sort_tile_kernel(
+    int64_t N, const double* __restrict__ keys, const int64_t* __restrict__ order,
+    double* __restrict__ kOut, int64_t* __restrict__ vOut)
+{
+    __shared__ unsigned long long sk[kSortTile];
+    __shared__ int si[kSortTile];
+    const int64_t t0 = (int64_t)blockIdx.x * kSortTile;
+    const int cnt = (int)min((int64_t)kSortTile, N - t0);
+    for (int j = threadIdx.x; j < kSortTile; j += kSortThreads) {
+        // desc_key is never 0 on a key, so a real rank is below the padding's
+        sk[j] = j < cnt ? ~desc_key(keys[t0 + j]) : ~0ull;
+        si[j] = j;
+    }
+    __syncthreads();
+    for (int size = 2; size <= kSortTile; size <<= 1) {
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            for (int j = threadIdx.x; j < kSortTile / 2; j += kSortThreads) {
+                const int a = 2 * j - (j & (stride - 1)), b = a + stride;
+                const unsigned long long ka = sk[a], kb = sk[b];
+                const int ia = si[a], ib = si[b];
+                const bool a_first = ka < kb || (ka == kb && ia < ib);
+                if (a_first != ((a & size) == 0)) {
+                    sk[a] = kb;
+                    sk[b] = ka;
+                    si[a] = ib;
+                    si[b] = ia;
+                }
+            }
+            __syncthreads();
+        }
+    }
+    for (int j = threadIdx.x; j < cnt; j += kSortThreads) {
+        kOut[t0 + j] = from_desc_key(~sk[j]);
+        vOut[t0 + j] = order[t0 + si[j]];
+    }
+}
+
+// Rank of key x in the descending run [lo, hi): the number of its elements
+// strictly before x (tied elements after x), or, with ties_first, before
+// or tied with x.
+__device__ __forceinline__ int64_t run_rank(const double* __restrict__ k, int64_t lo, int64_t hi,
+                                            unsigned long long x, bool ties_first)
+{
+    const int64_t start = lo;
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        const unsigned long long y = desc_key(k[mid]);
+        if (y > x || (ties_first && y == x)) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo - start;
+}
+
+// One pass of the stable merge sort: runs of 2^lg sorted descending, runs
+// 4g .. 4g + 3 merged into one (ties: the earlier run first).  Each element
+// goes to its group's start + its index in its run + its rank in each
+// other run of the group (ties first for the runs before its own).
+__global__ void merge4_pass_kernel(int64_t N, int lg, const double* __restrict__ kIn,
+                                   const int64_t* __restrict__ vIn, double* __restrict__ kOut,
+                                   int64_t* __restrict__ vOut)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= N) return;
+    const int64_t run = (int64_t)1 << lg;
+    const int64_t r = t >> lg;
+    const int q = (int)(r & 3);
+    const int64_t g0 = (r - q) << lg;      // the group's start
+    const unsigned long long x = desc_key(kIn[t]);
+    int64_t pos = t - (r << lg);
+#pragma unroll
+    for (int o = 0; o < 4; ++o) {
+        if (o == q) continue;
+        const int64_t lo = g0 + o * run;
+        if (lo >= N) continue;
+        pos += run_rank(kIn, lo, min(lo + run, N), x, o < q);
+    }
+    kOut[g0 + pos] = kIn[t];
+    vOut[g0 + pos] = vIn[t];
 }
 
 // Slot ids of the children just appended: child t < *total (a survivor)
@@ -496,6 +599,40 @@ extern "C" int gh_merge_desc_f64(int64_t nA, const double* kA, const int64_t* vA
     if ((nA && (!kA || !vA)) || (nB && (!kB || !vB)) || !kO || !vO) return GH_EINVAL;
     hipLaunchKernelGGL(merge_desc_kernel, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, stream,
                        nA, kA, vA, nB, kB, vB, kO, vO);
+    return status();
+}
+
+extern "C" int gh_frontier_sort_f64(int64_t N, int S, const int64_t* order, const double* lb,
+                                    const double* thr, double* kOut, int64_t* vOut, double* kTmp,
+                                    int64_t* vTmp, hipStream_t stream)
+{
+    if (N < 0 || S <= 0) return GH_EINVAL;
+    if (N == 0) return GH_OK;
+    if (!order || !lb || !thr || !kOut || !vOut || !kTmp || !vTmp) return GH_EINVAL;
+    int passes = 0;
+    for (int lg = kSortTileLog; ((int64_t)1 << lg) < N; lg += 2) ++passes;
+    // keys first (into the buffer the tile sort does not write), the tile
+    // sort where the last merge pass must end in kOut
+    double* k0 = (passes & 1) ? kTmp : kOut;
+    int64_t* v0 = (passes & 1) ? vTmp : vOut;
+    double* k1 = (passes & 1) ? kOut : kTmp;
+    int64_t* v1 = (passes & 1) ? vOut : vTmp;
+    const unsigned blocks = (unsigned)((N + 255) / 256);
+    hipLaunchKernelGGL(frontier_keys_kernel, dim3(blocks), dim3(256), 0, stream, N, S, order, lb,
+                       thr, k1);
+    const int64_t tiles = (N + kSortTile - 1) / kSortTile;
+    hipLaunchKernelGGL(sort_tile_kernel, dim3((unsigned)tiles), dim3(kSortThreads), 0, stream, N,
+                       (const double*)k1, order, k0, v0);
+    for (int p = 0, lg = kSortTileLog; p < passes; ++p, lg += 2) {
+        hipLaunchKernelGGL(merge4_pass_kernel, dim3(blocks), dim3(256), 0, stream, N, lg, k0, v0,
+                           k1, v1);
+        double* kt = k0;
+        k0 = k1;
+        k1 = kt;
+        int64_t* vt = v0;
+        v0 = v1;
+        v1 = vt;
+    }
     return status();
 }
 

@@ -379,6 +379,17 @@ int gh_min_rows_f64(int64_t N, int64_t stride, const int64_t *idx, const double 
 int gh_merge_desc_f64(int64_t nA, const double *kA, const int64_t *vA, int64_t nB,
                       const double *kB, const int64_t *vB, double *kO, int64_t *vO,
                       hipStream_t stream);
+/* Stable sort of the N rows named by order[0..N) by their sort key
+ * max_s (lb[row][s] - thr[s]) descending (every NaN first, as torch's
+ * descending sort; -0 equals +0; equal keys keep their input order): the
+ * keys and slot ids in sorted order into kOut, vOut (N each); kTmp, vTmp
+ * (N each) are scratch.  Equal to gh_frontier_keys_f64 followed by torch's
+ * stable descending argsort and the gathers of keys and slot ids
+ * (UnsortedInputDomainList.sort, branching_domains.py:1171-1241); a key
+ * written out is the row's key up to the NaN payload and the sign of zero. */
+int gh_frontier_sort_f64(int64_t N, int S, const int64_t *order, const double *lb,
+                         const double *thr, double *kOut, int64_t *vOut, double *kTmp,
+                         int64_t *vTmp, hipStream_t stream);
 /* Slot ids of B children just appended by gh_filter_append_f64 at
  * physical row hw: out[t] = hw + t for the *total survivors, `dead` (the
  * frontier's dead record) for the rest, so the host need not read the

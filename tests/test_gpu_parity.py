@@ -381,6 +381,40 @@ def test_qcqp_crown_f64_staged_bit_exact(cuda, n, m, B):
     assert _bits_equal(lbias.cpu().numpy(), lbias_o)
 
 
+@pytest.mark.parametrize("N", [1, 100, 8191, 8192, 8193, 20000, 70000, 200000])
+def test_frontier_sort_equals_stable_argsort(cuda, N):
+    """gh_frontier_sort_f64 orders the rows named by a slot list exactly as
+    torch's stable descending argsort of their keys max_s (lb - thr) does
+    (ties, NaN, +-inf, -0, a repeated dead slot), and writes the keys."""
+    from gear_amd._lib import lib, ptr, stream_ptr
+    print(f"case N={N}", flush=True)
+    rng = np.random.default_rng(N)
+    S, rows = 3, N + 50
+    lb = np.round(rng.standard_normal((rows, S)) * 3) / 2                # many ties
+    lb[rng.random(rows) < 0.03, 1] = np.nan
+    lb[rng.random(rows) < 0.02, 0] = np.inf
+    lb[rng.random(rows) < 0.02, :] = -np.inf
+    lb[rng.random(rows) < 0.05, :] = -0.0
+    thr = np.array([0.0, 0.5, np.inf]) if N % 2 else np.array([0.0, -0.5, 1.0])
+    order = rng.permutation(rows)[:N]
+    order[rng.random(N) < 0.1] = rows - 1          # slots naming one record
+    keys = np.max(lb[order] - thr, axis=1)
+    keys[np.isnan(lb[order] - thr).any(axis=1)] = np.nan
+    kt = torch.as_tensor(keys, device=cuda)
+    perm = torch.argsort(kt, descending=True, stable=True)
+    o = torch.as_tensor(order, dtype=torch.int64, device=cuda)
+    kO = torch.empty(N, dtype=torch.float64, device=cuda)
+    vO = torch.empty(N, dtype=torch.int64, device=cuda)
+    kT, vT = torch.empty_like(kO), torch.empty_like(vO)
+    assert lib().gh_frontier_sort_f64(N, S, ptr(o), ptr(_dev(lb, cuda)), ptr(_dev(thr, cuda)),
+                                      ptr(kO), ptr(vO), ptr(kT), ptr(vT), stream_ptr()) == 0
+    assert torch.equal(vO, o[perm])
+    got, want = kO.cpu().numpy(), keys[perm.cpu().numpy()]
+    assert np.array_equal(np.isnan(got), np.isnan(want))
+    ok = ~np.isnan(want)
+    assert np.array_equal(got[ok], want[ok])        # -0 == +0
+
+
 def test_merge_desc_equals_stable_sort(cuda):
     """gh_merge_desc_f64 of a sorted prefix and a sorted tail equals torch's
     stable descending argsort of the concatenation (ties, NaN, +-inf, -0)."""
