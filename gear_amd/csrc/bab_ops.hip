@@ -701,15 +701,19 @@ constexpr double kU53 = 1.1102230246251565404236316680908203125e-16;
 // Wave-wide tail of a child key: the red64s sums of the per-lane partials
 // pm (plane value at the centre minus its radius term) and mg (magnitude),
 // the margin and the step toward -inf.
+__device__ __forceinline__ double child_key_final(int n, double pm_sum, double mg_sum, bool bad,
+                                                  double b, double p0)
+{
+    const double pm = pm_sum + b;
+    const double margin = (n + 4) * kU53 * (mg_sum + fabs(b));
+    const double key = nextafter(pm - margin, -INFINITY);
+    return (!bad && isfinite(key)) ? fmax(p0, key) : p0;
+}
+
 __device__ __forceinline__ double child_key_tail(int n, double pm, double mg, bool bad, double b,
                                                  double p0)
 {
-    pm = red64x(pm) + b;
-    mg = red64x(mg);
-    bad = __ballot(bad) != 0;
-    const double margin = (n + 4) * kU53 * (mg + fabs(b));
-    const double key = nextafter(pm - margin, -INFINITY);
-    return (!bad && isfinite(key)) ? fmax(p0, key) : p0;
+    return child_key_final(n, red64x(pm), red64x(mg), __ballot(bad) != 0, b, p0);
 }
 
 // Key of one child box [xl, xu] from its parent's objective plane (a, b)
@@ -862,122 +866,59 @@ branch_split_keys_kernel(int K, int n, int S, int R, const double *__restrict__ 
     }
 }
 
-// Register forms of first_widest, split_child_key and keep_row_wave when
-// n <= 64 and S <= 64: lane i holds coordinate i (li, ui, ai) and bound
-// row entry i (pr) with its threshold (th).  The same operations on the
-// same values as the pointer forms, whose loops then run once per lane.
-__device__ __forceinline__ int first_widest_lane(int n, int lane, double li, double ui)
-{
-    const double w = ui - li;
-    double bw = -INFINITY;
-    int bi = -1;
-    if (lane < n && !isnan(w)) {
-        bw = w;
-        bi = lane;
-    }
-    for (int o = 32; o >= 1; o >>= 1) {
-        const double w2 = __shfl_xor(bw, o);
-        const int i2 = __shfl_xor(bi, o);
-        if (i2 >= 0 && (bi < 0 || w2 > bw || (w2 == bw && i2 < bi))) {
-            bw = w2;
-            bi = i2;
-        }
-    }
-    const double w0 = __shfl(w, 0);
-    return (isnan(w0) || bi < 0) ? 0 : bi;
-}
-
-__device__ __forceinline__ double split_child_key_lane(int n, int lane, bool planes, double ai,
-                                                       double b, double li, double ui, int idx,
-                                                       int c, double p0, int* flags)
-{
-    const bool in = lane < n;
-    double lo = li, hi = ui;
-    split_coord(lo, hi, lane, idx, c);
-    const bool nonfin = in && (!isfinite(lo) || !isfinite(hi));
-    const bool inval = in && lo > hi;
-    *flags = (__ballot(nonfin) ? 1 : 0) | (__ballot(inval) ? 2 : 0);
-    if (!planes) return p0;
-    double pm = 0.0, mg = 0.0;
-    bool bad = false;
-    if (in) {
-        const double cc = (lo + hi) / 2, r = (hi - lo) / 2;
-        pm = pm + (ai * cc - fabs(ai) * r);
-        mg = mg + fabs(ai) * (fabs(cc) + r);
-        bad = !(lo <= hi);
-    }
-    return child_key_tail(n, pm, mg, bad, b, p0);
-}
-
-__device__ __forceinline__ bool keep_lane(int lane, int S, double L0, double pr, double th,
-                                          double T0, int box_flags, double* verified_thr)
-{
-    bool gt = false;
-    if (lane < S) {
-        const double Ls = lane == 0 ? L0 : pr;
-        gt = isfinite(Ls) && isfinite(th) && (Ls > th);
-        if (lane == 0) gt = gt || (isfinite(Ls) && isfinite(th) && (Ls >= th));
-    }
-    const bool ver = __ballot(gt) != 0 || (isinf(L0) && L0 > 0 && isfinite(T0));
-    *verified_thr = (ver && isfinite(T0)) ? T0 : INFINITY;
-    const bool nonfinite = box_flags & 1, invalid = box_flags & 2;
-    return !ver && !(invalid && !nonfinite);
-}
-
 // The reordered step's decision, split, child keys, prune and append
-// without materializing the children.  Pass 1, one wave per parent p
-// (workgroup = 4 parents): widest index, both children's keys, box flags
+// without materializing the children.  Pass 1, per parent p (workgroups
+// of kSplitBlock parents): widest index, both children's keys, box flags
 // and keep predicates (branch_split_keys_kernel's and filter_count_kernel's
 // values); per workgroup and side the kept count, and the dropped
 // thresholds' minimum into disc_min.
-template <bool SMALL>
-__global__ void __launch_bounds__(256)
-split_keep_kernel(int K, int n, int S, int R, const double *__restrict__ pL,
-                  const double *__restrict__ pU, const double *__restrict__ plb,
-                  const double *__restrict__ lA, const double *__restrict__ lbias,
-                  const double *__restrict__ thr, double *__restrict__ ckey,
-                  int32_t *__restrict__ sidx, int32_t *__restrict__ keep,
-                  int32_t *__restrict__ cnt, double *__restrict__ disc_min)
+constexpr int kSplitBlock = 16;
+
+// The workgroup's kept counts per side (nw waves' counts) and the dropped
+// thresholds' minimum m, written by thread 0.
+__device__ __forceinline__ void split_block_counts(int nw, const int* kc0, const int* kc1,
+                                                   int32_t* __restrict__ cnt, double m,
+                                                   double* __restrict__ disc_min)
 {
-    __shared__ int kc[2][4];
-    __shared__ double red[4];
+    if (threadIdx.x == 0) {
+        int a = 0, b = 0;
+        for (int w = 0; w < nw; ++w) {
+            a += kc0[w];
+            b += kc1[w];
+        }
+        cnt[blockIdx.x] = a;
+        cnt[gridDim.x + blockIdx.x] = b;
+        if (m < INFINITY) atomic_min_f64(disc_min, m);
+    }
+}
+
+// Any n and S: one wave per parent, 16 waves per workgroup.
+__global__ void __launch_bounds__(1024)
+split_keep_wave_kernel(int K, int n, int S, int R, const double *__restrict__ pL,
+                       const double *__restrict__ pU, const double *__restrict__ plb,
+                       const double *__restrict__ lA, const double *__restrict__ lbias,
+                       const double *__restrict__ thr, double *__restrict__ ckey,
+                       int32_t *__restrict__ sidx, int32_t *__restrict__ keep,
+                       int32_t *__restrict__ cnt, double *__restrict__ disc_min)
+{
+    __shared__ int kc[2][16];
+    __shared__ double red[16];
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int p = blockIdx.x * 4 + wv;
+    const int p = blockIdx.x * kSplitBlock + wv;
     int kept[2] = {0, 0};
     double vt = INFINITY;
     if (p < K) {
         const double *l = pL + (size_t)p * n, *u = pU + (size_t)p * n;
         const double *prow = plb + (size_t)p * S;
         const double *a = lA ? lA + (size_t)p * R * n : nullptr;
-        // SMALL: every operand in one round of independent loads
-        double li = 0.0, ui = 0.0, ai = 0.0, pr = 0.0, th = 0.0;
-        if constexpr (SMALL) {
-            if (lane < n) {
-                li = l[lane];
-                ui = u[lane];
-                if (a) ai = a[lane];
-            }
-            if (lane < S) {
-                pr = prow[lane];
-                th = thr[lane];
-            }
-        }
         const double b = lA ? lbias[(size_t)p * R] : 0.0;
-        const double p0 = SMALL ? __shfl(pr, 0) : prow[0];
-        const double T0 = SMALL ? __shfl(th, 0) : thr[0];
-        const int idx = SMALL ? first_widest_lane(n, lane, li, ui) : first_widest(n, lane, l, u);
+        const double p0 = prow[0];
+        const int idx = first_widest(n, lane, l, u);
         for (int c = 0; c < 2; ++c) {
             int fl;
             double v;
-            double key;
-            bool k;
-            if constexpr (SMALL) {
-                key = split_child_key_lane(n, lane, a != nullptr, ai, b, li, ui, idx, c, p0, &fl);
-                k = keep_lane(lane, S, key, pr, th, T0, fl, &v);
-            } else {
-                key = split_child_key(n, lane, a, b, l, u, idx, c, p0, &fl);
-                k = keep_row_wave(lane, key, prow, S, thr, fl, &v);
-            }
+            const double key = split_child_key(n, lane, a, b, l, u, idx, c, p0, &fl);
+            const bool k = keep_row_wave(lane, key, prow, S, thr, fl, &v);
             const size_t ch = (size_t)p + (size_t)c * K;
             if (lane == 0) {
                 ckey[ch] = key;
@@ -993,20 +934,155 @@ split_keep_kernel(int K, int n, int S, int R, const double *__restrict__ pL,
         kc[1][wv] = kept[1];
     }
     const double m = block_min(vt, red);   // synchronizes the workgroup first
-    if (threadIdx.x == 0) {
-        cnt[blockIdx.x] = kc[0][0] + kc[0][1] + kc[0][2] + kc[0][3];
-        cnt[gridDim.x + blockIdx.x] = kc[1][0] + kc[1][1] + kc[1][2] + kc[1][3];
-        if (m < INFINITY) atomic_min_f64(disc_min, m);
+    split_block_counts(16, kc[0], kc[1], cnt, m, disc_min);
+}
+
+// OR of a predicate over the 16-lane group (lanes 16g .. 16g + 15) of lane.
+__device__ __forceinline__ bool group_any(bool x, int lane)
+{
+    return ((__ballot(x) >> (lane & 48)) & 0xFFFFull) != 0;
+}
+
+// n <= 64 and S <= 64: four parents per wave, 16 lanes each; lane gl of a
+// group holds coordinates and bound rows gl + 16k (k < 4).  The child-key
+// sums are red64s: lane gl's four partials are the 64-lane form's lanes
+// gl + 16k, so levels l^32 and l^16 are register pairs (k, k ^ 2) and
+// (k, k ^ 1) and levels l^8 .. l^1 the 16-lane DPP tree -- the same operand
+// pairs.  The widest index is the unique (largest width, lowest index), so
+// any reduction order finds it.
+__global__ void __launch_bounds__(256)
+split_keep16_kernel(int K, int n, int S, int R, const double *__restrict__ pL,
+                    const double *__restrict__ pU, const double *__restrict__ plb,
+                    const double *__restrict__ lA, const double *__restrict__ lbias,
+                    const double *__restrict__ thr, double *__restrict__ ckey,
+                    int32_t *__restrict__ sidx, int32_t *__restrict__ keep,
+                    int32_t *__restrict__ cnt, double *__restrict__ disc_min)
+{
+    __shared__ int kc[2][4];
+    __shared__ double red[4];
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int gl = lane & 15;
+    const int p = blockIdx.x * kSplitBlock + wv * 4 + (lane >> 4);
+    const bool valid = p < K;
+    const bool planes = lA != nullptr;
+    double li[4], ui[4], ai[4], pr[4], th[4];
+    double b = 0.0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) li[k] = ui[k] = ai[k] = pr[k] = th[k] = 0.0;
+    if (valid) {
+        const double *l = pL + (size_t)p * n, *u = pU + (size_t)p * n;
+        const double *prow = plb + (size_t)p * S;
+        const double *a = planes ? lA + (size_t)p * R * n : nullptr;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int i = gl + 16 * k;
+            if (i < n) {
+                li[k] = l[i];
+                ui[k] = u[i];
+                if (planes) ai[k] = a[i];
+            }
+            if (i < S) {
+                pr[k] = prow[i];
+                th[k] = thr[i];
+            }
+        }
+        if (planes) b = lbias[(size_t)p * R];
     }
+    const int lead = lane & 48;
+    const double p0 = __shfl(pr[0], lead), T0 = __shfl(th[0], lead);
+    // first widest coordinate: per-lane candidates in index order, then the
+    // 16-lane reduction
+    double bw = -INFINITY;
+    int bi = -1;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int i = gl + 16 * k;
+        const double w = ui[k] - li[k];
+        if (i < n && (bi < 0 ? !isnan(w) : w > bw)) {
+            bw = w;
+            bi = i;
+        }
+    }
+#pragma unroll
+    for (int o = 8; o >= 1; o >>= 1) {
+        const double w2 = __shfl_xor(bw, o);
+        const int i2 = __shfl_xor(bi, o);
+        if (i2 >= 0 && (bi < 0 || w2 > bw || (w2 == bw && i2 < bi))) {
+            bw = w2;
+            bi = i2;
+        }
+    }
+    const double w0 = __shfl(ui[0] - li[0], lead);
+    const int idx = (isnan(w0) || bi < 0) ? 0 : bi;
+    int kept[2];
+    double vt = INFINITY;
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+        double pm[4], mg[4];
+        bool bad = false, nonfin = false, inval = false;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int i = gl + 16 * k;
+            double lo = li[k], hi = ui[k];
+            split_coord(lo, hi, i, idx, c);
+            pm[k] = 0.0;
+            mg[k] = 0.0;
+            if (i < n) {
+                nonfin = nonfin || !isfinite(lo) || !isfinite(hi);
+                inval = inval || lo > hi;
+                const double cc = (lo + hi) / 2, r = (hi - lo) / 2;
+                pm[k] = pm[k] + (ai[k] * cc - fabs(ai[k]) * r);
+                mg[k] = mg[k] + fabs(ai[k]) * (fabs(cc) + r);
+                bad = bad || !(lo <= hi);
+            }
+        }
+        const int fl = (group_any(nonfin, lane) ? 1 : 0) | (group_any(inval, lane) ? 2 : 0);
+        double key = p0;
+        if (planes) {
+            const double ps = gh::tree16((pm[0] + pm[2]) + (pm[1] + pm[3]));
+            const double ms = gh::tree16((mg[0] + mg[2]) + (mg[1] + mg[3]));
+            key = child_key_final(n, ps, ms, group_any(bad, lane), b, p0);
+        }
+        // keep predicate on the row [key, plb[p][1:]]
+        bool gt_any = false;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int s = gl + 16 * k;
+            if (s < S) {
+                const double Ls = s == 0 ? key : pr[k], Ts = th[k];
+                bool gt = isfinite(Ls) && isfinite(Ts) && (Ls > Ts);
+                if (s == 0) gt = gt || (isfinite(Ls) && isfinite(Ts) && (Ls >= Ts));
+                gt_any = gt_any || gt;
+            }
+        }
+        const bool ver = group_any(gt_any, lane) || (isinf(key) && key > 0 && isfinite(T0));
+        const double v = (ver && isfinite(T0)) ? T0 : INFINITY;
+        const bool kp = !ver && !((fl & 2) && !(fl & 1));
+        const size_t ch = (size_t)p + (size_t)c * K;
+        if (valid && gl == 0) {
+            ckey[ch] = key;
+            keep[ch] = kp ? 1 : 0;
+        }
+        kept[c] = __popcll(__ballot(valid && gl == 0 && kp));
+        if (valid) vt = fmin(vt, v);
+    }
+    if (valid && gl == 0) sidx[p] = idx;
+    if (lane == 0) {
+        kc[0][wv] = kept[0];
+        kc[1][wv] = kept[1];
+    }
+    const double m = block_min(vt, red);   // synchronizes the workgroup first
+    split_block_counts(4, kc[0], kc[1], cnt, m, disc_min);
 }
 
 // Pass 3 (after filter_scan_kernel turned the counts, side 0 blocks then
-// side 1 blocks, into exclusive offsets): one wave per parent p writes each
-// kept child's box, rebuilt from the parent, and bound row [key, plb[p][1:]]
-// to frontier row base + rank, and both children's slot ids (base + t for
-// the first *total positions t, the dead record after them).  Every load
-// is issued before the first store.
-__global__ void __launch_bounds__(256)
+// side 1 blocks, into exclusive offsets): one wave per parent p (16 per
+// workgroup, the blocks of pass 1) writes each kept child's box, rebuilt
+// from the parent, and bound row [key, plb[p][1:]] to frontier row
+// base + rank, and both children's slot ids (base + t for the first *total
+// positions t, the dead record after them).  Every load is issued before
+// the first store.
+__global__ void __launch_bounds__(1024)
 split_copy_kernel(int K, int n, int S, const double *__restrict__ pL,
                   const double *__restrict__ pU, const double *__restrict__ plb,
                   const double *__restrict__ ckey, const int32_t *__restrict__ sidx,
@@ -1017,11 +1093,11 @@ split_copy_kernel(int K, int n, int S, const double *__restrict__ pL,
 {
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int blk = blockIdx.x, nblk = gridDim.x;
-    const int p = blk * 4 + wv;
+    const int p = blk * kSplitBlock + wv;
     if (p >= K) return;
-    // keep flags of the block's parents: lanes 0..3 side 0, lanes 4..7 side 1
-    const int q = blk * 4 + (lane & 3);
-    const bool kq = lane < 8 && q < K && keep[(size_t)q + (size_t)(lane >> 2) * K];
+    // keep flags of the block's parents: lanes 0..15 side 0, 16..31 side 1
+    const int q = blk * kSplitBlock + (lane & 15);
+    const bool kq = lane < 32 && q < K && keep[(size_t)q + (size_t)(lane >> 4) * K];
     const int32_t off0 = offs[blk], off1 = offs[nblk + blk];
     const int idx = sidx[p];
     const int64_t T = *total;
@@ -1032,7 +1108,7 @@ split_copy_kernel(int K, int n, int S, const double *__restrict__ pL,
     const double pb = lane < S ? prow[lane] : 0.0;
     const unsigned long long km = __ballot(kq);
     for (int c = 0; c < 2; ++c) {
-        const unsigned long long side = (km >> (4 * c)) & 0xFull;
+        const unsigned long long side = (km >> (16 * c)) & 0xFFFFull;
         if (!((side >> wv) & 1)) continue;
         const int64_t row =
             base + (c ? off1 : off0) + __popcll(side & ((1ull << wv) - 1ull));
@@ -1089,7 +1165,7 @@ extern "C" int gh_child_keys_f64(int K, int n, int S, int R, const double *plb,
 extern "C" size_t gh_split_append_workspace_bytes(int K)
 {
     if (K < 0) return 0;
-    const size_t nblk = ((size_t)K + 3) / 4;
+    const size_t nblk = ((size_t)K + kSplitBlock - 1) / kSplitBlock;
     return sizeof(double) * 2 * (size_t)K + sizeof(int32_t) * (3 * (size_t)K + 2 * nblk) + 256;
 }
 
@@ -1107,19 +1183,19 @@ extern "C" int gh_split_append_f64(int K, int n, int S, int R, const double *pL,
     }
     if (!pL || !pU || !plb || !thr || !FL || !FU || !Flb || !order_out || !ws) return GH_EINVAL;
     if (base + 2 * (int64_t)K > cap) return GH_EINVAL;
-    const int nblk = (K + 3) / 4;
+    const int nblk = (K + kSplitBlock - 1) / kSplitBlock;
     double *ckey = (double *)ws;
     int32_t *sidx = (int32_t *)(ckey + 2 * (size_t)K);
     int32_t *keep = sidx + K;
     int32_t *cnt = keep + 2 * (size_t)K;
     if (n <= 64 && S <= 64)
-        hipLaunchKernelGGL(split_keep_kernel<true>, dim3(nblk), dim3(256), 0, stream, K, n, S, R,
-                           pL, pU, plb, lA, lbias, thr, ckey, sidx, keep, cnt, disc_min);
+        hipLaunchKernelGGL(split_keep16_kernel, dim3(nblk), dim3(256), 0, stream, K, n, S, R, pL,
+                           pU, plb, lA, lbias, thr, ckey, sidx, keep, cnt, disc_min);
     else
-        hipLaunchKernelGGL(split_keep_kernel<false>, dim3(nblk), dim3(256), 0, stream, K, n, S,
-                           R, pL, pU, plb, lA, lbias, thr, ckey, sidx, keep, cnt, disc_min);
+        hipLaunchKernelGGL(split_keep_wave_kernel, dim3(nblk), dim3(1024), 0, stream, K, n, S, R,
+                           pL, pU, plb, lA, lbias, thr, ckey, sidx, keep, cnt, disc_min);
     hipLaunchKernelGGL(filter_scan_kernel, dim3(1), dim3(1024), 0, stream, 2 * nblk, cnt, total);
-    hipLaunchKernelGGL(split_copy_kernel, dim3(nblk), dim3(256), 0, stream, K, n, S, pL, pU,
+    hipLaunchKernelGGL(split_copy_kernel, dim3(nblk), dim3(1024), 0, stream, K, n, S, pL, pU,
                        plb, ckey, sidx, keep, cnt, total, FL, FU, Flb, base, order_out, dead);
     return hipGetLastError() == hipSuccess ? GH_OK : GH_ELAUNCH;
 }
