@@ -443,7 +443,8 @@ def main():
         e0.record(stream)
         out = orig_bound(x_L, x_U, **kw)
         e1.record(stream)
-        events.append((e0, e1, x_L.shape[0]))
+        rows = kw.get("rows")
+        events.append((e0, e1, x_L.shape[0] if rows is None else rows.shape[0]))
         return out
 
     bb.net.bound = timed_bound

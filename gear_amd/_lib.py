@@ -55,8 +55,10 @@ SIGNATURES = {
     "gh_merge_desc_f64": [ctypes.c_int64, _P, _P, ctypes.c_int64, _P, _P, _P, _P, _P],
     "gh_split_append_workspace_bytes": [_I],
     "gh_frontier_sort_f64": [_L, _I, _P, _P, _P, _P, _P, _P, _P, _P],
-    "gh_split_append_f64": [_I, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _L, _L, _P, _L,
-                            _P, _P, _P, _P],
+    "gh_split_append_f64": [_I, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _L, _L, _P,
+                            _L, _P, _P, _P, _P],
+    "gh_qcqp_crown_rows_f64": [_I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I, _P],
+    "gh_qcqp_crown_rows_f32s": [_I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I, _P],
     "gh_dag_plan_create": [_I, _P, ctypes.POINTER(_P)],
     "gh_dag_plan_destroy": [_P],
     "gh_dag_plan_info": [_P, _P, _P, _P],

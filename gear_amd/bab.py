@@ -58,7 +58,7 @@ def no_cuts_row(n, S, device):
 
 
 def reordered_step(f, bound, thr, nb, n, S, planes=True, lp=None, clip=False,
-                   clip_max_rows=16, sync=True):
+                   clip_max_rows=16, sync=True, in_place=False):
     """One reordered B&B step on the last nb domains of frontier f
     (batch_branch_and_bound.py:395-560 with reorder_bab): bound them
     (CROWN; with clip 'complete' concretized over the box and the cuts they
@@ -69,7 +69,18 @@ def reordered_step(f, bound, thr, nb, n, S, planes=True, lp=None, clip=False,
     by its parent's bound and objective plane (:189-225), prune and append
     (children of a verified parent inherit its verified rows, so this one
     prune drops them).  bound(L, U, return_A=, plane_rows=) is the bound
-    of the served form (FP64 or FP32-sound)."""
+    of the served form (FP64 or FP32-sound).  in_place: bound(..., rows=)
+    accepts a row index, so without clip and LP the picked domains are
+    bound and split where the frontier stores them (no gather)."""
+    if in_place and not clip and lp is None and f.in_place_room(nb):
+        idx, pkey = f.pick_rows(nb)
+        L, U = f.storage()
+        if planes:
+            lb, lA, lbias = bound(L, U, return_A=True, plane_rows=1, rows=idx)
+        else:
+            lb, lA, lbias = bound(L, U, rows=idx), None, None
+        lb = torch.maximum(lb, pkey)
+        return f.split_append(lb, L, U, thr, lA, lbias, sync=sync, rows=idx)
     pL, pU, pkey = f.pick_out(nb)
     if planes:
         # the LP and the cuts read every row's plane, the child keys the
@@ -267,9 +278,10 @@ class QCQPBranchAndBound:
         if nb <= 0:
             return 0
         lp = self.raise_by_lp if (self.cfg.dual_lp_iters > 0 and self.P < float("inf")) else None
-        reordered_step(f, self.net.bound, self.thr, nb, self.p.n, self.p.S,
-                       planes=not getattr(self.net, "large", False), lp=lp, clip=self.clip,
-                       clip_max_rows=self.cfg.clip_max_rows, sync=sync)
+        large = getattr(self.net, "large", False)
+        reordered_step(f, self.net.bound, self.thr, nb, self.p.n, self.p.S, planes=not large,
+                       lp=lp, clip=self.clip, clip_max_rows=self.cfg.clip_max_rows, sync=sync,
+                       in_place=not large)
         self.stats.iterations += 1
         self.stats.boxes_bounded += nb
         return nb

@@ -40,6 +40,19 @@ class BoundedTensor:
         self.data, self.ptb = x, ptb
 
 
+def _check_boxes(x_L, x_U, n, rows):
+    """Number of boxes to bound: x_L.shape[0], or len(rows) when the boxes are
+    rows of x_L, x_U (int64 indices; the caller keeps them inside x_L, the
+    kernels do not check them)."""
+    if x_L.dim() != 2 or x_L.shape != x_U.shape or x_L.shape[1] != n:
+        raise ValueError(f"boxes must be (B, {n})")
+    if rows is None:
+        return x_L.shape[0]
+    if rows.dtype != torch.int64 or rows.dim() != 1 or not rows.is_contiguous():
+        raise ValueError("rows must be a contiguous int64 vector")
+    return rows.shape[0]
+
+
 class BoundedQCQP:
     """A QCQP program bound on one device.  `compute_bounds` is the CROWN
     lower bound of every row over every box (C = eye(S))."""
@@ -86,21 +99,22 @@ class BoundedQCQP:
                                          stream_ptr()), "gh_qcqp_pack_f32")
         self.last_flagged = 0
 
-    def bound(self, x_L, x_U, return_A=False, out=None, stream=None, plane_rows=None):
+    def bound(self, x_L, x_U, return_A=False, out=None, stream=None, plane_rows=None,
+              rows=None):
         """lb (B, S) [, lA (B, R, n), lbias (B, R)] for boxes x_L, x_U (B, n),
         with the planes of the leading R = plane_rows rows (default S).  In
         the FP32-sound form the planes are lA x + lbias <= f_s(x) on the box
-        with the planes' rigorous error folded into lbias (n <= 64)."""
+        with the planes' rigorous error folded into lbias (n <= 64).  rows
+        (B,) int64: bound the boxes rows[b] of x_L, x_U (any number of rows)
+        where they are stored, without gathering them."""
         if self.precision == "fp32s":
             return self.bound_fp32s(x_L, x_U, out=out, stream=stream, return_A=return_A,
-                                    plane_rows=plane_rows)
+                                    plane_rows=plane_rows, rows=rows)
         x_L = x_L.contiguous()
         x_U = x_U.contiguous()
         if x_L.dtype != torch.float64 or x_U.dtype != torch.float64:
             raise TypeError("the FP64 path takes float64 boxes")
-        B = x_L.shape[0]
-        if x_L.shape != (B, self.n) or x_U.shape != (B, self.n):
-            raise ValueError(f"boxes must be (B, {self.n})")
+        B = _check_boxes(x_L, x_U, self.n, rows)
         lb = out if out is not None else torch.empty(B, self.S, dtype=torch.float64,
                                                      device=self.device)
         lA = lbias = None
@@ -109,20 +123,28 @@ class BoundedQCQP:
             lA = torch.empty(B, R, self.n, dtype=torch.float64, device=self.device)
             lbias = torch.empty(B, R, dtype=torch.float64, device=self.device)
         d = self.dev
-        check(lib().gh_qcqp_crown_f64(B, self.n, self.S, ptr(self.packed), ptr(d.c), ptr(d.d),
-                                      ptr(x_L), ptr(x_U), ptr(lb), ptr(lA), ptr(lbias), R,
-                                      stream_ptr(stream)), "gh_qcqp_crown_f64")
+        if rows is None:
+            check(lib().gh_qcqp_crown_f64(B, self.n, self.S, ptr(self.packed), ptr(d.c),
+                                          ptr(d.d), ptr(x_L), ptr(x_U), ptr(lb), ptr(lA),
+                                          ptr(lbias), R, stream_ptr(stream)), "gh_qcqp_crown_f64")
+        else:
+            check(lib().gh_qcqp_crown_rows_f64(B, self.n, self.S, ptr(self.packed), ptr(d.c),
+                                               ptr(d.d), ptr(x_L), ptr(x_U), ptr(rows), ptr(lb),
+                                               ptr(lA), ptr(lbias), R, stream_ptr(stream)),
+                  "gh_qcqp_crown_rows_f64")
         return (lb, lA, lbias) if return_A else lb
 
-    def bound_fp32s(self, x_L, x_U, out=None, stream=None, return_A=False, plane_rows=None):
+    def bound_fp32s(self, x_L, x_U, out=None, stream=None, return_A=False, plane_rows=None,
+                    rows=None):
         """FP32-sound lower bounds (B, S) as float64 values, each <= the FP64
         bound of the same box.  `last_flag` marks boxes where a near-tie
-        McCormick choice was settled in FP64 (diagnostic)."""
+        McCormick choice was settled in FP64 (diagnostic).  rows: as bound()
+        (n <= 64)."""
         x_L = x_L.contiguous()
         x_U = x_U.contiguous()
-        B = x_L.shape[0]
-        if x_L.shape != (B, self.n) or x_U.shape != (B, self.n):
-            raise ValueError(f"boxes must be (B, {self.n})")
+        if rows is not None and self.large:
+            raise _lib.GearHipError("the large-n FP32-sound path bounds gathered boxes only")
+        B = _check_boxes(x_L, x_U, self.n, rows)
         lb = out if out is not None else torch.empty(B, self.S, dtype=torch.float64,
                                                      device=self.device)
         flag = torch.empty(B, dtype=torch.int32, device=self.device)
@@ -148,9 +170,16 @@ class BoundedQCQP:
         if return_A:
             lA = torch.empty(B, R, self.n, dtype=torch.float64, device=self.device)
             lbias = torch.empty(B, R, dtype=torch.float64, device=self.device)
-        check(lib().gh_qcqp_crown_f32s(B, self.n, self.S, ptr(self.packed32), ptr(d.Q), ptr(d.c),
-                                       ptr(d.d), ptr(x_L), ptr(x_U), ptr(lb), ptr(flag), ptr(lA),
-                                       ptr(lbias), R, stream_ptr(stream)), "gh_qcqp_crown_f32s")
+        if rows is None:
+            check(lib().gh_qcqp_crown_f32s(B, self.n, self.S, ptr(self.packed32), ptr(d.Q),
+                                           ptr(d.c), ptr(d.d), ptr(x_L), ptr(x_U), ptr(lb),
+                                           ptr(flag), ptr(lA), ptr(lbias), R, stream_ptr(stream)),
+                  "gh_qcqp_crown_f32s")
+        else:
+            check(lib().gh_qcqp_crown_rows_f32s(B, self.n, self.S, ptr(self.packed32), ptr(d.Q),
+                                                ptr(d.c), ptr(d.d), ptr(x_L), ptr(x_U), ptr(rows),
+                                                ptr(lb), ptr(flag), ptr(lA), ptr(lbias), R,
+                                                stream_ptr(stream)), "gh_qcqp_crown_rows_f32s")
         self.last_flag = flag
         return (lb, lA, lbias) if return_A else lb
 
@@ -426,6 +455,33 @@ class DeviceFrontier:
                                        stream_ptr()), "gh_gather_rows_f64")
         return pL, pU, plb
 
+    def pick_rows(self, batch):
+        """Slot ids and bound rows (idx, lb) of the last `batch` logical rows,
+        which leave the frontier as in pick_out; their boxes are read in place
+        (rows idx of the frontier's storage, self.storage()) by the bound and
+        split_append, which avoids gathering them.  idx is a view of the
+        order: the next append overwrites it (split_append reads each entry
+        before it writes the same slot)."""
+        nb = min(batch, self.count)
+        s = self.count - nb
+        self.count = s
+        self._sorted = min(self._sorted, s)
+        idx = self.order[s:s + nb]
+        self._picked = idx
+        plb = torch.empty(nb, self.S, dtype=torch.float64, device=self.device)
+        check(lib().gh_gather_rows_f64(nb, ptr(idx), self.S, ptr(self._lb), ptr(plb), 0, None,
+                                       None, 0, None, None, stream_ptr()), "gh_gather_rows_f64")
+        return idx, plb
+
+    def storage(self):
+        """The physical box arrays (x_L, x_U) that pick_rows' slot ids index."""
+        return self._L, self._U
+
+    def in_place_room(self, batch):
+        """True when picking `batch` rows with pick_rows and appending their
+        2 batch children leaves the picked rows in place (no compaction)."""
+        return self.hw + 2 * min(batch, self.count) <= self.phys
+
     def pick_payload(self, nb):
         """The payload rows of the domains the last pick_out(nb) returned."""
         return self._pay.index_select(0, self._picked[:nb])
@@ -436,17 +492,19 @@ class DeviceFrontier:
             self._ws = torch.empty(need, dtype=torch.uint8, device=self.device)
         return self._ws
 
-    def split_append(self, plb, pL, pU, thr, lA=None, lbias=None, sync=True):
+    def split_append(self, plb, pL, pU, thr, lA=None, lbias=None, sync=True, rows=None):
         """Widest branching, bisection, child keys from the parents'
         objective planes, prune and append for K parents in one call
         (gh_split_append_f64): the frontier branch_split_keys followed by
         filter_append of the 2K children would give, without materializing
         the children.  lA (K, R, n) / lbias (K, R) may be None (the keys are
-        the parents' bounds).  Returns the survivor count when sync is set,
-        else None (as filter_append)."""
+        the parents' bounds).  rows (K,) int64: parent p's box is row rows[p]
+        of pL, pU (e.g. this frontier's own storage after pick_rows; the
+        append must then not compact it, see in_place_room).  Returns the
+        survivor count when sync is set, else None (as filter_append)."""
         if self._pay is not None:
             raise ValueError("a frontier with a payload appends through filter_append")
-        K = pL.shape[0]
+        K = plb.shape[0] if rows is not None else pL.shape[0]
         B = 2 * K
         R = 1
         if lA is not None:
@@ -454,15 +512,23 @@ class DeviceFrontier:
             if lA.shape != (K, R, self.n) or lbias.shape != (K, R) or not lA.is_contiguous() \
                     or not lbias.is_contiguous():
                 raise ValueError("planes must be contiguous (K, R, n) and (K, R)")
-        if pL.shape != (K, self.n) or pU.shape != (K, self.n) or plb.shape != (K, self.S):
-            raise ValueError("parents must be (K, n) boxes with (K, S) bounds")
+        if rows is not None:
+            if rows.shape != (K,) or rows.dtype != torch.int64 or pL.shape[1] != self.n:
+                raise ValueError("rows must be (K,) int64 over (N, n) boxes")
+            if self.hw + B > self.phys and pL.data_ptr() == self._L.data_ptr():
+                raise RuntimeError("an append from the frontier's own rows would compact them")
+        elif pL.shape != (K, self.n) or pU.shape != (K, self.n):
+            raise ValueError("parents must be (K, n) boxes")
+        if plb.shape != (K, self.S):
+            raise ValueError("parent bounds must be (K, S)")
         if self.count + B > self.cap:
             raise RuntimeError("frontier capacity exceeded")
         self._room(B)
         self._last_B = B
         ws = self.workspace(B, lib().gh_split_append_workspace_bytes(K))
         check(lib().gh_split_append_f64(K, self.n, self.S, R, ptr(pL.contiguous()),
-                                        ptr(pU.contiguous()), ptr(plb.contiguous()), ptr(lA),
+                                        ptr(pU.contiguous()), ptr(rows), ptr(plb.contiguous()),
+                                        ptr(lA),
                                         ptr(lbias), ptr(thr.contiguous()), ptr(self._L),
                                         ptr(self._U), ptr(self._lb), self.hw, self.phys,
                                         ptr(self.order[self.count:self.count + B]), self.dead,

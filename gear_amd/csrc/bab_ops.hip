@@ -895,7 +895,8 @@ __device__ __forceinline__ void split_block_counts(int nw, const int* kc0, const
 // Any n and S: one wave per parent, 16 waves per workgroup.
 __global__ void __launch_bounds__(1024)
 split_keep_wave_kernel(int K, int n, int S, int R, const double *__restrict__ pL,
-                       const double *__restrict__ pU, const double *__restrict__ plb,
+                       const double *__restrict__ pU, const int64_t *__restrict__ prows,
+                  const double *__restrict__ plb,
                        const double *__restrict__ lA, const double *__restrict__ lbias,
                        const double *__restrict__ thr, double *__restrict__ ckey,
                        int32_t *__restrict__ sidx, int32_t *__restrict__ keep,
@@ -908,7 +909,8 @@ split_keep_wave_kernel(int K, int n, int S, int R, const double *__restrict__ pL
     int kept[2] = {0, 0};
     double vt = INFINITY;
     if (p < K) {
-        const double *l = pL + (size_t)p * n, *u = pU + (size_t)p * n;
+        const size_t br = prows ? (size_t)prows[p] : (size_t)p;   // the parent's box row
+        const double *l = pL + br * n, *u = pU + br * n;
         const double *prow = plb + (size_t)p * S;
         const double *a = lA ? lA + (size_t)p * R * n : nullptr;
         const double b = lA ? lbias[(size_t)p * R] : 0.0;
@@ -952,7 +954,8 @@ __device__ __forceinline__ bool group_any(bool x, int lane)
 // any reduction order finds it.
 __global__ void __launch_bounds__(256)
 split_keep16_kernel(int K, int n, int S, int R, const double *__restrict__ pL,
-                    const double *__restrict__ pU, const double *__restrict__ plb,
+                    const double *__restrict__ pU, const int64_t *__restrict__ prows,
+                  const double *__restrict__ plb,
                     const double *__restrict__ lA, const double *__restrict__ lbias,
                     const double *__restrict__ thr, double *__restrict__ ckey,
                     int32_t *__restrict__ sidx, int32_t *__restrict__ keep,
@@ -970,7 +973,8 @@ split_keep16_kernel(int K, int n, int S, int R, const double *__restrict__ pL,
 #pragma unroll
     for (int k = 0; k < 4; ++k) li[k] = ui[k] = ai[k] = pr[k] = th[k] = 0.0;
     if (valid) {
-        const double *l = pL + (size_t)p * n, *u = pU + (size_t)p * n;
+        const size_t br = prows ? (size_t)prows[p] : (size_t)p;   // the parent's box row
+        const double *l = pL + br * n, *u = pU + br * n;
         const double *prow = plb + (size_t)p * S;
         const double *a = planes ? lA + (size_t)p * R * n : nullptr;
 #pragma unroll
@@ -1084,7 +1088,8 @@ split_keep16_kernel(int K, int n, int S, int R, const double *__restrict__ pL,
 // the first store.
 __global__ void __launch_bounds__(1024)
 split_copy_kernel(int K, int n, int S, const double *__restrict__ pL,
-                  const double *__restrict__ pU, const double *__restrict__ plb,
+                  const double *__restrict__ pU, const int64_t *__restrict__ prows,
+                  const double *__restrict__ plb,
                   const double *__restrict__ ckey, const int32_t *__restrict__ sidx,
                   const int32_t *__restrict__ keep, const int32_t *__restrict__ offs,
                   const int64_t *__restrict__ total, double *__restrict__ FL,
@@ -1102,7 +1107,8 @@ split_copy_kernel(int K, int n, int S, const double *__restrict__ pL,
     const int idx = sidx[p];
     const int64_t T = *total;
     const double key0 = ckey[p], key1 = ckey[(size_t)p + K];
-    const double *l = pL + (size_t)p * n, *u = pU + (size_t)p * n;
+    const size_t br = prows ? (size_t)prows[p] : (size_t)p;   // the parent's box row
+    const double *l = pL + br * n, *u = pU + br * n;
     const double *prow = plb + (size_t)p * S;
     const double li = lane < n ? l[lane] : 0.0, ui = lane < n ? u[lane] : 0.0;
     const double pb = lane < S ? prow[lane] : 0.0;
@@ -1170,10 +1176,11 @@ extern "C" size_t gh_split_append_workspace_bytes(int K)
 }
 
 extern "C" int gh_split_append_f64(int K, int n, int S, int R, const double *pL, const double *pU,
-                                   const double *plb, const double *lA, const double *lbias,
-                                   const double *thr, double *FL, double *FU, double *Flb,
-                                   int64_t base, int64_t cap, int64_t *order_out, int64_t dead,
-                                   int64_t *total, double *disc_min, void *ws, hipStream_t stream)
+                                   const int64_t *prows, const double *plb, const double *lA,
+                                   const double *lbias, const double *thr, double *FL, double *FU,
+                                   double *Flb, int64_t base, int64_t cap, int64_t *order_out,
+                                   int64_t dead, int64_t *total, double *disc_min, void *ws,
+                                   hipStream_t stream)
 {
     if (K < 0 || n < 1 || S < 1 || base < 0 || !total || !disc_min) return GH_EINVAL;
     if (lA && (R < 1 || R > S || !lbias)) return GH_EINVAL;
@@ -1190,12 +1197,12 @@ extern "C" int gh_split_append_f64(int K, int n, int S, int R, const double *pL,
     int32_t *cnt = keep + 2 * (size_t)K;
     if (n <= 64 && S <= 64)
         hipLaunchKernelGGL(split_keep16_kernel, dim3(nblk), dim3(256), 0, stream, K, n, S, R, pL,
-                           pU, plb, lA, lbias, thr, ckey, sidx, keep, cnt, disc_min);
+                           pU, prows, plb, lA, lbias, thr, ckey, sidx, keep, cnt, disc_min);
     else
         hipLaunchKernelGGL(split_keep_wave_kernel, dim3(nblk), dim3(1024), 0, stream, K, n, S, R,
-                           pL, pU, plb, lA, lbias, thr, ckey, sidx, keep, cnt, disc_min);
+                           pL, pU, prows, plb, lA, lbias, thr, ckey, sidx, keep, cnt, disc_min);
     hipLaunchKernelGGL(filter_scan_kernel, dim3(1), dim3(1024), 0, stream, 2 * nblk, cnt, total);
     hipLaunchKernelGGL(split_copy_kernel, dim3(nblk), dim3(1024), 0, stream, K, n, S, pL, pU,
-                       plb, ckey, sidx, keep, cnt, total, FL, FU, Flb, base, order_out, dead);
+                       prows, plb, ckey, sidx, keep, cnt, total, FL, FU, Flb, base, order_out, dead);
     return hipGetLastError() == hipSuccess ? GH_OK : GH_ELAUNCH;
 }

@@ -297,8 +297,8 @@ template <int KT, int W>
 __global__ __launch_bounds__(W * 64) __attribute__((amdgpu_waves_per_eu(2))) void qcqp_crown_f64_kernel(
     int B, int n, int S, const double* __restrict__ QB, const double* __restrict__ QA,
     const double* __restrict__ Cm, const double* __restrict__ d, const double* __restrict__ xL,
-    const double* __restrict__ xU, double* __restrict__ lb, double* __restrict__ lA,
-    double* __restrict__ lbias, int PR, QcqpConsts k)
+    const double* __restrict__ xU, const int64_t* __restrict__ rows, double* __restrict__ lb,
+    double* __restrict__ lA, double* __restrict__ lbias, int PR, QcqpConsts k)
 {
     constexpr int NCH = (KT + 3) / 4;
     constexpr int NM = KT;
@@ -311,12 +311,14 @@ __global__ __launch_bounds__(W * 64) __attribute__((amdgpu_waves_per_eu(2))) voi
     const int bo = min(box, B - 1);      // rows past B compute on box B-1, never stored
     const bool live = box < B;
 
+    // the box's row of xL / xU: rows[bo] when an index is given
+    const size_t xr = rows ? (size_t)rows[bo] : (size_t)bo;
     BoxVals<KT> bx;
 #pragma unroll
     for (int m = 0; m < NM; ++m) {
         int i = lq + 4 * m;
-        bx.xl[m] = (i < n) ? xL[(size_t)bo * n + i] : 0.0;
-        bx.xu[m] = (i < n) ? xU[(size_t)bo * n + i] : 0.0;
+        bx.xl[m] = (i < n) ? xL[xr * n + i] : 0.0;
+        bx.xu[m] = (i < n) ? xU[xr * n + i] : 0.0;
     }
 
     const size_t rowstride = (size_t)NCH * KT * 64;
@@ -360,8 +362,9 @@ constexpr int kStagedWaves = 8;
 
 template <int KT>
 int launch_f64(int B, int n, int S, const double* QB, const double* QA, const double* Cm,
-               const double* d, const double* xL, const double* xU, double* lb, double* lA,
-               double* lbias, int PR, const QcqpConsts& k, hipStream_t stream)
+               const double* d, const double* xL, const double* xU, const int64_t* rows,
+               double* lb, double* lA, double* lbias, int PR, const QcqpConsts& k,
+               hipStream_t stream)
 {
     if (B >= kStagedMinBoxes) {
         constexpr int W = kStagedWaves;
@@ -377,11 +380,11 @@ int launch_f64(int B, int n, int S, const double* QB, const double* QA, const do
         }
         dim3 grid((B + 16 * W - 1) / (16 * W));
         hipLaunchKernelGGL((qcqp_crown_f64_kernel<KT, W>), grid, dim3(64 * W), lds, stream, B, n,
-                           S, QB, QA, Cm, d, xL, xU, lb, lA, lbias, PR, k);
+                           S, QB, QA, Cm, d, xL, xU, rows, lb, lA, lbias, PR, k);
     } else {
         dim3 grid((B + 15) / 16);
         hipLaunchKernelGGL((qcqp_crown_f64_kernel<KT, 1>), grid, dim3(64), 0, stream, B, n, S,
-                           QB, QA, Cm, d, xL, xU, lb, lA, lbias, PR, k);
+                           QB, QA, Cm, d, xL, xU, rows, lb, lA, lbias, PR, k);
     }
     return hipGetLastError() == hipSuccess ? GH_OK : GH_ELAUNCH;
 }
@@ -409,9 +412,11 @@ extern "C" int gh_qcqp_pack_f64(int n, int S, const double* Q, void* packed, hip
     return hipGetLastError() == hipSuccess ? GH_OK : GH_ELAUNCH;
 }
 
-extern "C" int gh_qcqp_crown_f64(int B, int n, int S, const void* packed, const double* Cm,
-                                 const double* d, const double* xL, const double* xU, double* lb,
-                                 double* lA, double* lbias, int plane_rows, hipStream_t stream)
+namespace {
+
+int crown_f64(int B, int n, int S, const void* packed, const double* Cm, const double* d,
+              const double* xL, const double* xU, const int64_t* rows, double* lb, double* lA,
+              double* lbias, int plane_rows, hipStream_t stream)
 {
     if (B < 0 || n <= 0 || S <= 0) return GH_EINVAL;
     if ((lA || lbias) && (plane_rows < 1 || plane_rows > S)) return GH_EINVAL;
@@ -430,12 +435,30 @@ extern "C" int gh_qcqp_crown_f64(int B, int n, int S, const void* packed, const 
     const double* QB = (const double*)packed;
     const double* QA = QB + (size_t)S * nch * kt * 64;
 #define GH_KT(K) \
-    case K: return launch_f64<K>(B, n, S, QB, QA, Cm, d, xL, xU, lb, lA, lbias, plane_rows, k, \
-                                 stream);
+    case K: return launch_f64<K>(B, n, S, QB, QA, Cm, d, xL, xU, rows, lb, lA, lbias, \
+                                 plane_rows, k, stream);
     switch (kt) {
         GH_KT(1) GH_KT(2) GH_KT(3) GH_KT(4) GH_KT(5) GH_KT(6) GH_KT(7) GH_KT(8)
         GH_KT(9) GH_KT(10) GH_KT(11) GH_KT(12) GH_KT(13) GH_KT(14) GH_KT(15) GH_KT(16)
     default: return GH_EUNSUPPORTED;
     }
 #undef GH_KT
+}
+
+}  // namespace
+
+extern "C" int gh_qcqp_crown_f64(int B, int n, int S, const void* packed, const double* Cm,
+                                 const double* d, const double* xL, const double* xU, double* lb,
+                                 double* lA, double* lbias, int plane_rows, hipStream_t stream)
+{
+    return crown_f64(B, n, S, packed, Cm, d, xL, xU, nullptr, lb, lA, lbias, plane_rows, stream);
+}
+
+extern "C" int gh_qcqp_crown_rows_f64(int B, int n, int S, const void* packed, const double* Cm,
+                                      const double* d, const double* xL, const double* xU,
+                                      const int64_t* rows, double* lb, double* lA, double* lbias,
+                                      int plane_rows, hipStream_t stream)
+{
+    if (B > 0 && !rows) return GH_EINVAL;
+    return crown_f64(B, n, S, packed, Cm, d, xL, xU, rows, lb, lA, lbias, plane_rows, stream);
 }

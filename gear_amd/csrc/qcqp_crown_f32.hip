@@ -114,9 +114,9 @@ template <int KT, bool STAGED>
 __global__ __launch_bounds__(STAGED ? 512 : 64) __attribute__((amdgpu_waves_per_eu(2))) void qcqp_crown_f32s_kernel(
     int B, int n, int S, const float* __restrict__ QB, const float* __restrict__ QA,
     const double* __restrict__ Q64, const double* __restrict__ Cm, const double* __restrict__ d,
-    const double* __restrict__ xL, const double* __restrict__ xU, double* __restrict__ lb,
-    int* __restrict__ flag, double* __restrict__ lA, double* __restrict__ lbias, int PR,
-    F32Consts k)
+    const double* __restrict__ xL, const double* __restrict__ xU,
+    const int64_t* __restrict__ rows, double* __restrict__ lb, int* __restrict__ flag,
+    double* __restrict__ lA, double* __restrict__ lbias, int PR, F32Consts k)
 {
     constexpr int NCH = (KT + 3) / 4;
     constexpr int NM = KT;
@@ -129,13 +129,15 @@ __global__ __launch_bounds__(STAGED ? 512 : 64) __attribute__((amdgpu_waves_per_
     const int box = (blockIdx.x * W + wv) * 16 + lr;
     const int bo = min(box, B - 1);
     const bool live = box < B;
+    // the box's row of xL / xU: rows[bo] when an index is given
+    const size_t xr = rows ? (size_t)rows[bo] : (size_t)bo;
 
     float xl[NM], xu[NM];
 #pragma unroll
     for (int m = 0; m < NM; ++m) {
         int i = lq + 4 * m;
-        xl[m] = (i < n) ? down_f32(xL[(size_t)bo * n + i]) : 0.0f;
-        xu[m] = (i < n) ? up_f32(xU[(size_t)bo * n + i]) : 0.0f;
+        xl[m] = (i < n) ? down_f32(xL[xr * n + i]) : 0.0f;
+        xu[m] = (i < n) ? up_f32(xU[xr * n + i]) : 0.0f;
     }
     int ambiguous = 0;
 
@@ -239,7 +241,7 @@ __global__ __launch_bounds__(STAGED ? 512 : 64) __attribute__((amdgpu_waves_per_
                                 // near a tie: take the FP64 path's choice
                                 ambiguous = 1;
                                 r1 = gh::fp64_choice(n, Q64 + ((size_t)s * n + j) * n,
-                                                 xL + (size_t)bo * n, xU + (size_t)bo * n, j,
+                                                 xL + xr * n, xU + xr * n, j,
                                                  k.kn64);
                             }
                             float al = r1 ? yl : yu;
@@ -329,9 +331,9 @@ __global__ __launch_bounds__(STAGED ? 512 : 64) __attribute__((amdgpu_waves_per_
 
 template <int KT>
 int launch_f32(int B, int n, int S, const float* QB, const float* QA, const double* Q64,
-               const double* Cm, const double* d, const double* xL, const double* xU, double* lb,
-               int* flag, double* lA, double* lbias, int PR, const F32Consts& k,
-               hipStream_t stream)
+               const double* Cm, const double* d, const double* xL, const double* xU,
+               const int64_t* rows, double* lb, int* flag, double* lA, double* lbias, int PR,
+               const F32Consts& k, hipStream_t stream)
 {
     if (B >= 16384) {   // the LDS-staged form, 128 boxes per workgroup
         const size_t rowstride = (size_t)((KT + 3) / 4) * KT * 64;
@@ -345,12 +347,12 @@ int launch_f32(int B, int n, int S, const float* QB, const float* QA, const doub
             attr = true;
         }
         hipLaunchKernelGGL((qcqp_crown_f32s_kernel<KT, true>), dim3((B + 127) / 128), dim3(512),
-                           lds, stream, B, n, S, QB, QA, Q64, Cm, d, xL, xU, lb, flag, lA, lbias,
-                           PR, k);
+                           lds, stream, B, n, S, QB, QA, Q64, Cm, d, xL, xU, rows, lb, flag, lA,
+                           lbias, PR, k);
     } else {
         hipLaunchKernelGGL((qcqp_crown_f32s_kernel<KT, false>), dim3((B + 15) / 16), dim3(64), 0,
-                           stream, B, n, S, QB, QA, Q64, Cm, d, xL, xU, lb, flag, lA, lbias, PR,
-                           k);
+                           stream, B, n, S, QB, QA, Q64, Cm, d, xL, xU, rows, lb, flag, lA, lbias,
+                           PR, k);
     }
     return hipGetLastError() == hipSuccess ? GH_OK : GH_ELAUNCH;
 }
@@ -378,10 +380,12 @@ extern "C" int gh_qcqp_pack_f32(int n, int S, const double* Q, void* packed, hip
     return hipGetLastError() == hipSuccess ? GH_OK : GH_ELAUNCH;
 }
 
-extern "C" int gh_qcqp_crown_f32s(int B, int n, int S, const void* packed, const double* Q,
-                                  const double* Cm, const double* d, const double* xL,
-                                  const double* xU, double* lb, int32_t* flag, double* lA,
-                                  double* lbias, int plane_rows, hipStream_t stream)
+namespace {
+
+int crown_f32s(int B, int n, int S, const void* packed, const double* Q, const double* Cm,
+               const double* d, const double* xL, const double* xU, const int64_t* rows,
+               double* lb, int32_t* flag, double* lA, double* lbias, int plane_rows,
+               hipStream_t stream)
 {
     if (B < 0 || n <= 0 || S <= 0) return GH_EINVAL;
     if ((lA || lbias) && (plane_rows < 1 || plane_rows > S)) return GH_EINVAL;
@@ -407,12 +411,34 @@ extern "C" int gh_qcqp_crown_f32s(int B, int n, int S, const void* packed, const
     const float* QB = (const float*)packed;
     const float* QA = QB + (size_t)S * nch * kt * 64;
 #define GH_KT(K) \
-    case K: return launch_f32<K>(B, n, S, QB, QA, Q, Cm, d, xL, xU, lb, flag, lA, lbias, \
-                                 plane_rows, k, stream);
+    case K: return launch_f32<K>(B, n, S, QB, QA, Q, Cm, d, xL, xU, rows, lb, flag, lA, \
+                                 lbias, plane_rows, k, stream);
     switch (kt) {
         GH_KT(1) GH_KT(2) GH_KT(3) GH_KT(4) GH_KT(5) GH_KT(6) GH_KT(7) GH_KT(8)
         GH_KT(9) GH_KT(10) GH_KT(11) GH_KT(12) GH_KT(13) GH_KT(14) GH_KT(15) GH_KT(16)
     default: return GH_EUNSUPPORTED;
     }
 #undef GH_KT
+}
+
+}  // namespace
+
+extern "C" int gh_qcqp_crown_f32s(int B, int n, int S, const void* packed, const double* Q,
+                                  const double* Cm, const double* d, const double* xL,
+                                  const double* xU, double* lb, int32_t* flag, double* lA,
+                                  double* lbias, int plane_rows, hipStream_t stream)
+{
+    return crown_f32s(B, n, S, packed, Q, Cm, d, xL, xU, nullptr, lb, flag, lA, lbias,
+                      plane_rows, stream);
+}
+
+extern "C" int gh_qcqp_crown_rows_f32s(int B, int n, int S, const void* packed, const double* Q,
+                                       const double* Cm, const double* d, const double* xL,
+                                       const double* xU, const int64_t* rows, double* lb,
+                                       int32_t* flag, double* lA, double* lbias, int plane_rows,
+                                       hipStream_t stream)
+{
+    if (B > 0 && !rows) return GH_EINVAL;
+    return crown_f32s(B, n, S, packed, Q, Cm, d, xL, xU, rows, lb, flag, lA, lbias, plane_rows,
+                      stream);
 }

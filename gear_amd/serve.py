@@ -254,9 +254,11 @@ class Session:
         returns the domains picked."""
         nb = min(2 * self.batch, len(f), f.cap - len(f))
         if nb > 0:
+            # the library bound takes a row index; a given bound_fn may not
             reordered_step(f, self.bound, self.thr, nb, self.p.n, self.p.S, planes=self.planes,
                            lp=self._raise_by_lp if self._lp() else None, clip=self.clip,
-                           clip_max_rows=self.cfg.clip_max_rows, sync=False)
+                           clip_max_rows=self.cfg.clip_max_rows, sync=False,
+                           in_place=self.bound_fn is None and self.planes)
             self._bounded = nb
         return nb
 

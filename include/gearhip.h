@@ -70,6 +70,14 @@ int gh_qcqp_crown_f64(int B, int n, int S, const void *packed,
                       const double *Cm, const double *d, const double *xL,
                       const double *xU, double *lb, double *lA, double *lbias,
                       int plane_rows, hipStream_t stream);
+/* gh_qcqp_crown_f64 on the boxes rows[b] (b < B, int64) of xL, xU: a
+ * caller's picked domains bound where its storage holds them, without
+ * gathering them first (pick_out_batch, branching_domains.py:831-962,
+ * then the bound).  lb, lA, lbias are indexed by b as above. */
+int gh_qcqp_crown_rows_f64(int B, int n, int S, const void *packed, const double *Cm,
+                           const double *d, const double *xL, const double *xU,
+                           const int64_t *rows, double *lb, double *lA, double *lbias,
+                           int plane_rows, hipStream_t stream);
 
 /*
  * FP32-sound form of gh_qcqp_crown_f64: the same relaxation evaluated in
@@ -98,6 +106,12 @@ int gh_qcqp_crown_f32s(int B, int n, int S, const void *packed,
                        const double *xL, const double *xU, double *lb,
                        int32_t *flag, double *lA, double *lbias, int plane_rows,
                        hipStream_t stream);
+/* gh_qcqp_crown_f32s on the boxes rows[b] of xL, xU (as
+ * gh_qcqp_crown_rows_f64). */
+int gh_qcqp_crown_rows_f32s(int B, int n, int S, const void *packed, const double *Q,
+                            const double *Cm, const double *d, const double *xL,
+                            const double *xU, const int64_t *rows, double *lb, int32_t *flag,
+                            double *lA, double *lbias, int plane_rows, hipStream_t stream);
 
 /*
  * FP32-sound CROWN lower bound of a QCQP of any size.  Same contract and
@@ -328,12 +342,16 @@ size_t gh_split_append_workspace_bytes(int K);
  * 2K children (base, cap, total, disc_min as there), then
  * gh_fill_order_i64(2K, total, base, dead, order_out), without writing
  * the children anywhere but the frontier: each survivor's box is rebuilt
- * from its parent.  ws: gh_split_append_workspace_bytes(K) bytes.  The
- * caller guarantees base + 2K <= cap.  Frontiers whose domains carry a
+ * from its parent.  Parent p's box is row prows[p] of pL, pU (row p when
+ * prows is NULL), so the parents can be read in place in the frontier as
+ * long as the append does not overwrite them; plb, lA, lbias are indexed
+ * by p.  ws: gh_split_append_workspace_bytes(K) bytes.  The caller
+ * guarantees base + 2K <= cap.  Frontiers whose domains carry a
  * payload use the separate calls.
  */
 int gh_split_append_f64(int K, int n, int S, int R, const double *pL, const double *pU,
-                        const double *plb, const double *lA, const double *lbias,
+                        const int64_t *prows, const double *plb, const double *lA,
+                        const double *lbias,
                         const double *thr, double *FL, double *FU, double *Flb, int64_t base,
                         int64_t cap, int64_t *order_out, int64_t dead, int64_t *total,
                         double *disc_min, void *ws, hipStream_t stream);

@@ -336,6 +336,42 @@ def test_split_append_equals_separate_calls(cuda, K, n, m, planes):
     assert a.compact_order() == b.compact_order() == 13 + kept
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32s"])
+def test_bound_rows_equals_gathered(cuda, precision):
+    """Bounding boxes in place through a row index (gh_qcqp_crown_rows_*)
+    gives bit for bit the bounds and planes of bounding the gathered boxes,
+    and split_append from the index the frontier of split_append from the
+    gathered parents."""
+    from gear_amd.crown import BoundedQCQP, DeviceFrontier
+    p = qcqp_dense(50, 20, seed=3)
+    L, U = random_bisection_boxes(p.xl, p.xu, 20000, seed=4)
+    net = BoundedQCQP(p, cuda, precision=precision)
+    rng = np.random.default_rng(5)
+    for B in (17000, 300):        # the staged and the unstaged kernel
+        idx = torch.as_tensor(rng.choice(len(L), B, replace=False), dtype=torch.int64,
+                              device=cuda)
+        dL, dU = _dev(L, cuda), _dev(U, cuda)
+        a = net.bound(dL, dU, return_A=True, plane_rows=2, rows=idx)
+        b = net.bound(dL[idx], dU[idx], return_A=True, plane_rows=2)
+        torch.cuda.synchronize()
+        for x, y in zip(a, b):
+            assert _bits_equal(x.cpu().numpy(), y.cpu().numpy()), (precision, B)
+        thr = torch.full((p.S,), float("inf"), dtype=torch.float64, device=cuda)
+        thr[0] = float(torch.quantile(a[0][:, 0], 0.3))
+        fr = []
+        for rows in (idx, None):
+            f = DeviceFrontier(p.n, p.S, 2 * B + 8, device=cuda)
+            if rows is None:
+                f.split_append(a[0], dL[idx], dU[idx], thr, a[1][:, :1].contiguous(),
+                               a[2][:, :1].contiguous())
+            else:
+                f.split_append(a[0], dL, dU, thr, a[1][:, :1].contiguous(),
+                               a[2][:, :1].contiguous(), rows=rows)
+            fr.append(f.rows())
+        for x, y in zip(*fr):
+            assert _bits_equal(x.cpu().numpy(), y.cpu().numpy()), (precision, B)
+
+
 def test_unsynced_steps_equal_synced(cuda):
     """Steps issued without reading the survivor count (dead slots for the
     pruned children) leave, after compact_order, the same frontier and
