@@ -354,16 +354,38 @@ constexpr int kSortTileLog = 10;
 // slot order[i]): a bitonic network in LDS on (~desc_key, position)
 // ascending -- descending key, ties in input order; positions past N pad
 // the tile after every key.  Out: the tile's keys and slot ids in sorted
-// order.
+// order.  Thread t handles pairs t and t + 256 of each step; with a stride
+// of at most 64 the pairs of wave w lie in its own segments [128 w, +128)
+// and [512 + 128 w, +128), so those steps need no workgroup barrier (the
+// wave's LDS operations are ordered), only the steps of larger strides.
+__device__ __forceinline__ void sort_pair(unsigned long long* sk, int* si, int j, int stride,
+                                          int size)
+{
+    const int a = 2 * j - (j & (stride - 1)), b = a + stride;
+    const unsigned long long ka = sk[a], kb = sk[b];
+    const int ia = si[a], ib = si[b];
+    const bool a_first = ka < kb || (ka == kb && ia < ib);
+    if (a_first != ((a & size) == 0)) {
+        sk[a] = kb;
+        sk[b] = ka;
+        si[a] = ib;
+        si[b] = ia;
+    }
+}
+
 __global__ __launch_bounds__(kSortThreads) void sort_tile_kernel(
     int64_t N, const double* __restrict__ keys, const int64_t* __restrict__ order,
     double* __restrict__ kOut, int64_t* __restrict__ vOut)
 {
+    static_assert(kSortTile == 4 * kSortThreads && kSortThreads == 256, "pair layout");
     __shared__ unsigned long long sk[kSortTile];
     __shared__ int si[kSortTile];
     const int64_t t0 = (int64_t)blockIdx.x * kSortTile;
     const int cnt = (int)min((int64_t)kSortTile, N - t0);
-    for (int j = threadIdx.x; j < kSortTile; j += kSortThreads) {
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int j = t + k * kSortThreads;
         // desc_key is never 0 on a key, so a real rank is below the padding's
         sk[j] = j < cnt ? ~desc_key(keys[t0 + j]) : ~0ull;
         si[j] = j;
@@ -371,47 +393,37 @@ __global__ __launch_bounds__(kSortThreads) void sort_tile_kernel(
     __syncthreads();
     for (int size = 2; size <= kSortTile; size <<= 1) {
         for (int stride = size >> 1; stride > 0; stride >>= 1) {
-            for (int j = threadIdx.x; j < kSortTile / 2; j += kSortThreads) {
-                const int a = 2 * j - (j & (stride - 1)), b = a + stride;
-                const unsigned long long ka = sk[a], kb = sk[b];
-                const int ia = si[a], ib = si[b];
-                const bool a_first = ka < kb || (ka == kb && ia < ib);
-                if (a_first != ((a & size) == 0)) {
-                    sk[a] = kb;
-                    sk[b] = ka;
-                    si[a] = ib;
-                    si[b] = ia;
-                }
+            const bool wide = stride > 64;
+            if (wide) __syncthreads();
+            sort_pair(sk, si, t, stride, size);
+            sort_pair(sk, si, t + kSortThreads, stride, size);
+            if (wide) {
+                __syncthreads();
+            } else {
+                // the next step's LDS reads stay after this step's writes
+                __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+                __builtin_amdgcn_wave_barrier();
             }
-            __syncthreads();
         }
     }
-    for (int j = threadIdx.x; j < cnt; j += kSortThreads) {
-        kOut[t0 + j] = from_desc_key(~sk[j]);
-        vOut[t0 + j] = order[t0 + si[j]];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int j = t + k * kSortThreads;
+        if (j < cnt) {
+            kOut[t0 + j] = from_desc_key(~sk[j]);
+            vOut[t0 + j] = order[t0 + si[j]];
+        }
     }
-}
-
-// Rank of key x in the descending run [lo, hi): the number of its elements
-// strictly before x (tied elements after x), or, with ties_first, before
-// or tied with x.
-__device__ __forceinline__ int64_t run_rank(const double* __restrict__ k, int64_t lo, int64_t hi,
-                                            unsigned long long x, bool ties_first)
-{
-    const int64_t start = lo;
-    while (lo < hi) {
-        const int64_t mid = (lo + hi) >> 1;
-        const unsigned long long y = desc_key(k[mid]);
-        if (y > x || (ties_first && y == x)) lo = mid + 1;
-        else hi = mid;
-    }
-    return lo - start;
 }
 
 // One pass of the stable merge sort: runs of 2^lg sorted descending, runs
 // 4g .. 4g + 3 merged into one (ties: the earlier run first).  Each element
 // goes to its group's start + its index in its run + its rank in each
-// other run of the group (ties first for the runs before its own).
+// other run of the group: the length of the run's prefix whose keys come
+// before it (strictly, or tied as well for the runs before its own).  The
+// three ranks are found by one interleaved binary search over prefix
+// lengths, so their loads are in flight together.
 __global__ void merge4_pass_kernel(int64_t N, int lg, const double* __restrict__ kIn,
                                    const int64_t* __restrict__ vIn, double* __restrict__ kOut,
                                    int64_t* __restrict__ vOut)
@@ -423,16 +435,27 @@ __global__ void merge4_pass_kernel(int64_t N, int lg, const double* __restrict__
     const int q = (int)(r & 3);
     const int64_t g0 = (r - q) << lg;      // the group's start
     const unsigned long long x = desc_key(kIn[t]);
-    int64_t pos = t - (r << lg);
+    int64_t lo[3], len[3], pos[3] = {0, 0, 0};
+    bool tie[3];
 #pragma unroll
-    for (int o = 0; o < 4; ++o) {
-        if (o == q) continue;
-        const int64_t lo = g0 + o * run;
-        if (lo >= N) continue;
-        pos += run_rank(kIn, lo, min(lo + run, N), x, o < q);
+    for (int u = 0; u < 3; ++u) {
+        const int o = u < q ? u : u + 1;    // the other runs, in order
+        lo[u] = g0 + o * run;
+        len[u] = lo[u] < N ? min(run, N - lo[u]) : 0;
+        tie[u] = o < q;
     }
-    kOut[g0 + pos] = kIn[t];
-    vOut[g0 + pos] = vIn[t];
+    for (int64_t step = run; step > 0; step >>= 1) {
+#pragma unroll
+        for (int u = 0; u < 3; ++u) {
+            if (pos[u] + step <= len[u]) {
+                const unsigned long long y = desc_key(kIn[lo[u] + pos[u] + step - 1]);
+                if (y > x || (tie[u] && y == x)) pos[u] += step;
+            }
+        }
+    }
+    const int64_t dst = g0 + (t - (r << lg)) + pos[0] + pos[1] + pos[2];
+    kOut[dst] = kIn[t];
+    vOut[dst] = vIn[t];
 }
 
 // Slot ids of the children just appended: child t < *total (a survivor)
