@@ -46,6 +46,22 @@ __device__ __forceinline__ d4 mfma16(double a, double b, d4 c)
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 
+// D = C + A * B on four 4x4x4 FP64 blocks.  Block b: A_b[i][k] in lane
+// 16k + 4b + i, B_b[k][j] in lane 16k + 4b + j, D_b[i][j] in lane
+// 16i + 4b + j -- so with A replicated over the four blocks (lane 16k + 4b + i
+// holding row i) lane l gets D[l / 16][l % 16], the 16x16x4 layout's element
+// v = 0, with the lane's own B operand.  Each output is the FMA chain over
+// its 4 k in order.
+__device__ __forceinline__ double mfma4(double a, double b, double c)
+{
+    return __builtin_amdgcn_mfma_f64_4x4x4f64(a, b, c, 0, 0, 0);
+}
+
+// With KT = 1 (mod 4) the last 16-row tile c = NCH - 1 holds one index per
+// lane row (m = KT - 1, rows 16c .. 16c + 3): it is packed replicated over
+// the four 4x4 blocks and run on mfma4, a quarter of the 16x16x4 work.
+__host__ __device__ constexpr bool tail_tile(int kt) { return kt % 4 == 1; }
+
 // Packed Q operands (built once per problem by gh_qcqp_pack_f64):
 //   QB[s][c][t][l] = Q[s][16c + l%16][4t + l/16]   IBP B operand (Q^T tiles)
 //   QA[s][c][t][l] = Q[s][4t + l/16][16c + l%16]   backward B operand
@@ -64,9 +80,11 @@ __global__ void qcqp_pack_kernel(int n, int S, int nch, int kt, const double* __
     r -= (size_t)c * kt * 64;
     int t = (int)(r / 64), l = (int)(r - (size_t)t * 64);
     const double* Qs = Q + (size_t)s * n * n;
-    int j = 16 * c + (l & 15), i = 4 * t + (l >> 4);
+    // the tail tile's lane l carries row 16c + l % 4 (replicated per block)
+    const int row = (tail_tile(kt) && c == nch - 1) ? (l & 3) : (l & 15);
+    int j = 16 * c + row, i = 4 * t + (l >> 4);
     QB[u] = (j < n && i < n) ? Qs[(size_t)j * n + i] : 0.0;
-    int j2 = 4 * t + (l >> 4), i2 = 16 * c + (l & 15);
+    int j2 = 4 * t + (l >> 4), i2 = 16 * c + row;
     QA[u] = (j2 < n && i2 < n) ? Qs[(size_t)j2 * n + i2] : 0.0;
 }
 
@@ -92,21 +110,67 @@ struct RowState {
     double ql, qu, gsum, own;
 };
 
+// The plane work of index m (j = q + 4m) of box r from the IBP products
+// cc, dv, mg: the interval of y_j, the interval of z_j and the
+// coeff-aware McCormick lower plane; partial sums in the lane's class v.
+template <int KT>
+__device__ __forceinline__ void plane_item(int n, int m, int j, double cc, double dv, double mg,
+                                           const double* cs, const BoxVals<KT>& bx,
+                                           const QcqpConsts& k, RowState<KT>& st, double& p_ql,
+                                           double& p_qu, double& p_g, double& p_own)
+{
+    st.acm[m] = 0.0;
+    st.beta[m] = 0.0;
+    if (j < n) {
+        double e = mg * k.kn;
+        double yl = gh::widen((cc - dv) + 0.0, e, -1.0);
+        double yu = gh::widen((cc + dv) + 0.0, e, 1.0);
+        double xlj = bx.xl[m], xuj = bx.xu[m];
+        // interval of z_j (bivariate.py:563-597)
+        double r0 = xlj * yl, r1 = xlj * yu, r2 = xuj * yl, r3 = xuj * yu;
+        p_ql = p_ql + fmin(fmin(r0, r1), fmin(r2, r3));
+        p_qu = p_qu + fmax(fmax(r0, r1), fmax(r2, r3));
+        // coeff-aware McCormick lower plane (bivariate.py:190-261, 40-75)
+        double a1 = yl, b1 = xlj, c1 = (-xlj) * yl;
+        double a2 = yu, b2 = xuj, c2 = (-xuj) * yu;
+        double m1 = a1 * (a1 >= 0 ? xlj : xuj);
+        m1 = m1 + b1 * (b1 >= 0 ? yl : yu);
+        m1 = m1 + c1;
+        double m2 = a2 * (a2 >= 0 ? xlj : xuj);
+        m2 = m2 + b2 * (b2 >= 0 ? yl : yu);
+        m2 = m2 + c2;
+        double rr = (m1 >= m2) ? 1.0 : 0.0;
+        double t = (yl - yu) * rr;
+        double al = t + yu;
+        t = (xlj - xuj) * rr;
+        double be = t + xuj;
+        double pp = yu * xuj, qq = yl * xlj;
+        t = (pp - qq) * rr;
+        double ga = t - pp;
+        st.acm[m] = cs[j] + al;
+        st.beta[m] = be;
+        p_g = p_g + ga;
+        p_own = fma(fabs(be), k.hsn * mg, p_own);
+    }
+}
+
 // Forward half of row s.  qb points at the row's packed IBP tiles (+ lane);
-// G IBP tiles are accumulated at a time.
+// G IBP tiles are accumulated at a time; with a tail tile (tail_tile(KT))
+// the last tile runs on mfma4.
 template <int KT, int G>
 __device__ __forceinline__ void row_forward(int n, const double* qb, const double* cs,
                                             const BoxVals<KT>& bx, int lq,
                                             const QcqpConsts& k, RowState<KT>& st)
 {
-    constexpr int NCH = (KT + 3) / 4;    // 16-wide tiles of j (and of i)
-    constexpr int NM = KT;               // per-lane indices q + 4m
+    constexpr int NCH = (KT + 3) / 4;            // 16-wide tiles of j (and of i)
+    constexpr int NM = KT;                       // per-lane indices q + 4m
+    constexpr int NR = tail_tile(KT) ? NCH - 1 : NCH;   // tiles on 16x16x4
     double p_ql[4], p_qu[4], p_g[4], p_own[4];
 #pragma unroll
     for (int v = 0; v < 4; ++v) p_ql[v] = p_qu[v] = p_g[v] = p_own[v] = 0.0;
 
 #pragma unroll
-    for (int c0 = 0; c0 < NCH; c0 += G) {
+    for (int c0 = 0; c0 < NR; c0 += G) {
         // ---- 1. IBP of y_s for tiles c0..c0+G-1: D[j][box] = sum_i
         // Q[j][i] x[box][i] on MFMA (A = Q tile, B = the lane's x at
         // index q + 4t); centre, radius and magnitude as in
@@ -120,7 +184,7 @@ __device__ __forceinline__ void row_forward(int n, const double* qb, const doubl
             const double x0 = bx.x0(t), ep = bx.ep(t), mx = bx.mx(t);
 #pragma unroll
             for (int g = 0; g < G; ++g) {
-                if (c0 + g < NCH) {
+                if (c0 + g < NR) {
                     double q = qb[((size_t)(c0 + g) * KT + t) * 64];
                     double qa_ = fabs(q);
                     Dc[g] = mfma16(q, x0, Dc[g]);
@@ -137,46 +201,26 @@ __device__ __forceinline__ void row_forward(int n, const double* qb, const doubl
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const int c = c0 + g, m = 4 * c + v;
-                if (c < NCH && m < NM) {
-                    const int j = lq + 4 * m;
-                    st.acm[m] = 0.0;
-                    st.beta[m] = 0.0;
-                    if (j < n) {
-                        double cc = Dc[g][v], dv = Dd[g][v], mg = Dm[g][v];
-                        double e = mg * k.kn;
-                        double yl = gh::widen((cc - dv) + 0.0, e, -1.0);
-                        double yu = gh::widen((cc + dv) + 0.0, e, 1.0);
-                        double xlj = bx.xl[m], xuj = bx.xu[m];
-                        // interval of z_j (bivariate.py:563-597)
-                        double r0 = xlj * yl, r1 = xlj * yu, r2 = xuj * yl, r3 = xuj * yu;
-                        p_ql[v] = p_ql[v] + fmin(fmin(r0, r1), fmin(r2, r3));
-                        p_qu[v] = p_qu[v] + fmax(fmax(r0, r1), fmax(r2, r3));
-                        // coeff-aware McCormick lower plane
-                        // (bivariate.py:190-261, 40-75)
-                        double a1 = yl, b1 = xlj, c1 = (-xlj) * yl;
-                        double a2 = yu, b2 = xuj, c2 = (-xuj) * yu;
-                        double m1 = a1 * (a1 >= 0 ? xlj : xuj);
-                        m1 = m1 + b1 * (b1 >= 0 ? yl : yu);
-                        m1 = m1 + c1;
-                        double m2 = a2 * (a2 >= 0 ? xlj : xuj);
-                        m2 = m2 + b2 * (b2 >= 0 ? yl : yu);
-                        m2 = m2 + c2;
-                        double rr = (m1 >= m2) ? 1.0 : 0.0;
-                        double t = (yl - yu) * rr;
-                        double al = t + yu;
-                        t = (xlj - xuj) * rr;
-                        double be = t + xuj;
-                        double pp = yu * xuj, qq = yl * xlj;
-                        t = (pp - qq) * rr;
-                        double ga = t - pp;
-                        st.acm[m] = cs[j] + al;
-                        st.beta[m] = be;
-                        p_g[v] = p_g[v] + ga;
-                        p_own[v] = fma(fabs(be), k.hsn * mg, p_own[v]);
-                    }
-                }
+                if (c < NR && m < NM)
+                    plane_item<KT>(n, m, lq + 4 * m, Dc[g][v], Dd[g][v], Dm[g][v], cs, bx, k, st,
+                                   p_ql[v], p_qu[v], p_g[v], p_own[v]);
             }
         }
+    }
+    if constexpr (tail_tile(KT)) {
+        // the last tile: index m = KT - 1 (j = q + 4m) only
+        double tc = 0.0, td = 0.0, tm = 0.0;
+#pragma unroll
+        for (int t = 0; t < KT; ++t) {
+            const double q = qb[((size_t)NR * KT + t) * 64];
+            const double qa_ = fabs(q);
+            tc = mfma4(q, bx.x0(t), tc);
+            td = mfma4(qa_, bx.ep(t), td);
+            tm = mfma4(qa_, bx.mx(t), tm);
+            if (t % 4 == 3) __builtin_amdgcn_sched_barrier(0);
+        }
+        plane_item<KT>(n, NM - 1, lq + 4 * (NM - 1), tc, td, tm, cs, bx, k, st, p_ql[0], p_qu[0],
+                       p_g[0], p_own[0]);
     }
     st.ql = gh::tree4x4(p_ql);
     st.qu = gh::tree4x4(p_qu);
@@ -195,20 +239,26 @@ __device__ __forceinline__ void row_backward(int n, int S, int s, const double* 
 {
     constexpr int NCH = (KT + 3) / 4;
     constexpr int NM = KT;
+    constexpr int NR = tail_tile(KT) ? NCH - 1 : NCH;   // tiles on 16x16x4
     // ---- 3. Backward product: D[i][box] = sum_j Q[j][i] beta[box][j]
-    // (A = Q^T tile c, B = the lane's beta at index q + 4t).
+    // (A = Q^T tile c, B = the lane's beta at index q + 4t); the tail tile
+    // (index m = KT - 1 only) on mfma4.
     d4 Da[NCH];
+    double Dt = 0.0;
 #pragma unroll
     for (int c = 0; c < NCH; ++c) Da[c] = (d4){0.0, 0.0, 0.0, 0.0};
 #pragma unroll
     for (int t = 0; t < KT; ++t) {
 #pragma unroll
-        for (int c = 0; c < NCH; ++c)
+        for (int c = 0; c < NR; ++c)
             Da[c] = mfma16(qa[((size_t)c * KT + t) * 64], st.beta[t], Da[c]);
+        if constexpr (tail_tile(KT))
+            Dt = mfma4(qa[((size_t)NR * KT + t) * 64], st.beta[t], Dt);
         // keep the operand loads of later steps from being hoisted here
         // (they would hold KT * NCH registers)
         if (t % 4 == 3) __builtin_amdgcn_sched_barrier(0);
     }
+    if constexpr (tail_tile(KT)) Da[NCH - 1][0] = Dt;
 
     // ---- 4. Merges, charges and concretization of box r, in two passes
     // over the lane's indices to bound the live partial sums.
