@@ -421,33 +421,66 @@ __global__ __launch_bounds__(kSortThreads) void sort_tile_kernel(
 // 4g .. 4g + 3 merged into one (ties: the earlier run first).  Each element
 // goes to its group's start + its index in its run + its rank in each
 // other run of the group: the length of the run's prefix whose keys come
-// before it (strictly, or tied as well for the runs before its own).  The
-// three ranks are found by one interleaved binary search over prefix
-// lengths, so their loads are in flight together.
-__global__ void merge4_pass_kernel(int64_t N, int lg, const double* __restrict__ kIn,
-                                   const int64_t* __restrict__ vIn, double* __restrict__ kOut,
-                                   int64_t* __restrict__ vOut)
+// before it (strictly, or tied as well for the runs before its own).  A
+// workgroup's 256 elements lie in one run, so it stages every run's window
+// pivots (the last key of each window of P keys, P >= 64, at most 256
+// windows) in LDS: a search first counts the whole windows before the
+// element there, then searches one window in global memory.
+constexpr int kMergePivots = 256;
+
+__global__ __launch_bounds__(256) void merge4_pass_kernel(int64_t N, int lg,
+                                                          const double* __restrict__ kIn,
+                                                          const int64_t* __restrict__ vIn,
+                                                          double* __restrict__ kOut,
+                                                          int64_t* __restrict__ vOut)
 {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= N) return;
+    __shared__ unsigned long long piv[4][kMergePivots];
+    const int64_t t0 = (int64_t)blockIdx.x * 256;
     const int64_t run = (int64_t)1 << lg;
-    const int64_t r = t >> lg;
+    const int64_t r = t0 >> lg;                      // the block's run (runs >= 1,024)
     const int q = (int)(r & 3);
-    const int64_t g0 = (r - q) << lg;      // the group's start
+    const int64_t g0 = (r - q) << lg;                // the group's start
+    const int np = run / 64 < kMergePivots ? (int)(run / 64) : kMergePivots;
+    const int64_t P = run / np;                      // window length, a power of two
+    for (int idx = threadIdx.x; idx < 4 * np; idx += 256) {
+        const int o = idx / np, w = idx - o * np;
+        const int64_t at = g0 + o * run + (int64_t)(w + 1) * P - 1;
+        // a window reaching past N never counts as wholly before
+        piv[o][w] = (at < N && at < g0 + (o + 1) * run) ? desc_key(kIn[at]) : 0ull;
+    }
+    __syncthreads();
+    const int64_t t = t0 + threadIdx.x;
+    if (t >= N) return;
     const unsigned long long x = desc_key(kIn[t]);
-    int64_t lo[3], len[3], pos[3] = {0, 0, 0};
+    int64_t lo[3], len[3], pos[3];
+    int ou[3];
     bool tie[3];
 #pragma unroll
     for (int u = 0; u < 3; ++u) {
-        const int o = u < q ? u : u + 1;    // the other runs, in order
+        const int o = u < q ? u : u + 1;             // the other runs, in order
+        ou[u] = o;
         lo[u] = g0 + o * run;
         len[u] = lo[u] < N ? min(run, N - lo[u]) : 0;
         tie[u] = o < q;
+        pos[u] = 0;
     }
-    for (int64_t step = run; step > 0; step >>= 1) {
+    // whole windows before x (pivot keys are 0 past the run: never before)
+#pragma unroll
+    for (int u = 0; u < 3; ++u) {
+        int w = 0;
+        for (int step = kMergePivots; step > 0; step >>= 1) {
+            if (w + step <= np) {
+                const unsigned long long y = piv[ou[u]][w + step - 1];
+                if (y > x || (tie[u] && y == x && y != 0ull)) w += step;
+            }
+        }
+        pos[u] = (int64_t)w * P;
+    }
+    // then inside the next window, the three searches interleaved
+    for (int64_t step = P; step > 0; step >>= 1) {
 #pragma unroll
         for (int u = 0; u < 3; ++u) {
-            if (pos[u] + step <= len[u]) {
+            if (pos[u] + step <= len[u] && ((pos[u] & (P - 1)) + step) <= P) {
                 const unsigned long long y = desc_key(kIn[lo[u] + pos[u] + step - 1]);
                 if (y > x || (tie[u] && y == x)) pos[u] += step;
             }
