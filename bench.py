@@ -28,7 +28,7 @@ FP32_PEAK_TFLOPS = 157.3
 # HBM bytes per bound launch of 32,768 boxes read in place, measured with rocprofv3
 # FETCH_SIZE (x2.00) + WRITE_SIZE (x1.00), scale factors from a 256 MiB copy
 # (profiles/qcqp_crown_f64_n50_m20.md); scaled to the launch size below.
-MEASURED_TRAFFIC_BYTES_32K = 109.6e6
+MEASURED_TRAFFIC_BYTES_32K = 80.9e6
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the expression-program bound on the 256-instance
 # workload (rocprofv3 FETCH_SIZE x2.00 + WRITE_SIZE x1.00,
