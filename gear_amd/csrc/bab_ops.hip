@@ -1141,7 +1141,10 @@ split_keep16_kernel(int K, int n, int S, int R, const double *__restrict__ pL,
 // from the parent, and bound row [key, plb[p][1:]] to frontier row
 // base + rank, and both children's slot ids (base + t for the first *total
 // positions t, the dead record after them).  Every load is issued before
-// the first store.
+// the first store.  prows may be the first K entries of order_out itself
+// (the frontier's picks, read in place): the wave of parent p reads only
+// prows[p], before it writes order_out[p] and order_out[p + K], and no
+// other wave reads either.
 __global__ void __launch_bounds__(1024)
 split_copy_kernel(int K, int n, int S, const double *__restrict__ pL,
                   const double *__restrict__ pU, const int64_t *__restrict__ prows,
