@@ -535,6 +535,25 @@ gather_rows_kernel(int64_t N, const int64_t* __restrict__ idx, int w0,
     }
 }
 
+// gather_rows_kernel for rows of W = w0 + w1 + w2 <= LPR entries: 64 / LPR
+// rows per wave, one entry per lane.
+template <int LPR>
+__global__ void __launch_bounds__(256)
+gather_rows_lanes_kernel(int64_t N, const int64_t* __restrict__ idx, int w0,
+                         const double* __restrict__ s0, double* __restrict__ d0, int w1,
+                         const double* __restrict__ s1, double* __restrict__ d1, int w2,
+                         const double* __restrict__ s2, double* __restrict__ d2)
+{
+    const int lane = threadIdx.x & 63, c = lane % LPR;
+    const int64_t i = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * (64 / LPR) + lane / LPR;
+    const int W = w0 + w1 + w2;
+    if (i >= N || c >= W) return;
+    const int64_t r = idx[i];
+    if (c < w0) d0[i * w0 + c] = s0[r * w0 + c];
+    else if (c < w0 + w1) d1[i * w1 + (c - w0)] = s1[r * w1 + (c - w0)];
+    else d2[i * w2 + (c - w0 - w1)] = s2[r * w2 + (c - w0 - w1)];
+}
+
 // min over i < N of x[idx[i] * stride] into *out (atomic min).
 __global__ __launch_bounds__(256) void min_rows_kernel(int64_t N, int64_t stride,
                                                        const int64_t* __restrict__ idx,
@@ -563,6 +582,15 @@ __global__ void filter_payload_kernel(int B, int W, int P, const double* __restr
 }
 
 inline int status() { return hipGetLastError() == hipSuccess ? GH_OK : GH_ELAUNCH; }
+
+// Keys of N rows, one thread per row (the row's S loads in flight together
+// beat lane groups, which need a reduction).
+void launch_frontier_keys(int64_t N, int S, const int64_t* order, const double* lb,
+                          const double* thr, double* keys, hipStream_t stream)
+{
+    hipLaunchKernelGGL(frontier_keys_kernel, dim3((unsigned)((N + 255) / 256)), dim3(256), 0,
+                       stream, N, S, order, lb, thr, keys);
+}
 
 }  // namespace
 
@@ -640,8 +668,7 @@ extern "C" int gh_frontier_keys_f64(int64_t N, int S, const int64_t* order, cons
 {
     if (N < 0 || S <= 0 || (N > 0 && (!order || !lb || !thr || !keys))) return GH_EINVAL;
     if (N == 0) return GH_OK;
-    hipLaunchKernelGGL(frontier_keys_kernel, dim3((unsigned)((N + 255) / 256)), dim3(256), 0,
-                       stream, N, S, order, lb, thr, keys);
+    launch_frontier_keys(N, S, order, lb, thr, keys, stream);
     return status();
 }
 
@@ -674,8 +701,7 @@ extern "C" int gh_frontier_sort_f64(int64_t N, int S, const int64_t* order, cons
     double* k1 = (passes & 1) ? kOut : kTmp;
     int64_t* v1 = (passes & 1) ? vOut : vTmp;
     const unsigned blocks = (unsigned)((N + 255) / 256);
-    hipLaunchKernelGGL(frontier_keys_kernel, dim3(blocks), dim3(256), 0, stream, N, S, order, lb,
-                       thr, k1);
+    launch_frontier_keys(N, S, order, lb, thr, k1, stream);
     const int64_t tiles = (N + kSortTile - 1) / kSortTile;
     hipLaunchKernelGGL(sort_tile_kernel, dim3((unsigned)tiles), dim3(kSortThreads), 0, stream, N,
                        (const double*)k1, order, k0, v0);
@@ -710,8 +736,16 @@ extern "C" int gh_gather_rows_f64(int64_t N, const int64_t* idx, int w0, const d
     if (N == 0 || w0 + w1 + w2 == 0) return GH_OK;
     if (!idx || (w0 && (!s0 || !d0)) || (w1 && (!s1 || !d1)) || (w2 && (!s2 || !d2)))
         return GH_EINVAL;
-    hipLaunchKernelGGL(gather_rows_kernel, dim3((unsigned)((N + 3) / 4)), dim3(256), 0, stream, N,
-                       idx, w0, s0, d0, w1, s1, d1, w2, s2, d2);
+    const int W = w0 + w1 + w2;
+    if (W <= 16)
+        hipLaunchKernelGGL(gather_rows_lanes_kernel<16>, dim3((unsigned)((N + 15) / 16)),
+                           dim3(256), 0, stream, N, idx, w0, s0, d0, w1, s1, d1, w2, s2, d2);
+    else if (W <= 32)
+        hipLaunchKernelGGL(gather_rows_lanes_kernel<32>, dim3((unsigned)((N + 7) / 8)), dim3(256),
+                           0, stream, N, idx, w0, s0, d0, w1, s1, d1, w2, s2, d2);
+    else
+        hipLaunchKernelGGL(gather_rows_kernel, dim3((unsigned)((N + 3) / 4)), dim3(256), 0,
+                           stream, N, idx, w0, s0, d0, w1, s1, d1, w2, s2, d2);
     return status();
 }
 
@@ -1200,6 +1234,69 @@ split_copy_kernel(int K, int n, int S, const double *__restrict__ pL,
     }
 }
 
+// split_copy_kernel for n <= 64 and S <= 64: four parents per wave, 16
+// lanes each (lane gl of a group holds coordinates and bound entries
+// gl + 16k), the same writes.  prows may alias order_out as there: the
+// group of parent p reads only prows[p] before it writes order_out[p] and
+// order_out[p + K].
+__global__ void __launch_bounds__(256)
+split_copy16_kernel(int K, int n, int S, const double *__restrict__ pL,
+                    const double *__restrict__ pU, const int64_t *__restrict__ prows,
+                    const double *__restrict__ plb, const double *__restrict__ ckey,
+                    const int32_t *__restrict__ sidx, const int32_t *__restrict__ keep,
+                    const int32_t *__restrict__ offs, const int64_t *__restrict__ total,
+                    double *__restrict__ FL, double *__restrict__ FU, double *__restrict__ Flb,
+                    int64_t base, int64_t *__restrict__ order_out, int64_t dead)
+{
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63, gl = lane & 15;
+    const int blk = blockIdx.x, nblk = gridDim.x;
+    const int j = wv * 4 + (lane >> 4);           // the parent's place in the block
+    const int p = blk * kSplitBlock + j;
+    // keep flags of the block's parents: lanes 0..15 side 0, 16..31 side 1
+    const int q = blk * kSplitBlock + (lane & 15);
+    const bool kq = lane < 32 && q < K && keep[(size_t)q + (size_t)(lane >> 4) * K];
+    const unsigned long long km = __ballot(kq);
+    if (p >= K) return;
+    const int32_t off0 = offs[blk], off1 = offs[nblk + blk];
+    const int idx = sidx[p];
+    const int64_t T = *total;
+    const double key0 = ckey[p], key1 = ckey[(size_t)p + K];
+    const size_t br = prows ? (size_t)prows[p] : (size_t)p;   // the parent's box row
+    const double *l = pL + br * n, *u = pU + br * n;
+    const double *prow = plb + (size_t)p * S;
+    double li[4], ui[4], pb[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int i = gl + 16 * k;
+        li[k] = i < n ? l[i] : 0.0;
+        ui[k] = i < n ? u[i] : 0.0;
+        pb[k] = i < S ? prow[i] : 0.0;
+    }
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+        const unsigned long long side = (km >> (16 * c)) & 0xFFFFull;
+        if (!((side >> j) & 1)) continue;
+        const int64_t row = base + (c ? off1 : off0) + __popcll(side & ((1ull << j) - 1ull));
+        double *fl = FL + (size_t)row * n, *fu = FU + (size_t)row * n;
+        double *fb = Flb + (size_t)row * S;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int i = gl + 16 * k;
+            if (i < n) {
+                double lo = li[k], hi = ui[k];
+                split_coord(lo, hi, i, idx, c);
+                fl[i] = lo;
+                fu[i] = hi;
+            }
+            if (i < S) fb[i] = i == 0 ? (c ? key1 : key0) : pb[k];
+        }
+    }
+    if (gl < 2) {
+        const int64_t ch = (int64_t)p + (int64_t)gl * K;
+        order_out[ch] = ch < T ? base + ch : dead;
+    }
+}
+
 }  // namespace
 
 extern "C" int gh_branch_split_keys_f64(int K, int n, int S, int R, const double *pL,
@@ -1261,7 +1358,13 @@ extern "C" int gh_split_append_f64(int K, int n, int S, int R, const double *pL,
         hipLaunchKernelGGL(split_keep_wave_kernel, dim3(nblk), dim3(1024), 0, stream, K, n, S, R,
                            pL, pU, prows, plb, lA, lbias, thr, ckey, sidx, keep, cnt, disc_min);
     hipLaunchKernelGGL(filter_scan_kernel, dim3(1), dim3(1024), 0, stream, 2 * nblk, cnt, total);
-    hipLaunchKernelGGL(split_copy_kernel, dim3(nblk), dim3(1024), 0, stream, K, n, S, pL, pU,
-                       prows, plb, ckey, sidx, keep, cnt, total, FL, FU, Flb, base, order_out, dead);
+    if (n <= 64 && S <= 64)
+        hipLaunchKernelGGL(split_copy16_kernel, dim3(nblk), dim3(256), 0, stream, K, n, S, pL, pU,
+                           prows, plb, ckey, sidx, keep, cnt, total, FL, FU, Flb, base, order_out,
+                           dead);
+    else
+        hipLaunchKernelGGL(split_copy_kernel, dim3(nblk), dim3(1024), 0, stream, K, n, S, pL, pU,
+                           prows, plb, ckey, sidx, keep, cnt, total, FL, FU, Flb, base, order_out,
+                           dead);
     return hipGetLastError() == hipSuccess ? GH_OK : GH_ELAUNCH;
 }

@@ -417,21 +417,23 @@ def test_qcqp_crown_f64_staged_bit_exact(cuda, n, m, B):
     assert _bits_equal(lbias.cpu().numpy(), lbias_o)
 
 
-@pytest.mark.parametrize("N", [1, 100, 8191, 8192, 8193, 20000, 70000, 200000])
-def test_frontier_sort_equals_stable_argsort(cuda, N):
+@pytest.mark.parametrize("N,S", [(1, 3), (100, 3), (8191, 21), (8192, 3), (8193, 40),
+                                 (20000, 21), (70000, 3), (200000, 17)])
+def test_frontier_sort_equals_stable_argsort(cuda, N, S):
     """gh_frontier_sort_f64 orders the rows named by a slot list exactly as
     torch's stable descending argsort of their keys max_s (lb - thr) does
     (ties, NaN, +-inf, -0, a repeated dead slot), and writes the keys."""
     from gear_amd._lib import lib, ptr, stream_ptr
-    print(f"case N={N}", flush=True)
+    print(f"case N={N} S={S}", flush=True)
     rng = np.random.default_rng(N)
-    S, rows = 3, N + 50
+    rows = N + 50
     lb = np.round(rng.standard_normal((rows, S)) * 3) / 2                # many ties
-    lb[rng.random(rows) < 0.03, 1] = np.nan
+    lb[rng.random(rows) < 0.03, 1 % S] = np.nan
     lb[rng.random(rows) < 0.02, 0] = np.inf
     lb[rng.random(rows) < 0.02, :] = -np.inf
     lb[rng.random(rows) < 0.05, :] = -0.0
-    thr = np.array([0.0, 0.5, np.inf]) if N % 2 else np.array([0.0, -0.5, 1.0])
+    thr = np.round(rng.standard_normal(S), 1)
+    thr[S - 1] = np.inf if N % 2 else thr[S - 1]
     order = rng.permutation(rows)[:N]
     order[rng.random(N) < 0.1] = rows - 1          # slots naming one record
     keys = np.max(lb[order] - thr, axis=1)
