@@ -57,10 +57,22 @@ __device__ __forceinline__ double mfma4(double a, double b, double c)
     return __builtin_amdgcn_mfma_f64_4x4x4f64(a, b, c, 0, 0, 0);
 }
 
-// With KT = 1 (mod 4) the last 16-row tile c = NCH - 1 holds one index per
-// lane row (m = KT - 1, rows 16c .. 16c + 3): it is packed replicated over
-// the four 4x4 blocks and run on mfma4, a quarter of the 16x16x4 work.
-__host__ __device__ constexpr bool tail_tile(int kt) { return kt % 4 == 1; }
+// The last 16-row tile c = NCH - 1 holds V = KT - 4 (NCH - 1) indices per
+// lane row (m = 4c + v, rows 16c + 4v .. 16c + 4v + 3 for v < V).  With
+// V < 4 each class v is packed as its own tile, replicated over the four
+// 4x4 blocks, and run on mfma4 (~1/6 of a 16x16x4 each); tail_v gives V,
+// or 0 where the tile stays on 16x16x4 (V = 4, or the packed row of KT = 15
+// would not fit the staged kernel's double-buffered LDS).
+__host__ __device__ constexpr int tail_v(int kt)
+{
+    const int nch = (kt + 3) / 4, v = kt - 4 * (nch - 1);
+    return (v < 4 && (size_t)(nch - 1 + v) * kt * 64 * 32 <= 160 * 1024) ? v : 0;
+}
+// Tiles per packed row: the regular 16x16x4 tiles, then the tail classes.
+__host__ __device__ constexpr int packed_tiles(int kt)
+{
+    return (kt + 3) / 4 - (tail_v(kt) ? 1 : 0) + tail_v(kt);
+}
 
 // Packed Q operands (built once per problem by gh_qcqp_pack_f64):
 //   QB[s][c][t][l] = Q[s][16c + l%16][4t + l/16]   IBP B operand (Q^T tiles)
@@ -70,7 +82,8 @@ __host__ __device__ constexpr bool tail_tile(int kt) { return kt % 4 == 1; }
 __global__ void qcqp_pack_kernel(int n, int S, int nch, int kt, const double* __restrict__ Q,
                                  double* __restrict__ QB, double* __restrict__ QA)
 {
-    size_t per = (size_t)nch * kt * 64;
+    const int nt = packed_tiles(kt), nr = tail_v(kt) ? nch - 1 : nch;
+    size_t per = (size_t)nt * kt * 64;
     size_t total = (size_t)S * per;
     size_t u = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (u >= total) return;
@@ -80,11 +93,12 @@ __global__ void qcqp_pack_kernel(int n, int S, int nch, int kt, const double* __
     r -= (size_t)c * kt * 64;
     int t = (int)(r / 64), l = (int)(r - (size_t)t * 64);
     const double* Qs = Q + (size_t)s * n * n;
-    // the tail tile's lane l carries row 16c + l % 4 (replicated per block)
-    const int row = (tail_tile(kt) && c == nch - 1) ? (l & 3) : (l & 15);
-    int j = 16 * c + row, i = 4 * t + (l >> 4);
+    // a tail class v's lane l carries row 16 (NCH - 1) + 4v + l % 4
+    // (replicated per block); a regular tile's row 16c + l % 16
+    const int row = c < nr ? 16 * c + (l & 15) : 16 * nr + 4 * (c - nr) + (l & 3);
+    int j = row, i = 4 * t + (l >> 4);
     QB[u] = (j < n && i < n) ? Qs[(size_t)j * n + i] : 0.0;
-    int j2 = 4 * t + (l >> 4), i2 = 16 * c + row;
+    int j2 = 4 * t + (l >> 4), i2 = row;
     QA[u] = (j2 < n && i2 < n) ? Qs[(size_t)j2 * n + i2] : 0.0;
 }
 
@@ -164,7 +178,8 @@ __device__ __forceinline__ void row_forward(int n, const double* qb, const doubl
 {
     constexpr int NCH = (KT + 3) / 4;            // 16-wide tiles of j (and of i)
     constexpr int NM = KT;                       // per-lane indices q + 4m
-    constexpr int NR = tail_tile(KT) ? NCH - 1 : NCH;   // tiles on 16x16x4
+    constexpr int TV = tail_v(KT);               // tail classes on mfma4
+    constexpr int NR = TV ? NCH - 1 : NCH;       // tiles on 16x16x4
     double p_ql[4], p_qu[4], p_g[4], p_own[4];
 #pragma unroll
     for (int v = 0; v < 4; ++v) p_ql[v] = p_qu[v] = p_g[v] = p_own[v] = 0.0;
@@ -207,20 +222,30 @@ __device__ __forceinline__ void row_forward(int n, const double* qb, const doubl
             }
         }
     }
-    if constexpr (tail_tile(KT)) {
-        // the last tile: index m = KT - 1 (j = q + 4m) only
-        double tc = 0.0, td = 0.0, tm = 0.0;
+    if constexpr (TV > 0) {
+        // the last tile: indices m = 4 NR + v, v < TV, one packed class each
+        double tc[TV], td[TV], tm[TV];
+#pragma unroll
+        for (int v = 0; v < TV; ++v) tc[v] = td[v] = tm[v] = 0.0;
 #pragma unroll
         for (int t = 0; t < KT; ++t) {
-            const double q = qb[((size_t)NR * KT + t) * 64];
-            const double qa_ = fabs(q);
-            tc = mfma4(q, bx.x0(t), tc);
-            td = mfma4(qa_, bx.ep(t), td);
-            tm = mfma4(qa_, bx.mx(t), tm);
+            const double x0 = bx.x0(t), ep = bx.ep(t), mx = bx.mx(t);
+#pragma unroll
+            for (int v = 0; v < TV; ++v) {
+                const double q = qb[((size_t)(NR + v) * KT + t) * 64];
+                const double qa_ = fabs(q);
+                tc[v] = mfma4(q, x0, tc[v]);
+                td[v] = mfma4(qa_, ep, td[v]);
+                tm[v] = mfma4(qa_, mx, tm[v]);
+            }
             if (t % 4 == 3) __builtin_amdgcn_sched_barrier(0);
         }
-        plane_item<KT>(n, NM - 1, lq + 4 * (NM - 1), tc, td, tm, cs, bx, k, st, p_ql[0], p_qu[0],
-                       p_g[0], p_own[0]);
+#pragma unroll
+        for (int v = 0; v < TV; ++v) {
+            const int m = 4 * NR + v;
+            plane_item<KT>(n, m, lq + 4 * m, tc[v], td[v], tm[v], cs, bx, k, st, p_ql[v], p_qu[v],
+                           p_g[v], p_own[v]);
+        }
     }
     st.ql = gh::tree4x4(p_ql);
     st.qu = gh::tree4x4(p_qu);
@@ -239,26 +264,31 @@ __device__ __forceinline__ void row_backward(int n, int S, int s, const double* 
 {
     constexpr int NCH = (KT + 3) / 4;
     constexpr int NM = KT;
-    constexpr int NR = tail_tile(KT) ? NCH - 1 : NCH;   // tiles on 16x16x4
+    constexpr int TV = tail_v(KT);               // tail classes on mfma4
+    constexpr int NR = TV ? NCH - 1 : NCH;       // tiles on 16x16x4
     // ---- 3. Backward product: D[i][box] = sum_j Q[j][i] beta[box][j]
-    // (A = Q^T tile c, B = the lane's beta at index q + 4t); the tail tile
-    // (index m = KT - 1 only) on mfma4.
+    // (A = Q^T tile c, B = the lane's beta at index q + 4t); the tail
+    // classes (indices m = 4 NR + v) on mfma4.
     d4 Da[NCH];
-    double Dt = 0.0;
+    double Dt[TV > 0 ? TV : 1];
 #pragma unroll
     for (int c = 0; c < NCH; ++c) Da[c] = (d4){0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int v = 0; v < TV; ++v) Dt[v] = 0.0;
 #pragma unroll
     for (int t = 0; t < KT; ++t) {
 #pragma unroll
         for (int c = 0; c < NR; ++c)
             Da[c] = mfma16(qa[((size_t)c * KT + t) * 64], st.beta[t], Da[c]);
-        if constexpr (tail_tile(KT))
-            Dt = mfma4(qa[((size_t)NR * KT + t) * 64], st.beta[t], Dt);
+#pragma unroll
+        for (int v = 0; v < TV; ++v)
+            Dt[v] = mfma4(qa[((size_t)(NR + v) * KT + t) * 64], st.beta[t], Dt[v]);
         // keep the operand loads of later steps from being hoisted here
         // (they would hold KT * NCH registers)
         if (t % 4 == 3) __builtin_amdgcn_sched_barrier(0);
     }
-    if constexpr (tail_tile(KT)) Da[NCH - 1][0] = Dt;
+#pragma unroll
+    for (int v = 0; v < TV; ++v) Da[NCH - 1][v] = Dt[v];
 
     // ---- 4. Merges, charges and concretization of box r, in two passes
     // over the lane's indices to bound the live partial sums.
@@ -371,7 +401,7 @@ __global__ __launch_bounds__(W * 64) __attribute__((amdgpu_waves_per_eu(2))) voi
         bx.xu[m] = (i < n) ? xU[xr * n + i] : 0.0;
     }
 
-    const size_t rowstride = (size_t)NCH * KT * 64;
+    const size_t rowstride = (size_t)packed_tiles(KT) * KT * 64;
     RowState<KT> st;
     if constexpr (W > 1) {
         gh::stage_row<double, W>(QB, QA, rowstride, 0, qsm, wv, l);
@@ -418,7 +448,7 @@ int launch_f64(int B, int n, int S, const double* QB, const double* QA, const do
 {
     if (B >= kStagedMinBoxes) {
         constexpr int W = kStagedWaves;
-        const size_t rowstride = (size_t)((KT + 3) / 4) * KT * 64;
+        const size_t rowstride = (size_t)packed_tiles(KT) * KT * 64;
         const size_t lds = 4 * rowstride * sizeof(double);
         static bool attr = false;   // once per instantiation (same value every call)
         if (!attr) {
@@ -446,7 +476,8 @@ inline int nch_of(int n) { return (n + 15) / 16; }
 extern "C" size_t gh_qcqp_pack_bytes(int n, int S)
 {
     if (n <= 0 || S <= 0 || n > 64) return 0;
-    return sizeof(double) * 2 * (size_t)S * nch_of(n) * ((n + 3) / 4) * 64;
+    const int kt = (n + 3) / 4;
+    return sizeof(double) * 2 * (size_t)S * packed_tiles(kt) * kt * 64;
 }
 
 extern "C" int gh_qcqp_pack_f64(int n, int S, const double* Q, void* packed, hipStream_t stream)
@@ -454,7 +485,7 @@ extern "C" int gh_qcqp_pack_f64(int n, int S, const double* Q, void* packed, hip
     if (n <= 0 || S <= 0 || !Q || !packed) return GH_EINVAL;
     if (n > 64) return GH_EUNSUPPORTED;
     int nch = nch_of(n), kt = (n + 3) / 4;
-    size_t per = (size_t)S * nch * kt * 64;
+    size_t per = (size_t)S * packed_tiles(kt) * kt * 64;
     double* QB = (double*)packed;
     double* QA = QB + per;
     hipLaunchKernelGGL(qcqp_pack_kernel, dim3((unsigned)((per + 255) / 256)), dim3(256), 0, stream,
@@ -481,9 +512,9 @@ int crown_f64(int B, int n, int S, const void* packed, const double* Cm, const d
     k.h1 = (double)(1 + 1) * eps / 2.0 * (1.0 + 1e-6);
     k.hs = (double)(S + 1) * eps / 2.0 * (1.0 + 1e-6);
     k.hsn = (double)((long)S * n + 1) * eps / 2.0 * (1.0 + 1e-6);
-    int nch = nch_of(n), kt = (n + 3) / 4;
+    int kt = (n + 3) / 4;
     const double* QB = (const double*)packed;
-    const double* QA = QB + (size_t)S * nch * kt * 64;
+    const double* QA = QB + (size_t)S * packed_tiles(kt) * kt * 64;
 #define GH_KT(K) \
     case K: return launch_f64<K>(B, n, S, QB, QA, Cm, d, xL, xU, rows, lb, lA, lbias, \
                                  plane_rows, k, stream);

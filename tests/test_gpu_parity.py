@@ -21,7 +21,9 @@ def _bits_equal(a, b):
 
 
 @pytest.mark.parametrize("n,m,B", [(50, 20, 1000), (8, 3, 1024), (1, 0, 33), (7, 2, 65),
-                                   (16, 4, 31), (17, 1, 100), (33, 5, 257), (64, 3, 200)])
+                                   (16, 4, 31), (17, 1, 100), (33, 5, 257), (64, 3, 200),
+                                   (11, 2, 70), (22, 3, 130), (41, 4, 90), (55, 2, 60),
+                                   (60, 1, 50)])
 def test_qcqp_crown_f64_bit_exact(cuda, n, m, B):
     from gear_amd.crown import BoundedQCQP
     print(f"case n={n} m={m} B={B}", flush=True)
@@ -397,7 +399,8 @@ def test_unsynced_steps_equal_synced(cuda):
     assert _bits_equal(L0, L1) and _bits_equal(U0, U1) and _bits_equal(lb0, lb1)
 
 
-@pytest.mark.parametrize("n,m,B", [(50, 20, 16384 + 37), (17, 4, 20000)])
+@pytest.mark.parametrize("n,m,B", [(50, 20, 16384 + 37), (17, 4, 20000), (55, 3, 16400),
+                                   (41, 2, 16500), (10, 3, 17000)])
 def test_qcqp_crown_f64_staged_bit_exact(cuda, n, m, B):
     """Batches of 16,384 boxes or more take the LDS-staged kernel (128 boxes
     per workgroup, Q tiles double-buffered in LDS); bounds and planes stay
