@@ -32,7 +32,7 @@ MEASURED_TRAFFIC_BYTES_32K = 80.9e6
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the expression-program bound on the 256-instance
 # workload (rocprofv3 FETCH_SIZE x2.00 + WRITE_SIZE x1.00,
-# profiles/dag_crown_config3.md).
+# profiles/dag_crown_uneven256.md).
 MEASURED_DAG_TRAFFIC_BYTES = 32.58e9
 
 
@@ -118,10 +118,10 @@ def bench_uneven_dag(reps=20, cpu=True):
 
     from gear_amd.dag import DagPlan
     from gear_amd.problems import random_bisection_boxes
-    from gear_amd.program import config3_instances
+    from gear_amd.program import uneven_nlp_instances
 
     dev = torch.device("cuda:0")
-    insts = config3_instances()
+    insts = uneven_nlp_instances()
     rng = np.random.default_rng(0)
     counts = [int(64 * 2 ** rng.integers(0, 6)) for _ in insts]
     boxes = [random_bisection_boxes(P.xl, P.xu, B, seed=k)
@@ -170,7 +170,7 @@ def bench_uneven_dag(reps=20, cpu=True):
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": round(MEASURED_DAG_TRAFFIC_BYTES) if MEASURED_DAG_TRAFFIC_BYTES else None,
-            "traffic_source": "rocprofv3 FETCH_SIZE/WRITE_SIZE, profiles/dag_crown_config3.md",
+            "traffic_source": "rocprofv3 FETCH_SIZE/WRITE_SIZE, profiles/dag_crown_uneven256.md",
             "algorithmic_bytes": state + io,
             "algorithmic": "per-box state the backward passes read and write "
                            "(A, relaxation planes, node bounds) plus box / bound I/O",
@@ -203,10 +203,10 @@ def bench_precision_large(boxes=2048, reps=3, cpu=True, sample=16):
     import torch
 
     from gear_amd.crown import BoundedQCQP
-    from gear_amd.problems import config4_problem, random_bisection_boxes
+    from gear_amd.problems import qcqp_n500_m300_problem, random_bisection_boxes
 
     dev = torch.device("cuda:0")
-    p = config4_problem()
+    p = qcqp_n500_m300_problem()
     n, S = p.n, p.S
     L, U = random_bisection_boxes(p.xl, p.xu, boxes, seed=7)
     net = BoundedQCQP(p, dev, precision="fp32s")
@@ -333,9 +333,9 @@ def bench_clip(boxes=32768, reps=10):
     from gear_amd.bab import BaBConfig, QCQPBranchAndBound
     from gear_amd.concretize import clip_domains, constrained_bound
     from gear_amd.crown import BoundedQCQP, input_split_branching, input_split_parallel
-    from gear_amd.problems import config2_problem, random_bisection_boxes
+    from gear_amd.problems import qcqp_n50_m20_problem, random_bisection_boxes
     dev = torch.device("cuda:0")
-    p = config2_problem()
+    p = qcqp_n50_m20_problem()
     n, S = p.n, p.S
     net = BoundedQCQP(p, dev)
     L, U = random_bisection_boxes(p.xl, p.xu, boxes // 2, seed=5)
@@ -445,11 +445,11 @@ def main():
 
     from gear_amd.bab import BaBConfig, QCQPBranchAndBound
     from gear_amd.crown import input_split_branching, input_split_parallel
-    from gear_amd.problems import config2_problem, flops_per_box, bytes_per_box
+    from gear_amd.problems import qcqp_n50_m20_problem, flops_per_box, bytes_per_box
 
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
-    p = config2_problem()
+    p = qcqp_n50_m20_problem()
     n, S = p.n, p.S
     cap = 1 << 20
     # a reordered step bounds 2 batch domains and appends at most 4 batch children

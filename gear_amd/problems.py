@@ -113,13 +113,13 @@ def qcqp_dense(n, m, seed, name=None, box=1.0):
                 meta={"seed": seed})
 
 
-def config2_problem():
+def qcqp_n50_m20_problem():
     """Full batched B&B workload: n = 50, m = 20, coefficients N(0,1)/sqrt(n),
     x in [-1, 1]^50, seed 3."""
     return qcqp_dense(50, 20, seed=3, name="qcqp_n50_m20")
 
 
-def config4_problem():
+def qcqp_n500_m300_problem():
     """Precision workload: dense QCQP n = 500, m = 300, seed 5."""
     return qcqp_dense(500, 300, seed=5, name="qcqp_n500_m300")
 

@@ -2,7 +2,7 @@
 DAGs over x in a box, in the node format of include/gearhip_program.h.
 
 Row 0 is the objective (minimised), rows 1.. are constraints f_r(x) <= rhs_r.
-`Program` builds the node list; `config1_problem` and `config3_instances`
+`Program` builds the node list; `nlp_n8_m3_problem` and `uneven_nlp_instances`
 are the seeded synthetic generators of the small NLP and the uneven batch.
 """
 from dataclasses import dataclass, field
@@ -210,7 +210,7 @@ def _quadratic(P, Q, c=None, d=0.0):
     return acc
 
 
-def config1_problem(seed=1):
+def nlp_n8_m3_problem(seed=1):
     """n = 8, x in [-2, 2]^8.  f0 = x^T Q x + sum_i c_i exp(0.3 x_i) + 4
     bilinear x_a x_b (a != b); Q ~ N(0,1), c ~ U(-1,1).  Three constraints
     x^T A_k x + b_k^T x + d_k <= 1 with N(0,1) coefficients."""
@@ -279,7 +279,7 @@ def random_nlp(n, m, seed, terms_per_row=3):
     return P
 
 
-def config3_instances(count=256, seed=4):
+def uneven_nlp_instances(count=256, seed=4):
     """Uneven batch: n = clip(round(lognormal(ln 40, 0.8)), 5, 200), m = n // 4."""
     rng = np.random.default_rng(seed)
     ns = np.clip(np.round(rng.lognormal(np.log(40.0), 0.8, size=count)), 5, 200).astype(int)

@@ -10,7 +10,7 @@ import torch
 from scipy.optimize import minimize
 
 from gear_amd.alm import ALMConfig, ALMPGDSolver, replay_float64
-from gear_amd.problems import QCQP, DeviceQCQP, config2_problem, qcqp_dense
+from gear_amd.problems import QCQP, DeviceQCQP, qcqp_n50_m20_problem, qcqp_dense
 
 pytestmark = pytest.mark.gpu
 
@@ -40,7 +40,7 @@ def test_convex_box_qp_reaches_optimum(cuda):
 
 @pytest.mark.parametrize("n,m,seed", [(50, 20, 3), (20, 8, 1), (90, 10, 2)])
 def test_kernel_matches_autograd_iteration(cuda, n, m, seed):
-    p = config2_problem() if (n, m, seed) == (50, 20, 3) else qcqp_dense(n, m, seed=seed)
+    p = qcqp_n50_m20_problem() if (n, m, seed) == (50, 20, 3) else qcqp_dense(n, m, seed=seed)
     dev = DeviceQCQP(p, cuda)
     cfg = ALMConfig(num_restarts=32)
     L, U = _box(p, cuda)

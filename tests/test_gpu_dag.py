@@ -11,7 +11,7 @@ import pytest
 import torch
 
 from gear_amd.problems import random_bisection_boxes
-from gear_amd.program import Program, config1_problem, config3_instances, random_nlp
+from gear_amd.program import Program, nlp_n8_m3_problem, uneven_nlp_instances, random_nlp
 from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
@@ -47,12 +47,12 @@ def _bound(plans_progs, boxes, cuda, waves=None):
     return [lb.cpu().numpy() for lb in lbs]
 
 
-def test_config1_vs_oracle(cuda):
-    print("case config1 n=8 m=3 B=1024", flush=True)
-    P = config1_problem()
+def test_nlp_n8_m3_vs_oracle(cuda):
+    print("case nlp n=8 m=3 B=1024", flush=True)
+    P = nlp_n8_m3_problem()
     L, U = random_bisection_boxes(P.xl, P.xu, 1024, seed=11)
     lb = _bound([P], [(L, U)], cuda)[0]
-    _check(lb, O.dag_crown(P, L, U, threads=8), "config1")
+    _check(lb, O.dag_crown(P, L, U, threads=8), "nlp_n8_m3")
 
 
 def test_random_programs_uneven_batches(cuda):
@@ -99,14 +99,14 @@ def test_edge_program(cuda):
     _check(lb, O.dag_crown(P, L, U, threads=8), "edge program")
 
 
-def test_config3_subset(cuda):
-    insts = config3_instances()[:24]
+def test_uneven_instances_subset(cuda):
+    insts = uneven_nlp_instances()[:24]
     rng = np.random.default_rng(0)
     boxes = []
     for k, P in enumerate(insts):
         B = int(rng.integers(1, 150))
         boxes.append(random_bisection_boxes(P.xl, P.xu, B, seed=k))
-    print("case config3 subset", [P.n for P in insts], flush=True)
+    print("case uneven subset", [P.n for P in insts], flush=True)
     lbs = _bound(insts, boxes, cuda)
     for k, (P, (L, U)) in enumerate(zip(insts, boxes)):
         _check(lbs[k], O.dag_crown(P, L, U, threads=8), f"instance {k} n={P.n}")

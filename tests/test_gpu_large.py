@@ -5,7 +5,7 @@ import numpy as np
 import pytest
 import torch
 
-from gear_amd.problems import config4_problem, qcqp_dense, random_bisection_boxes
+from gear_amd.problems import qcqp_n500_m300_problem, qcqp_dense, random_bisection_boxes
 from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
@@ -46,14 +46,14 @@ def test_large_below_fp64_oracle(cuda, n, m, B):
     print(f"near-tie boxes settled in FP64: {int(flag.sum())}", flush=True)
 
 
-def test_config4_sample_below_fp64_oracle(cuda):
+def test_qcqp_n500_sample_below_fp64_oracle(cuda):
     """n = 500, m = 300: a 16-box sample against the oracle."""
-    print("case config4 n=500 m=300 B=16", flush=True)
-    p = config4_problem()
+    print("case qcqp n=500 m=300 B=16", flush=True)
+    p = qcqp_n500_m300_problem()
     L, U = random_bisection_boxes(p.xl, p.xu, 16, seed=7)
     lb32, _ = _run(p, L, U, cuda)
     lb64 = O.qcqp_crown(p.Q, p.c, p.d, L, U, threads=16)
-    _compare(lb32, lb64, "config4", 5e-2)
+    _compare(lb32, lb64, "qcqp_n500_m300", 5e-2)
 
 
 def test_large_sampled_soundness(cuda):

@@ -9,7 +9,7 @@ import pytest
 import torch
 from scipy.optimize import linprog
 
-from gear_amd.problems import config2_problem, qcqp_dense, random_bisection_boxes
+from gear_amd.problems import qcqp_n50_m20_problem, qcqp_dense, random_bisection_boxes
 from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
@@ -24,11 +24,11 @@ def _exact_D(c, c0, G, h, l, u, y):
     return F(float(c0)) - sum(F(float(h[i])) * y[i] for i in range(G.shape[0])) + sum(t)
 
 
-@pytest.mark.parametrize("which", ["small", "config2"])
+@pytest.mark.parametrize("which", ["small", "n50_m20"])
 def test_lp_certificate(cuda, which):
     from gear_amd.crown import BoundedQCQP
     from gear_amd.lp import qcqp_lp_bound
-    p = qcqp_dense(10, 3, seed=4) if which == "small" else config2_problem()
+    p = qcqp_dense(10, 3, seed=4) if which == "small" else qcqp_n50_m20_problem()
     L, U = random_bisection_boxes(p.xl, p.xu, 64, seed=5)
     net = BoundedQCQP(p, cuda)
     xL = torch.as_tensor(L, device=cuda)

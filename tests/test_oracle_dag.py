@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 from gear_amd.problems import random_bisection_boxes
-from gear_amd.program import Program, config1_problem, random_nlp
+from gear_amd.program import Program, nlp_n8_m3_problem, random_nlp
 from oracle import oracle as O
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -139,8 +139,8 @@ def test_pole_falls_back_to_interval():
     assert lb[0] == -math.inf and np.isfinite(lb[1])
 
 
-def test_config1_sound():
-    P = config1_problem()
+def test_nlp_n8_m3_sound():
+    P = nlp_n8_m3_problem()
     L, U = random_bisection_boxes(P.xl, P.xu, 128, seed=2)
     lb = O.dag_crown(P, L, U)
     rng = np.random.default_rng(3)
