@@ -17,9 +17,13 @@
 // double-double recomputation of r is not done: the bound is rigorous
 // without it, and at most looser where |G|^T y is large.)
 //
-// n, m <= 64: one wavefront per box (lp_dual_wave_kernel).  Larger: one
+// n, m <= 16: sixteen lanes per box, four boxes per wavefront
+// (lp_dual_g16_kernel).  n, m <= 64: one wavefront per box
+// (lp_dual_wave_kernel).  Larger: one
 // workgroup (256 threads) per box; the per-box vectors live in dynamic LDS
 // sized by (n, m), and G too when m n <= kGlds.
+#include <type_traits>
+
 #include "common.h"
 #include "gearhip_internal.h"
 
@@ -474,6 +478,226 @@ lp_dual_wave_kernel(int B, int m, int n, int S, const double *__restrict__ lA,
     if (y_out && lane < m) y_out[(size_t)b * m + lane] = ybest;
 }
 
+// ------------------------------------------------------------------------
+// Sixteen lanes per box for n <= 16 and m <= 16: four boxes per wavefront,
+// sixteen per 256-thread workgroup.  Lane s of a group holds column s of G
+// (its gradient entry) and row s of G (its dual step) in registers; the
+// vectors each step needs from the other lanes of the group come by DPP
+// row_newbcast (one 16-lane row of the wavefront is one box), so the
+// iteration touches neither LDS nor a barrier.  Same iteration, same
+// rigorous evaluation as the kernels above (the evaluation's sums run over
+// the group's lanes; its error bound holds for any summation order).
+
+constexpr int kG = 16;              // lanes per box
+constexpr int kGThreads = 256;      // 16 boxes per workgroup
+
+// lane I of this lane's 16-lane row
+template <int I>
+__device__ __forceinline__ double bc16(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, 0x150 + I, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x150 + I, 0xf, 0xf, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// f(integral_constant<int, I>) for I = 0 .. 15 while I < count (count uniform)
+template <int I = 0, class F>
+__device__ __forceinline__ void for_lanes(int count, F &&f)
+{
+    if constexpr (I < kG) {
+        if (I < count) {
+            f(std::integral_constant<int, I>{});
+            for_lanes<I + 1>(count, f);
+        }
+    }
+}
+
+__device__ __forceinline__ double gsum16(double v)
+{
+    for (int o = kG / 2; o >= 1; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+// rigorous D(y) of the group's LP: lane s holds column s of G (gc), y_s (row
+// lanes) and its column's data
+__device__ double rigorous_dual_g16(int m, int n, const double (&gc)[kG], double ys_eval,
+                                    double c0, double cj, double lj, double uj, double hi,
+                                    bool act_i)
+{
+    const int s = threadIdx.x & (kG - 1);
+    const double infl = 1.0 + gamma_k(2.0 * (m + n) + 16.0);
+    double r = 0.0, rho = 0.0, k = 0.0;
+    for_lanes(m, [&](auto I) {
+        const double yi = bc16<decltype(I)::value>(ys_eval), g = gc[decltype(I)::value];
+        if (yi != 0.0 && g != 0.0) {
+            const double p = g * yi;
+            r += p;
+            rho += fabs(p);
+            k += 1.0;
+        }
+    });
+    double t = 0.0, M = 0.0, eps = 0.0;
+    bool ok = true;
+    if (s < n) {
+        r = r + cj;
+        rho = rho + fabs(cj);
+        const double kk = k > 0 ? gamma_k(k + 1.0) : 0.0;
+        eps = kk * rho + 4.0 * ETA * (k + 2.0);
+        const bool fl = isfinite(lj), fu = isfinite(uj);
+        const double eps_sign = eps * infl;
+        const bool pos = (r > eps_sign) && fl, neg = (r < -eps_sign) && fu;
+        const bool zero = (rho == 0.0) && (k == 0.0);
+        const double l0 = fl ? lj : 0.0, u0 = fu ? uj : 0.0;
+        if (fl && fu) {
+            t = fmin(r * l0, r * u0);
+            M = fmax(fabs(l0), fabs(u0));
+        } else if (pos) {
+            t = r * l0;
+            M = fabs(l0);
+        } else if (neg) {
+            t = r * u0;
+            M = fabs(u0);
+        } else if (!zero) {
+            ok = false;
+        }
+    }
+    double hy = 0.0, ny = 0.0;
+    if (s < m && act_i && ys_eval > 0) {
+        hy = hi * ys_eval;
+        ny = 1.0;
+    }
+    const double t_sum = gsum16(t), t_abs = gsum16(fabs(t)), me = gsum16(M * eps);
+    const double nz_t = gsum16(t != 0.0 ? 1.0 : 0.0);
+    const double hy_sum = gsum16(hy), hy_abs = gsum16(fabs(hy)), nyy = gsum16(ny);
+    const double bad = gsum16(ok ? 0.0 : 1.0);
+    const double val = (c0 - hy_sum) + t_sum;
+    const double N = 1.0 + nyy + nz_t;
+    double E = gamma_k(N + 2.0) * (fabs(c0) + hy_abs + t_abs) + me + (m + n + 3) * ETA;
+    E = E * infl;
+    E = E + 4.0 * U * (fabs(val) + E);
+    const double lb = val - E;
+    return (bad == 0.0 && isfinite(lb)) ? lb : -INFINITY;
+}
+
+__global__ void __launch_bounds__(kGThreads)
+lp_dual_g16_kernel(int B, int m, int n, int S, const double *__restrict__ lA,
+                   const double *__restrict__ lbias, const double *__restrict__ thr,
+                   const double *__restrict__ xL, const double *__restrict__ xU, int iters,
+                   int ruiz_iters, int check_every, double *__restrict__ lb_out,
+                   double *__restrict__ y_out)
+{
+    const int b = blockIdx.x * (kGThreads / kG) + (threadIdx.x >> 4);
+    if (b >= B) return;  // a whole 16-lane row leaves: no broadcast reads it
+    const int s = threadIdx.x & (kG - 1);
+    const bool col = s < n, row = s < m;
+    const double *Gb = lA + (size_t)b * S * n + n;
+    const double c0 = lbias[(size_t)b * S];
+    const double cj = col ? lA[(size_t)b * S * n + s] : 0.0;
+    const double lj = col ? xL[(size_t)b * n + s] : 0.0;
+    const double uj = col ? xU[(size_t)b * n + s] : 0.0;
+    // row s (threshold, activity, entries) and column s of G, inactive rows zeroed
+    double gr[kG], gc[kG];
+    double hi = 1.0;
+    bool act_i = false;
+#pragma unroll
+    for (int j = 0; j < kG; ++j) gr[j] = (row && j < n) ? Gb[(size_t)s * n + j] : 0.0;
+    if (row) {
+        const double h0 = thr[s + 1] - lbias[(size_t)b * S + s + 1];
+        bool a = isfinite(h0);
+#pragma unroll
+        for (int j = 0; j < kG; ++j) a = a && isfinite(gr[j]);
+        act_i = a;
+        hi = a ? h0 : 1.0;
+    }
+#pragma unroll
+    for (int j = 0; j < kG; ++j) gr[j] = act_i ? gr[j] : 0.0;
+    const double actd = act_i ? 1.0 : 0.0;
+#pragma unroll
+    for (int i = 0; i < kG; ++i) gc[i] = 0.0;
+    for_lanes(m, [&](auto I) {
+        constexpr int i = decltype(I)::value;
+        const double g = col ? Gb[(size_t)i * n + s] : 0.0;
+        gc[i] = bc16<i>(actd) != 0.0 ? g : 0.0;
+    });
+    double best = rigorous_dual_g16(m, n, gc, 0.0, c0, cj, lj, uj, hi, act_i);
+    double ybest = 0.0;
+    if (iters > 0 && m > 0) {
+        // scalings: lane j holds w_j and Dc_j (column), lane i holds Dr_i (row)
+        const double w = col ? fmax(uj - lj, 0.0) : 0.0;
+        double dc = 1.0, dr = 1.0;
+        // hs = h - G l
+        double gl = 0.0;
+        for_lanes(n, [&](auto J) {
+            gl += gr[decltype(J)::value] * bc16<decltype(J)::value>(lj);
+        });
+        double hs = row ? hi - gl : 0.0;
+        for (int it = 0; it < ruiz_iters; ++it) {
+            const double v = w * dc, y = dr;
+            double rn = 0.0, cn = 0.0;
+            for_lanes(n, [&](auto J) {
+                constexpr int j = decltype(J)::value;
+                rn = fmax(rn, fabs(gr[j] * bc16<j>(v) * dr));
+            });
+            for_lanes(m, [&](auto I) {
+                constexpr int i = decltype(I)::value;
+                cn = fmax(cn, fabs(gc[i] * bc16<i>(y) * w * dc));
+            });
+            if (row) dr = dr / (rn > 0 ? sqrt(rn) : 1.0);
+            if (col) dc = dc / (cn > 0 ? sqrt(cn) : 1.0);
+        }
+        const double wdc = col ? w * dc : 0.0;
+        const double cs = cj * w * dc, zub = 1.0 / dc;
+        hs = hs * dr;
+        const double drr = row ? dr : 0.0;
+        double tau = 0.0, sig = 0.0;
+        {
+            double a = 0.0, q = 0.0;
+            for_lanes(m, [&](auto I) {
+                constexpr int i = decltype(I)::value;
+                a += fabs(gc[i] * bc16<i>(drr) * wdc);
+            });
+            for_lanes(n, [&](auto J) {
+                constexpr int j = decltype(J)::value;
+                q += fabs(gr[j] * dr * bc16<j>(wdc));
+            });
+            if (col) tau = 1.0 / fmax(a, 1e-12);
+            if (row) sig = 1.0 / fmax(q, 1e-12);
+        }
+        double zs = col ? 0.5 * zub : 0.0, ys = 0.0;
+        for (int k = 0; k < iters; ++k) {
+            // gradient of column s from the rows' Dr_i ys_i; primal step
+            const double yd = row ? dr * ys : 0.0;
+            double g = 0.0;
+            for_lanes(m, [&](auto I) {
+                constexpr int i = decltype(I)::value;
+                g += gc[i] * bc16<i>(yd);
+            });
+            g = cs + wdc * g;
+            const double zn = col ? fmin(fmax(zs - tau * g, 0.0), zub) : 0.0;
+            // dual step of row s from the columns' w_j Dc_j (2 zn_j - zs_j)
+            const double v = col ? wdc * (2.0 * zn - zs) : 0.0;
+            double a = 0.0;
+            for_lanes(n, [&](auto J) {
+                constexpr int j = decltype(J)::value;
+                a += gr[j] * bc16<j>(v);
+            });
+            if (row) ys = fmax(ys + sig * (dr * a - hs), 0.0);
+            zs = zn;
+            if ((k + 1) % check_every == 0 || k + 1 == iters) {
+                const double yi = (row && act_i) ? ys * dr : 0.0;
+                const double d = rigorous_dual_g16(m, n, gc, yi, c0, cj, lj, uj, hi, act_i);
+                if (isfinite(d) && (!isfinite(best) || d > best)) {
+                    best = d;
+                    ybest = yi;
+                }
+            }
+        }
+    }
+    if (s == 0) lb_out[b] = best;
+    if (y_out && row) y_out[(size_t)b * m + s] = ybest;
+}
+
 }  // namespace
 
 extern "C" int gh_lp_dual_bound_f64(int B, int n, int S, const double *lA, const double *lbias,
@@ -485,6 +709,12 @@ extern "C" int gh_lp_dual_bound_f64(int B, int n, int S, const double *lA, const
         return GH_EINVAL;
     const int m = S - 1;
     if (n > kMaxN || m > kMaxM) return GH_EUNSUPPORTED;
+    if (n <= kG && m <= kG) {
+        const int per = kGThreads / kG;
+        hipLaunchKernelGGL(lp_dual_g16_kernel, dim3((B + per - 1) / per), dim3(kGThreads), 0,
+                           stream, B, m, n, S, lA, lbias, thr, xL, xU, iters, 10, 50, lb, y);
+        return hipGetLastError() == hipSuccess ? GH_OK : GH_ELAUNCH;
+    }
     if (n <= 64 && m <= 64) {
         const size_t wlds = sizeof(double) * ((size_t)m * n + 128);
         hipLaunchKernelGGL(lp_dual_wave_kernel, dim3(B), dim3(64), wlds, stream, B, m, n, S, lA,

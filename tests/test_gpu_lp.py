@@ -24,12 +24,21 @@ def _exact_D(c, c0, G, h, l, u, y):
     return F(float(c0)) - sum(F(float(h[i])) * y[i] for i in range(G.shape[0])) + sum(t)
 
 
-@pytest.mark.parametrize("which", ["small", "n50_m20"])
+# (n, m, boxes): 16 lanes per box up to n, m = 16 (a partly filled last
+# workgroup at 37 boxes), one wavefront per box up to 64, one workgroup above
+@pytest.mark.parametrize("which", ["small", "n16_m16", "n5_m1", "n13_m6_b37", "n17_m4",
+                                   "n6_m17", "n50_m20"])
 def test_lp_certificate(cuda, which):
     from gear_amd.crown import BoundedQCQP
     from gear_amd.lp import qcqp_lp_bound
-    p = qcqp_dense(10, 3, seed=4) if which == "small" else qcqp_n50_m20_problem()
-    L, U = random_bisection_boxes(p.xl, p.xu, 64, seed=5)
+    sizes = {"small": (10, 3, 64), "n16_m16": (16, 16, 64), "n5_m1": (5, 1, 64),
+             "n13_m6_b37": (13, 6, 37), "n17_m4": (17, 4, 64), "n6_m17": (6, 17, 64)}
+    if which in sizes:
+        n, m, nbox = sizes[which]
+        p = qcqp_dense(n, m, seed=4)
+    else:
+        p, nbox = qcqp_n50_m20_problem(), 64
+    L, U = random_bisection_boxes(p.xl, p.xu, nbox, seed=5)
     net = BoundedQCQP(p, cuda)
     xL = torch.as_tensor(L, device=cuda)
     xU = torch.as_tensor(U, device=cuda)
