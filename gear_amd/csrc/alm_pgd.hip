@@ -20,15 +20,63 @@
 
 namespace {
 
+// The value lane (lane ^ 32) / (lane ^ 16) holds: v_permlane32_swap /
+// v_permlane16_swap exchange the two halves / the odd and even 16-lane rows
+// of a register, so with both operands v each lane finds its partner's value
+// in the operand it did not keep.
+__device__ __forceinline__ double xor32(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const auto lo = __builtin_amdgcn_permlane32_swap((unsigned)b, (unsigned)b, false, false);
+    const auto hi =
+        __builtin_amdgcn_permlane32_swap((unsigned)(b >> 32), (unsigned)(b >> 32), false, false);
+    const int e = (threadIdx.x & 32) ? 0 : 1;
+    return __longlong_as_double(((long long)hi[e] << 32) | (unsigned)lo[e]);
+}
+__device__ __forceinline__ double xor16(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const auto lo = __builtin_amdgcn_permlane16_swap((unsigned)b, (unsigned)b, false, false);
+    const auto hi =
+        __builtin_amdgcn_permlane16_swap((unsigned)(b >> 32), (unsigned)(b >> 32), false, false);
+    const int e = (threadIdx.x & 16) ? 0 : 1;
+    return __longlong_as_double(((long long)hi[e] << 32) | (unsigned)lo[e]);
+}
+// v of another lane of the same 16-lane row by a DPP control
+template <int CTRL>
+__device__ __forceinline__ double dpp(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xf, 0xf, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+constexpr int kRowRor8 = 0x128, kRowRor4 = 0x124;   // lane (i + 8) / (i + 4) mod 16
+constexpr int kQuadXor1 = 0xB1, kQuadXor2 = 0x4E;   // quad_perm [1,0,3,2] / [2,3,0,1]
+
+// Wavefront reduction as the butterfly v = op(v, v[lane ^ o]) for
+// o = 32, 16, 8, 4, 2, 1, all in VALU/DPP.  After the o = 8 step lanes i and
+// i ^ 8 of a row hold the same value, so lane i + 4 mod 16 (row_ror:4) holds
+// what lane i ^ 4 holds; op is commutative, so every lane ends with the
+// butterfly's value bit for bit.
+template <class Op>
+__device__ __forceinline__ double wreduce(double v, Op op)
+{
+    v = op(v, xor32(v));
+    v = op(v, xor16(v));
+    v = op(v, dpp<kRowRor8>(v));
+    v = op(v, dpp<kRowRor4>(v));
+    v = op(v, dpp<kQuadXor2>(v));
+    v = op(v, dpp<kQuadXor1>(v));
+    return v;
+}
 __device__ __forceinline__ double wsum(double v)
 {
-    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o);
-    return v;
+    return wreduce(v, [](double a, double b) { return a + b; });
 }
 __device__ __forceinline__ double wmax(double v)
 {
-    for (int o = 32; o >= 1; o >>= 1) v = fmax(v, __shfl_xor(v, o));
-    return v;
+    return wreduce(v, [](double a, double b) { return fmax(a, b); });
 }
 
 __global__ void __launch_bounds__(256)
@@ -45,6 +93,13 @@ alm_symmetrize_kernel(int n, int S, const double *__restrict__ Q, double *__rest
 }
 
 constexpr int kMaxS = 1024;  // rows held in LDS per restart
+// Adam's per-step scalars (1 - beta1^t, sqrt(1 - beta2^t), the decay) are
+// tabulated in LDS once per workgroup for up to this many inner steps; a
+// longer schedule computes them in the step.  Same expressions either way.
+constexpr int kStepTab = 256;
+// Q (S n^2 doubles) is copied into LDS when it fits this many doubles and
+// n <= 64; above, the rows are read from global memory every step.
+constexpr int kQlds = 4096;
 
 // f_s for the rows s = w, w + W, ... of this wavefront at x (in LDS), and
 // optionally this wavefront's part of the ALM gradient.
@@ -121,9 +176,9 @@ __device__ __forceinline__ void eval_rows(int n, int S, const double *__restrict
 // One restart per workgroup of W wavefronts: the rows are split over the
 // wavefronts; wavefront 0 holds the iterate, sums the partial gradients in
 // a fixed order, and applies the projected Adam step.
-template <int K, int W>
+template <int K, int W, bool QL>
 __global__ void __launch_bounds__(64 * W)
-alm_pgd_kernel(int N, int n, int S, const double *__restrict__ Qs, const double *__restrict__ Cm,
+alm_pgd_kernel(int N, int n, int S, const double *__restrict__ Qg, const double *__restrict__ Cm,
                const double *__restrict__ d, const double *__restrict__ rhs,
                const double *__restrict__ lo_all, const double *__restrict__ hi_all,
                double *__restrict__ x_all, double *__restrict__ best_x_all,
@@ -133,9 +188,24 @@ alm_pgd_kernel(int N, int n, int S, const double *__restrict__ Qs, const double 
     __shared__ double fs[kMaxS];
     __shared__ double mu[kMaxS];
     __shared__ double gpart[W][K][64];
+    __shared__ double tab_bc1[kStepTab], tab_sq2[kStepTab], tab_decay[kStepTab];
     __shared__ double rho_s;
+    __shared__ double qlds[QL ? kQlds : 1];
     const int r = blockIdx.x;
     if (r >= N) return;
+    const bool tab = cfg.inner_steps <= kStepTab;
+    if (tab)
+        for (int t = threadIdx.x; t < cfg.inner_steps; t += 64 * W) {
+            const int step = t + 1;
+            tab_bc1[t] = 1.0 - pow(cfg.beta1, (double)step);
+            tab_sq2[t] = sqrt(1.0 - pow(cfg.beta2, (double)step));
+            tab_decay[t] = 1.0 - 0.9 * t / (double)max(cfg.inner_steps, 1);
+        }
+    const double *Qs = Qg;
+    if constexpr (QL) {
+        for (int e = threadIdx.x; e < S * n * n; e += 64 * W) qlds[e] = Qg[e];
+        Qs = qlds;
+    }
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const bool owner = w == 0;
     const double *lo_r = lo_all + (size_t)r * n, *hi_r = hi_all + (size_t)r * n;
@@ -179,12 +249,18 @@ alm_pgd_kernel(int N, int n, int S, const double *__restrict__ Qs, const double 
                         grad[k] = g;
                     }
                 }
-                const int step = t + 1;
-                const double bc1 = 1.0 - pow(cfg.beta1, (double)step);
-                const double bc2 = 1.0 - pow(cfg.beta2, (double)step);
-                const double decay = 1.0 - 0.9 * t / (double)max(cfg.inner_steps, 1);
+                double bc1, sq2, decay;
+                if (tab) {
+                    bc1 = tab_bc1[t];
+                    sq2 = tab_sq2[t];
+                    decay = tab_decay[t];
+                } else {
+                    const int step = t + 1;
+                    bc1 = 1.0 - pow(cfg.beta1, (double)step);
+                    sq2 = sqrt(1.0 - pow(cfg.beta2, (double)step));
+                    decay = 1.0 - 0.9 * t / (double)max(cfg.inner_steps, 1);
+                }
                 const double lr = cfg.lr0 * decay;
-                const double sq2 = sqrt(bc2);
 #pragma unroll
                 for (int k = 0; k < K; ++k) {
                     double g = grad[k];
@@ -240,8 +316,12 @@ void launch(int N, int n, int S, const double *Qs, const double *Cm, const doubl
             const double *rhs, const double *lo, const double *hi, double *x, double *bx,
             double *bo, const gh_alm_config &cfg, hipStream_t stream)
 {
-    hipLaunchKernelGGL((alm_pgd_kernel<K, W>), dim3(N), dim3(64 * W), 0, stream, N, n, S, Qs, Cm,
-                       d, rhs, lo, hi, x, bx, bo, cfg);
+    if (K == 1 && (size_t)S * n * n <= (size_t)kQlds)
+        hipLaunchKernelGGL((alm_pgd_kernel<K, W, K == 1>), dim3(N), dim3(64 * W), 0, stream, N, n,
+                           S, Qs, Cm, d, rhs, lo, hi, x, bx, bo, cfg);
+    else
+        hipLaunchKernelGGL((alm_pgd_kernel<K, W, false>), dim3(N), dim3(64 * W), 0, stream, N, n,
+                           S, Qs, Cm, d, rhs, lo, hi, x, bx, bo, cfg);
 }
 
 }  // namespace
