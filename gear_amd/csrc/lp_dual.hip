@@ -484,9 +484,11 @@ lp_dual_wave_kernel(int B, int m, int n, int S, const double *__restrict__ lA,
 // (its gradient entry) and row s of G (its dual step) in registers; the
 // vectors each step needs from the other lanes of the group come by DPP
 // row_newbcast (one 16-lane row of the wavefront is one box), so the
-// iteration touches neither LDS nor a barrier.  Same iteration, same
-// rigorous evaluation as the kernels above (the evaluation's sums run over
-// the group's lanes; its error bound holds for any summation order).
+// iteration touches neither LDS nor a barrier.  Same iteration (its two
+// matrix-vector sums as fused multiply-adds: the proposal's rounding never
+// touches soundness) and the same rigorous evaluation as the kernels above
+// (the evaluation's sums run over the group's lanes; its error bound holds
+// for any summation order).
 
 constexpr int kG = 16;              // lanes per box
 constexpr int kGThreads = 256;      // 16 boxes per workgroup
@@ -495,10 +497,7 @@ constexpr int kGThreads = 256;      // 16 boxes per workgroup
 template <int I>
 __device__ __forceinline__ double bc16(double v)
 {
-    const long long b = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, 0x150 + I, 0xf, 0xf, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x150 + I, 0xf, 0xf, false);
-    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+    return __builtin_amdgcn_update_dpp(0.0, v, 0x150 + I, 0xf, 0xf, false);  // v_mov_b64_dpp
 }
 
 // f(integral_constant<int, I>) for I = 0 .. 15 while I < count (count uniform)
@@ -671,7 +670,7 @@ lp_dual_g16_kernel(int B, int m, int n, int S, const double *__restrict__ lA,
             double g = 0.0;
             for_lanes(m, [&](auto I) {
                 constexpr int i = decltype(I)::value;
-                g += gc[i] * bc16<i>(yd);
+                g = fma(gc[i], bc16<i>(yd), g);
             });
             g = cs + wdc * g;
             const double zn = col ? fmin(fmax(zs - tau * g, 0.0), zub) : 0.0;
@@ -680,7 +679,7 @@ lp_dual_g16_kernel(int B, int m, int n, int S, const double *__restrict__ lA,
             double a = 0.0;
             for_lanes(n, [&](auto J) {
                 constexpr int j = decltype(J)::value;
-                a += gr[j] * bc16<j>(v);
+                a = fma(gr[j], bc16<j>(v), a);
             });
             if (row) ys = fmax(ys + sig * (dr * a - hs), 0.0);
             zs = zn;
