@@ -14,7 +14,11 @@
 // holds entries i, i + 64, ... of x, reads Qs[s][j][i] (coalesced; Qs is
 // symmetric) and x_j from LDS; f_s is a wavefront sum; the gradient
 // accumulates w_s (2 y + c_s) with w_0 = 1, w_s = rho relu(g_s + mu_s / rho).
-// A restart's rows are split over the W wavefronts of its workgroup.
+// A restart's rows are split over the W wavefronts of its workgroup; for
+// n <= 16 and S <= 8 one wavefront holds all of a restart's rows in
+// registers (alm_pgd_small_kernel).
+#include <type_traits>
+
 #include "common.h"
 #include "gearhip_internal.h"
 
@@ -311,6 +315,178 @@ alm_pgd_kernel(int N, int n, int S, const double *__restrict__ Qg, const double 
     }
 }
 
+// ------------------------------------------------------------------------
+// Small problems (n <= 16, S <= 4 G): one wavefront per restart with every
+// quantity of a step in registers.  Lane 16 r + i is column i of row slot r;
+// row slot r holds rows s = 4 g + r (g < G).  It keeps row i of Qs[s]
+// (Qs is symmetric) and c_s[i]; x, the box, Adam's moments and the best
+// point are replicated in the four row slots, so x_j reaches every lane of a
+// slot by DPP row_newbcast.  f_s is a 16-lane butterfly sum, the gradient
+// sum_s w_s (2 y_s + c_s) adds the slots by permlane swaps, and the Adam
+// step runs identically in every slot: a step touches no LDS but the
+// per-step scalars and needs no barrier.  Same iteration as above; only the
+// order of the sums differs.
+
+template <int I>
+__device__ __forceinline__ double bc16(double v)
+{
+    return __builtin_amdgcn_update_dpp(0.0, v, 0x150 + I, 0xf, 0xf, false);  // v_mov_b64_dpp
+}
+
+// f(integral_constant<int, J>) for J = 0 .. 15 while J < count (count uniform)
+template <int J = 0, class F>
+__device__ __forceinline__ void for_cols(int count, F &&f)
+{
+    if constexpr (J < 16) {
+        if (J < count) {
+            f(std::integral_constant<int, J>{});
+            for_cols<J + 1>(count, f);
+        }
+    }
+}
+
+// butterfly over the 16 lanes of a row (o = 8, 4, 2, 1), as in wreduce
+template <class Op>
+__device__ __forceinline__ double rreduce(double v, Op op)
+{
+    v = op(v, dpp<kRowRor8>(v));
+    v = op(v, dpp<kRowRor4>(v));
+    v = op(v, dpp<kQuadXor2>(v));
+    v = op(v, dpp<kQuadXor1>(v));
+    return v;
+}
+// across the four row slots (o = 32, 16)
+template <class Op>
+__device__ __forceinline__ double sreduce(double v, Op op)
+{
+    v = op(v, xor32(v));
+    v = op(v, xor16(v));
+    return v;
+}
+
+template <int G>
+__global__ void __launch_bounds__(64)
+alm_pgd_small_kernel(int N, int n, int S, const double *__restrict__ Qs,
+                     const double *__restrict__ Cm, const double *__restrict__ d,
+                     const double *__restrict__ rhs, const double *__restrict__ lo_all,
+                     const double *__restrict__ hi_all, double *__restrict__ x_all,
+                     double *__restrict__ best_x_all, double *__restrict__ best_obj_all,
+                     gh_alm_config cfg)
+{
+    __shared__ double tab_bc1[kStepTab], tab_sq2[kStepTab], tab_decay[kStepTab];
+    const int r = blockIdx.x;
+    if (r >= N) return;
+    const int lane = threadIdx.x, slot = lane >> 4, i = lane & 15;
+    const bool col = i < n;
+    const auto add = [](double a, double b) { return a + b; };
+    const auto mx = [](double a, double b) { return fmax(a, b); };
+    const bool tab = cfg.inner_steps <= kStepTab;
+    if (tab)
+        for (int t = lane; t < cfg.inner_steps; t += 64) {
+            const int step = t + 1;
+            tab_bc1[t] = 1.0 - pow(cfg.beta1, (double)step);
+            tab_sq2[t] = sqrt(1.0 - pow(cfg.beta2, (double)step));
+            tab_decay[t] = 1.0 - 0.9 * t / (double)max(cfg.inner_steps, 1);
+        }
+    double q[G][16], c[G], dd[G], rh[G], mu[G];
+    bool valid[G];
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+        const int s = 4 * g + slot;
+        valid[g] = s < S;
+#pragma unroll
+        for (int j = 0; j < 16; ++j)
+            q[g][j] = (valid[g] && col && j < n) ? Qs[((size_t)s * n + i) * n + j] : 0.0;
+        c[g] = (valid[g] && col) ? Cm[(size_t)s * n + i] : 0.0;
+        dd[g] = valid[g] ? d[s] : 0.0;
+        rh[g] = valid[g] ? rhs[s] : 0.0;
+        mu[g] = 0.0;
+    }
+    const double lo = col ? lo_all[(size_t)r * n + i] : 0.0;
+    const double hi = col ? hi_all[(size_t)r * n + i] : 0.0;
+    double x = col ? x_all[(size_t)r * n + i] : 0.0;
+    const double lrv = cfg.inner_lr * (hi - lo);
+    double bx = x, best = INFINITY, prev_viol = INFINITY, rho = cfg.rho0;
+    __syncthreads();  // the step tables
+    // y_s = Qs[s] x (row i), f_s at x for every row of the slot
+    auto eval = [&](double (&y)[G], double (&f)[G]) {
+#pragma unroll
+        for (int g = 0; g < G; ++g) y[g] = 0.0;
+        for_cols(n, [&](auto J) {
+            constexpr int j = decltype(J)::value;
+            const double xj = bc16<j>(x);
+#pragma unroll
+            for (int g = 0; g < G; ++g) y[g] = fma(q[g][j], xj, y[g]);
+        });
+#pragma unroll
+        for (int g = 0; g < G; ++g) f[g] = rreduce(x * (y[g] + c[g]), add) + dd[g];
+    };
+    for (int outer = 0; outer < cfg.outer_max_iter; ++outer) {
+        double ex = 0.0, ex2 = 0.0;
+        for (int t = 0; t < cfg.inner_steps; ++t) {
+            double y[G], f[G];
+            eval(y, f);
+            double grad = 0.0;
+#pragma unroll
+            for (int g = 0; g < G; ++g) {
+                const int s = 4 * g + slot;
+                double wt = 1.0;
+                if (s > 0) wt = rho * fmax((f[g] - rh[g]) + mu[g] / rho, 0.0);
+                if (valid[g] && wt != 0.0) grad = fma(wt, 2.0 * y[g] + c[g], grad);
+            }
+            grad = sreduce(grad, add);
+            double bc1, sq2, decay;
+            if (tab) {
+                bc1 = tab_bc1[t];
+                sq2 = tab_sq2[t];
+                decay = tab_decay[t];
+            } else {
+                const int step = t + 1;
+                bc1 = 1.0 - pow(cfg.beta1, (double)step);
+                sq2 = sqrt(1.0 - pow(cfg.beta2, (double)step));
+                decay = 1.0 - 0.9 * t / (double)max(cfg.inner_steps, 1);
+            }
+            const double lr = cfg.lr0 * decay;
+            double gg = grad;
+            if (isnan(gg) || isinf(gg)) gg = 0.0;  // nan_to_num
+            ex = ex * cfg.beta1 + (1.0 - cfg.beta1) * gg;
+            ex2 = ex2 * cfg.beta2 + (1.0 - cfg.beta2) * gg * gg;
+            const double denom = sqrt(ex2) / sq2 + cfg.adam_eps;
+            double ms = lrv * decay;
+            if (cfg.scale_eps_with_bias) ms = ms / bc1;
+            const double upd = fmin(fmax(-(ex / denom) * lr, -ms), ms);
+            x = fmin(fmax(x + upd, lo), hi);
+        }
+        // outer bookkeeping at the new point (alm.py:1860-1914)
+        double y[G], f[G];
+        eval(y, f);
+        double v = 0.0;
+#pragma unroll
+        for (int g = 0; g < G; ++g)
+            if (valid[g] && 4 * g + slot > 0) v = fmax(v, fmax(f[g] - rh[g], 0.0));
+        const double viol = sreduce(v, mx);
+        // f_0 lives in slot 0 (every lane of it); lane 0 is always active
+        const long long f0b = __double_as_longlong(f[0]);
+        const int f0lo = __builtin_amdgcn_readfirstlane((int)f0b);
+        const int f0hi = __builtin_amdgcn_readfirstlane((int)(f0b >> 32));
+        const double f0 = __longlong_as_double(((long long)f0hi << 32) | (unsigned)f0lo);
+        if (viol <= cfg.feas_tol && f0 < best) {
+            best = f0;
+            bx = x;
+        }
+#pragma unroll
+        for (int g = 0; g < G; ++g)
+            if (valid[g] && 4 * g + slot > 0) mu[g] = fmax(mu[g] + rho * (f[g] - rh[g]), 0.0);
+        if (viol > 0.25 * prev_viol) rho = fmin(rho * cfg.kappa, cfg.max_rho);
+        prev_viol = viol;
+    }
+    if (slot == 0 && col) {
+        x_all[(size_t)r * n + i] = x;
+        best_x_all[(size_t)r * n + i] = bx;
+    }
+    if (lane == 0) best_obj_all[r] = best;
+}
+
 template <int K, int W>
 void launch(int N, int n, int S, const double *Qs, const double *Cm, const double *d,
             const double *rhs, const double *lo, const double *hi, double *x, double *bx,
@@ -352,6 +528,15 @@ extern "C" int gh_alm_pgd_f64(int N, int n, int S, const void *plan, const doubl
         return GH_EINVAL;
     if (S > kMaxS || n > 1024) return GH_EUNSUPPORTED;
     const double *Qs = static_cast<const double *>(plan);
+    if (n <= 16 && S <= 8) {
+        if (S <= 4)
+            hipLaunchKernelGGL(alm_pgd_small_kernel<1>, dim3(N), dim3(64), 0, stream, N, n, S, Qs,
+                               Cm, d, rhs, lo, hi, x, best_x, best_obj, *cfg);
+        else
+            hipLaunchKernelGGL(alm_pgd_small_kernel<2>, dim3(N), dim3(64), 0, stream, N, n, S, Qs,
+                               Cm, d, rhs, lo, hi, x, best_x, best_obj, *cfg);
+        return hipGetLastError() == hipSuccess ? GH_OK : GH_ELAUNCH;
+    }
     const int K = (n + 63) / 64;
     // rows split over 8 wavefronts for n <= 64 and 4 (one per SIMD, so the
     // iterate's registers fit without spills) above

@@ -38,7 +38,9 @@ def test_convex_box_qp_reaches_optimum(cuda):
     assert abs(res.obj - ref.fun) <= 1e-6 * (1 + abs(ref.fun))
 
 
-@pytest.mark.parametrize("n,m,seed", [(50, 20, 3), (20, 8, 1), (90, 10, 2)])
+# n <= 16 with S <= 4 / S <= 8 runs the one-wavefront register kernel
+@pytest.mark.parametrize("n,m,seed", [(50, 20, 3), (20, 8, 1), (90, 10, 2), (8, 3, 1), (15, 6, 2),
+                                      (5, 2, 3), (16, 7, 4)])
 def test_kernel_matches_autograd_iteration(cuda, n, m, seed):
     p = qcqp_n50_m20_problem() if (n, m, seed) == (50, 20, 3) else qcqp_dense(n, m, seed=seed)
     dev = DeviceQCQP(p, cuda)
