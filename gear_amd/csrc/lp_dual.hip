@@ -12,7 +12,9 @@
 //     D(y) = c0 - h^T y + sum_j min(r_j l_j, r_j u_j),   r = c + G^T y,
 // evaluated with the error bound of rigorous_dual.py:1-54 (Higham Lemma 3.1;
 // the sums may run in any order) at y = 0 and every `check_every`
-// iterations, keeping the best.  The proposal's precision never touches
+// iterations, keeping the best.  A box whose certified value reaches a
+// finite objective threshold thr[0] (the incumbent: the prune drops the box
+// whatever its exact bound) stops at that checkpoint.  The proposal's precision never touches
 // soundness; only the evaluation does.  (The reference's optional
 // double-double recomputation of r is not done: the bound is rigorous
 // without it, and at most looser where |G|^T y is large.)
@@ -36,6 +38,11 @@ constexpr int kMaxN = 512, kMaxM = 512;
 constexpr int kGlds = 4096;  // doubles of G staged in LDS
 
 __device__ __forceinline__ double gamma_k(double k) { return k * U / (1.0 - k * U); }
+// the certified value reached the objective threshold: the box is pruned
+__device__ __forceinline__ bool reached(double best, double thr0)
+{
+    return isfinite(thr0) && best >= thr0;
+}
 
 // block-wide sum of one double per thread (all threads return it)
 __device__ double block_sum(double v, double *red)
@@ -189,7 +196,8 @@ lp_dual_kernel(int B, int m, int n, int S, const double *__restrict__ lA,
         return Gl ? Gl[i * n + j] : (act[i] ? L.G[(size_t)i * n + j] : 0.0);
     };
     double best = rigorous_dual(L, Gl, h, yv, act, red);
-    if (iters > 0 && m > 0) {
+    const double thr0 = thr[0];
+    if (iters > 0 && m > 0 && !reached(best, thr0)) {
         // x = l + w z, z in [0, 1]; hs = h - G l
         for (int j = t; j < n; j += kThreads) wDc[j] = fmax(L.u[j] - L.l[j], 0.0);  // w, then w Dc
         for (int i = t; i < m; i += kThreads) {
@@ -268,6 +276,7 @@ lp_dual_kernel(int B, int m, int n, int S, const double *__restrict__ lA,
                 }
             }
             __syncthreads();
+            if (reached(best, thr0)) break;  // best is the same in every thread
         }
     }
     __syncthreads();
@@ -393,7 +402,8 @@ lp_dual_wave_kernel(int B, int m, int n, int S, const double *__restrict__ lA,
     __syncthreads();
     double best = rigorous_dual_wave(m, n, W, c0, cj, lj, uj, hi, act_i);
     double ybest = 0.0;
-    if (iters > 0 && m > 0) {
+    const double thr0 = thr[0];
+    if (iters > 0 && m > 0 && !reached(best, thr0)) {
         // scalings: lane j holds w_j and Dc_j, lane i holds Dr_i
         const double w = lane < n ? fmax(uj - lj, 0.0) : 0.0;
         double dc = 1.0, dr = 1.0;
@@ -472,6 +482,7 @@ lp_dual_wave_kernel(int B, int m, int n, int S, const double *__restrict__ lA,
                 }
             }
             __syncthreads();
+            if (reached(best, thr0)) break;  // best is the same in every lane
         }
     }
     if (lane == 0) lb_out[b] = best;
@@ -621,7 +632,8 @@ lp_dual_g16_kernel(int B, int m, int n, int S, const double *__restrict__ lA,
     });
     double best = rigorous_dual_g16(m, n, gc, 0.0, c0, cj, lj, uj, hi, act_i);
     double ybest = 0.0;
-    if (iters > 0 && m > 0) {
+    const double thr0 = thr[0];
+    if (iters > 0 && m > 0 && !reached(best, thr0)) {
         // scalings: lane j holds w_j and Dc_j (column), lane i holds Dr_i (row)
         const double w = col ? fmax(uj - lj, 0.0) : 0.0;
         double dc = 1.0, dr = 1.0;
@@ -690,6 +702,9 @@ lp_dual_g16_kernel(int B, int m, int n, int S, const double *__restrict__ lA,
                     best = d;
                     ybest = yi;
                 }
+                // best is the same in the group's 16 lanes; the group leaves
+                // as a whole, and no other row's broadcasts read its lanes
+                if (reached(best, thr0)) break;
             }
         }
     }

@@ -178,6 +178,10 @@ int gh_alm_pgd_f64(int N, int n, int S, const void *plan, const double *Cm, cons
  * checkpoint, evaluated with the error model of rigorous_dual.py:1-54 from
  * FP64 operations only, so it is a certificate whatever the iteration
  * count.  Rows with a non-finite threshold or plane are inactive (y = 0).
+ * With a finite objective threshold thr[0] (the incumbent), a box stops
+ * iterating at the first checkpoint whose certified value reaches thr[0]:
+ * the prune drops such a box whatever its exact bound; every other box runs
+ * all `iters` iterations.
  *   lA (B, S, n), lbias (B, S) from gh_qcqp_crown_f64; thr (S); xL, xU (B, n);
  *   lb (B): the certified bound (-inf if none); y (B, S - 1): its
  *   multipliers (may be NULL).

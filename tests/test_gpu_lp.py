@@ -148,3 +148,49 @@ def test_bab_with_lp_pruning_keeps_the_optimum(cuda):
     assert is_primal_dual_converged(P1, D1, 1e-5, 1e-4)
     assert D0 <= D1 + 1e-9 * max(1.0, abs(P0))
     assert n1 <= n0
+
+
+# 16 lanes per box, one wavefront per box, one workgroup per box
+@pytest.mark.parametrize("n,m", [(10, 3), (17, 4), (70, 5)])
+def test_lp_stops_at_the_objective_threshold(cuda, n, m):
+    """With a finite thr[0] a box stops at the first checkpoint whose
+    certified value reaches it: the prune decision (lb >= thr[0]) equals the
+    one of the full run, the boxes that stay below run every iteration
+    (bit-identical bounds), and every bound is the oracle's value of its own
+    multipliers."""
+    from gear_amd.crown import BoundedQCQP
+    from gear_amd.lp import qcqp_lp_bound
+    p = qcqp_dense(n, m, seed=7)
+    L, U = random_bisection_boxes(p.xl, p.xu, 48, seed=3)
+    xL = torch.as_tensor(L, device=cuda)
+    xU = torch.as_tensor(U, device=cuda)
+    if n <= 64:
+        _, lA, lbias = BoundedQCQP(p, cuda).bound(xL, xU, return_A=True)
+    else:   # the FP64 bound covers n <= 64: seeded planes of the same shape
+        rng = np.random.default_rng(9)
+        lA = torch.as_tensor(rng.standard_normal((len(L), m + 1, n)) / np.sqrt(n), device=cuda)
+        lbias = torch.as_tensor(rng.standard_normal((len(L), m + 1)) * 0.3, device=cuda)
+    thr = torch.as_tensor(np.concatenate([[np.inf], p.rhs[1:] + 1e-6]), device=cuda)
+    full = qcqp_lp_bound(lA, lbias, thr, xL, xU, iters=400).cpu().numpy()
+    fin = full[np.isfinite(full)]
+    assert len(fin) > 8
+    thr0 = float(np.median(fin))
+    thr_t = thr.clone()
+    thr_t[0] = thr0
+    early, y = qcqp_lp_bound(lA, lbias, thr_t, xL, xU, iters=400, return_y=True)
+    torch.cuda.synchronize()
+    early, y = early.cpu().numpy(), y.cpu().numpy()
+    assert np.array_equal(early >= thr0, full >= thr0)
+    below = early < thr0
+    assert below.any() and (~below).any()
+    assert np.array_equal(early[below], full[below])
+    assert np.all(early[~below] <= full[~below])
+    lAn, lbn, thrn = lA.cpu().numpy(), lbias.cpu().numpy(), thr.cpu().numpy()
+    for b in range(len(L)):
+        c, c0, G = lAn[b, 0], lbn[b, 0], lAn[b, 1:]
+        h = thrn[1:] - lbn[b, 1:]
+        o = O.lp_dual_value(c, c0, G, h, L[b], U[b], y[b])
+        if np.isfinite(early[b]):
+            assert abs(o - early[b]) <= 1e-10 * (1 + abs(o)), b
+    print(f"n={n} m={m}: {int((~below).sum())} of {len(L)} boxes stopped at thr0 {thr0:.6f}",
+          flush=True)
