@@ -303,9 +303,35 @@ __device__ __forceinline__ double wmax64(double v)
     return v;
 }
 
-// per-wavefront LDS: G (m x n, row stride n), then y[64], v[64]
+// sum_k a[k * sa] * b[k] for k < len, in order, from LDS: eight entries of
+// each are loaded before their products are added, so eight loads are in
+// flight where a plain loop waits for every load in turn
+__device__ __forceinline__ double dot_lds(int len, const double *a, int sa, const double *b)
+{
+    double acc = 0.0;
+    int k = 0;
+    for (; k + 8 <= len; k += 8) {
+        double av[8], bv[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            av[u] = a[(k + u) * sa];
+            bv[u] = b[k + u];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) acc += av[u] * bv[u];
+    }
+    for (; k < len; ++k) acc += a[k * sa] * b[k];
+    return acc;
+}
+
+// per-wavefront LDS: G (m x n, row stride ld = n rounded up to odd), then
+// y[64], v[64].  Lane j reads column j as G[i ld + j] (consecutive) and lane
+// i row i as G[i ld + j]: an odd stride puts the 32 lanes of a half-wave on
+// distinct bank pairs, so the row reads have no bank conflicts either.
+__host__ __device__ __forceinline__ int wave_ld(int n) { return n | 1; }
 struct WaveLds {
     double *G, *y, *v;
+    int ld;
 };
 
 // rigorous D(y) of the wavefront's LP; y in LDS, lane j handles column j
@@ -319,7 +345,7 @@ __device__ double rigorous_dual_wave(int m, int n, const WaveLds W, double c0, d
     if (lane < n) {
         double r = 0.0, rho = 0.0, k = 0.0;
         for (int i = 0; i < m; ++i) {
-            const double yi = W.y[i], g = W.G[i * n + lane];
+            const double yi = W.y[i], g = W.G[i * W.ld + lane];
             if (yi != 0.0 && g != 0.0) {
                 const double p = g * yi;
                 r += p;
@@ -375,7 +401,8 @@ lp_dual_wave_kernel(int B, int m, int n, int S, const double *__restrict__ lA,
                     double *__restrict__ y_out)
 {
     extern __shared__ double wl[];
-    WaveLds W{wl, wl + (size_t)m * n, wl + (size_t)m * n + 64};
+    const int ld = wave_ld(n);
+    WaveLds W{wl, wl + (size_t)m * ld, wl + (size_t)m * ld + 64, ld};
     const int b = blockIdx.x;
     if (b >= B) return;
     const int lane = threadIdx.x;
@@ -396,7 +423,7 @@ lp_dual_wave_kernel(int B, int m, int n, int S, const double *__restrict__ lA,
     }
     for (int i = 0; i < m; ++i) {
         const bool ai = __shfl(act_i ? 1 : 0, i) != 0;
-        if (lane < n) W.G[i * n + lane] = ai ? Gb[(size_t)i * n + lane] : 0.0;
+        if (lane < n) W.G[i * ld + lane] = ai ? Gb[(size_t)i * n + lane] : 0.0;
     }
     W.y[lane] = 0.0;
     __syncthreads();
@@ -413,7 +440,7 @@ lp_dual_wave_kernel(int B, int m, int n, int S, const double *__restrict__ lA,
         __syncthreads();
         if (lane < m) {
             double gl = 0.0;
-            for (int j = 0; j < n; ++j) gl += W.G[lane * n + j] * W.v[j];
+            for (int j = 0; j < n; ++j) gl += W.G[lane * ld + j] * W.v[j];
             hs = hi - gl;
         }
         __syncthreads();
@@ -424,9 +451,9 @@ lp_dual_wave_kernel(int B, int m, int n, int S, const double *__restrict__ lA,
             __syncthreads();
             double rn = 0.0, cn = 0.0;
             if (lane < m)
-                for (int j = 0; j < n; ++j) rn = fmax(rn, fabs(W.G[lane * n + j] * W.v[j] * dr));
+                for (int j = 0; j < n; ++j) rn = fmax(rn, fabs(W.G[lane * ld + j] * W.v[j] * dr));
             if (lane < n)
-                for (int i = 0; i < m; ++i) cn = fmax(cn, fabs(W.G[i * n + lane] * W.y[i] * w * dc));
+                for (int i = 0; i < m; ++i) cn = fmax(cn, fabs(W.G[i * ld + lane] * W.y[i] * w * dc));
             __syncthreads();
             if (lane < m) dr = dr / (rn > 0 ? sqrt(rn) : 1.0);
             if (lane < n) dc = dc / (cn > 0 ? sqrt(cn) : 1.0);
@@ -440,12 +467,12 @@ lp_dual_wave_kernel(int B, int m, int n, int S, const double *__restrict__ lA,
         double tau = 0.0, sig = 0.0;
         if (lane < n) {
             double a = 0.0;
-            for (int i = 0; i < m; ++i) a += fabs(W.G[i * n + lane] * W.y[i] * wdc);
+            for (int i = 0; i < m; ++i) a += fabs(W.G[i * ld + lane] * W.y[i] * wdc);
             tau = 1.0 / fmax(a, 1e-12);
         }
         if (lane < m) {
             double a = 0.0;
-            for (int j = 0; j < n; ++j) a += fabs(W.G[lane * n + j] * dr * W.v[j]);
+            for (int j = 0; j < n; ++j) a += fabs(W.G[lane * ld + j] * dr * W.v[j]);
             sig = 1.0 / fmax(a, 1e-12);
         }
         double zs = lane < n ? 0.5 * zub : 0.0, ys = 0.0;
@@ -456,17 +483,14 @@ lp_dual_wave_kernel(int B, int m, int n, int S, const double *__restrict__ lA,
             __syncthreads();
             double zn = 0.0;
             if (lane < n) {
-                double g = 0.0;
-                for (int i = 0; i < m; ++i) g += W.G[i * n + lane] * W.y[i];
-                g = cs + wdc * g;
+                const double g = cs + wdc * dot_lds(m, W.G + lane, ld, W.y);
                 zn = fmin(fmax(zs - tau * g, 0.0), zub);
             }
             // lane j publishes w_j Dc_j (2 zn_j - zs_j); lane i: dual step
             W.v[lane] = lane < n ? wdc * (2.0 * zn - zs) : 0.0;
             __syncthreads();
             if (lane < m) {
-                double a = 0.0;
-                for (int j = 0; j < n; ++j) a += W.G[lane * n + j] * W.v[j];
+                const double a = dot_lds(n, W.G + lane * ld, 1, W.v);
                 ys = fmax(ys + sig * (dr * a - hs), 0.0);
             }
             zs = zn;
@@ -730,7 +754,7 @@ extern "C" int gh_lp_dual_bound_f64(int B, int n, int S, const double *lA, const
         return hipGetLastError() == hipSuccess ? GH_OK : GH_ELAUNCH;
     }
     if (n <= 64 && m <= 64) {
-        const size_t wlds = sizeof(double) * ((size_t)m * n + 128);
+        const size_t wlds = sizeof(double) * ((size_t)m * wave_ld(n) + 128);
         hipLaunchKernelGGL(lp_dual_wave_kernel, dim3(B), dim3(64), wlds, stream, B, m, n, S, lA,
                            lbias, thr, xL, xU, iters, 10, 50, lb, y);
         return hipGetLastError() == hipSuccess ? GH_OK : GH_ELAUNCH;
