@@ -418,31 +418,37 @@ __global__ __launch_bounds__(kSortThreads) void sort_tile_kernel(
 }
 
 // One pass of the stable merge sort: runs of 2^lg sorted descending, runs
-// 4g .. 4g + 3 merged into one (ties: the earlier run first).  Each element
-// goes to its group's start + its index in its run + its rank in each
-// other run of the group: the length of the run's prefix whose keys come
-// before it (strictly, or tied as well for the runs before its own).  A
-// workgroup's 256 elements lie in one run, so it stages every run's window
-// pivots (the last key of each window of P keys, P >= 64, at most 256
-// windows) in LDS: a search first counts the whole windows before the
-// element there, then searches one window in global memory.
+// WAYS g .. WAYS g + WAYS - 1 merged into one (ties: the earlier run first).
+// Each element goes to its group's start + its index in its run + its rank
+// in each other run of the group: the length of the run's prefix whose keys
+// come before it (strictly, or tied as well for the runs before its own).
+// A workgroup's 256 elements lie in one run, so it stages every run's
+// window pivots (the last key of each window of P keys, P >= 64, at most
+// 256 windows) in LDS: a search first counts the whole windows before the
+// element there, then searches one window in global memory, the WAYS - 1
+// searches interleaved so their loads are in flight together.
 constexpr int kMergePivots = 256;
+// four-way: 1,024-row tiles to 65,536 rows in three passes (an eight-way
+// pass took 20.6 us against 11 us, two passes against three: 41 us, 33.8)
+constexpr int kMergeLogWays = 2;
 
-__global__ __launch_bounds__(256) void merge4_pass_kernel(int64_t N, int lg,
-                                                          const double* __restrict__ kIn,
-                                                          const int64_t* __restrict__ vIn,
-                                                          double* __restrict__ kOut,
-                                                          int64_t* __restrict__ vOut)
+template <int LOGW>
+__global__ __launch_bounds__(256) void merge_pass_kernel(int64_t N, int lg,
+                                                         const double* __restrict__ kIn,
+                                                         const int64_t* __restrict__ vIn,
+                                                         double* __restrict__ kOut,
+                                                         int64_t* __restrict__ vOut)
 {
-    __shared__ unsigned long long piv[4][kMergePivots];
+    constexpr int WAYS = 1 << LOGW, O = WAYS - 1;
+    __shared__ unsigned long long piv[WAYS][kMergePivots];
     const int64_t t0 = (int64_t)blockIdx.x * 256;
     const int64_t run = (int64_t)1 << lg;
     const int64_t r = t0 >> lg;                      // the block's run (runs >= 1,024)
-    const int q = (int)(r & 3);
+    const int q = (int)(r & (WAYS - 1));
     const int64_t g0 = (r - q) << lg;                // the group's start
     const int np = run / 64 < kMergePivots ? (int)(run / 64) : kMergePivots;
     const int64_t P = run / np;                      // window length, a power of two
-    for (int idx = threadIdx.x; idx < 4 * np; idx += 256) {
+    for (int idx = threadIdx.x; idx < WAYS * np; idx += 256) {
         const int o = idx / np, w = idx - o * np;
         const int64_t at = g0 + o * run + (int64_t)(w + 1) * P - 1;
         // a window reaching past N never counts as wholly before
@@ -452,11 +458,11 @@ __global__ __launch_bounds__(256) void merge4_pass_kernel(int64_t N, int lg,
     const int64_t t = t0 + threadIdx.x;
     if (t >= N) return;
     const unsigned long long x = desc_key(kIn[t]);
-    int64_t lo[3], len[3], pos[3];
-    int ou[3];
-    bool tie[3];
+    int64_t lo[O], len[O], pos[O];
+    int ou[O];
+    bool tie[O];
 #pragma unroll
-    for (int u = 0; u < 3; ++u) {
+    for (int u = 0; u < O; ++u) {
         const int o = u < q ? u : u + 1;             // the other runs, in order
         ou[u] = o;
         lo[u] = g0 + o * run;
@@ -466,7 +472,7 @@ __global__ __launch_bounds__(256) void merge4_pass_kernel(int64_t N, int lg,
     }
     // whole windows before x (pivot keys are 0 past the run: never before)
 #pragma unroll
-    for (int u = 0; u < 3; ++u) {
+    for (int u = 0; u < O; ++u) {
         int w = 0;
         for (int step = kMergePivots; step > 0; step >>= 1) {
             if (w + step <= np) {
@@ -476,17 +482,19 @@ __global__ __launch_bounds__(256) void merge4_pass_kernel(int64_t N, int lg,
         }
         pos[u] = (int64_t)w * P;
     }
-    // then inside the next window, the three searches interleaved
+    // then inside the next window, the searches interleaved
     for (int64_t step = P; step > 0; step >>= 1) {
 #pragma unroll
-        for (int u = 0; u < 3; ++u) {
+        for (int u = 0; u < O; ++u) {
             if (pos[u] + step <= len[u] && ((pos[u] & (P - 1)) + step) <= P) {
                 const unsigned long long y = desc_key(kIn[lo[u] + pos[u] + step - 1]);
                 if (y > x || (tie[u] && y == x)) pos[u] += step;
             }
         }
     }
-    const int64_t dst = g0 + (t - (r << lg)) + pos[0] + pos[1] + pos[2];
+    int64_t dst = g0 + (t - (r << lg));
+#pragma unroll
+    for (int u = 0; u < O; ++u) dst += pos[u];
     kOut[dst] = kIn[t];
     vOut[dst] = vIn[t];
 }
@@ -693,7 +701,7 @@ extern "C" int gh_frontier_sort_f64(int64_t N, int S, const int64_t* order, cons
     if (N == 0) return GH_OK;
     if (!order || !lb || !thr || !kOut || !vOut || !kTmp || !vTmp) return GH_EINVAL;
     int passes = 0;
-    for (int lg = kSortTileLog; ((int64_t)1 << lg) < N; lg += 2) ++passes;
+    for (int lg = kSortTileLog; ((int64_t)1 << lg) < N; lg += kMergeLogWays) ++passes;
     // keys first (into the buffer the tile sort does not write), the tile
     // sort where the last merge pass must end in kOut
     double* k0 = (passes & 1) ? kTmp : kOut;
@@ -705,9 +713,9 @@ extern "C" int gh_frontier_sort_f64(int64_t N, int S, const int64_t* order, cons
     const int64_t tiles = (N + kSortTile - 1) / kSortTile;
     hipLaunchKernelGGL(sort_tile_kernel, dim3((unsigned)tiles), dim3(kSortThreads), 0, stream, N,
                        (const double*)k1, order, k0, v0);
-    for (int p = 0, lg = kSortTileLog; p < passes; ++p, lg += 2) {
-        hipLaunchKernelGGL(merge4_pass_kernel, dim3(blocks), dim3(256), 0, stream, N, lg, k0, v0,
-                           k1, v1);
+    for (int p = 0, lg = kSortTileLog; p < passes; ++p, lg += kMergeLogWays) {
+        hipLaunchKernelGGL(merge_pass_kernel<kMergeLogWays>, dim3(blocks), dim3(256), 0, stream,
+                           N, lg, k0, v0, k1, v1);
         double* kt = k0;
         k0 = k1;
         k1 = kt;

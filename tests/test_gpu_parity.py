@@ -421,7 +421,7 @@ def test_qcqp_crown_f64_staged_bit_exact(cuda, n, m, B):
 
 
 @pytest.mark.parametrize("N,S", [(1, 3), (100, 3), (8191, 21), (8192, 3), (8193, 40),
-                                 (20000, 21), (70000, 3), (200000, 17)])
+                                 (20000, 21), (65536, 21), (70000, 3), (200000, 17), (600000, 3)])
 def test_frontier_sort_equals_stable_argsort(cuda, N, S):
     """gh_frontier_sort_f64 orders the rows named by a slot list exactly as
     torch's stable descending argsort of their keys max_s (lb - thr) does
