@@ -306,7 +306,8 @@ __device__ __forceinline__ double wmax64(double v)
 // sum_k a[k * sa] * b[k] for k < len, in order, from LDS: eight entries of
 // each are loaded before their products are added, so eight loads are in
 // flight where a plain loop waits for every load in turn
-__device__ __forceinline__ double dot_lds(int len, const double *a, int sa, const double *b)
+__device__ __forceinline__ double dot_lds(int len, const double *a, int sa, const double *b,
+                                          int sb = 1)
 {
     double acc = 0.0;
     int k = 0;
@@ -315,12 +316,12 @@ __device__ __forceinline__ double dot_lds(int len, const double *a, int sa, cons
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
             av[u] = a[(k + u) * sa];
-            bv[u] = b[k + u];
+            bv[u] = b[(k + u) * sb];
         }
 #pragma unroll
         for (int u = 0; u < 8; ++u) acc += av[u] * bv[u];
     }
-    for (; k < len; ++k) acc += a[k * sa] * b[k];
+    for (; k < len; ++k) acc += a[k * sa] * b[k * sb];
     return acc;
 }
 
@@ -476,6 +477,8 @@ lp_dual_wave_kernel(int B, int m, int n, int S, const double *__restrict__ lA,
             sig = 1.0 / fmax(a, 1e-12);
         }
         double zs = lane < n ? 0.5 * zub : 0.0, ys = 0.0;
+        // the dual step's row sums split over P <= 4 lane groups of m lanes
+        const int P = min(4, 64 / m), rp = lane / m, ri = lane - rp * m;
         __syncthreads();
         for (int k = 0; k < iters; ++k) {
             // lane i publishes Dr_i ys_i; lane j: gradient and primal step
@@ -486,13 +489,16 @@ lp_dual_wave_kernel(int B, int m, int n, int S, const double *__restrict__ lA,
                 const double g = cs + wdc * dot_lds(m, W.G + lane, ld, W.y);
                 zn = fmin(fmax(zs - tau * g, 0.0), zub);
             }
-            // lane j publishes w_j Dc_j (2 zn_j - zs_j); lane i: dual step
+            // lane j publishes w_j Dc_j (2 zn_j - zs_j); dual step of row i:
+            // lanes i + p m (p < P) sum the columns j = p, p + P, ..., lane i
+            // adds the parts
             W.v[lane] = lane < n ? wdc * (2.0 * zn - zs) : 0.0;
             __syncthreads();
-            if (lane < m) {
-                const double a = dot_lds(n, W.G + lane * ld, 1, W.v);
-                ys = fmax(ys + sig * (dr * a - hs), 0.0);
-            }
+            const double part =
+                rp < P ? dot_lds((n - rp + P - 1) / P, W.G + ri * ld + rp, P, W.v + rp, P) : 0.0;
+            double a = part;
+            for (int p = 1; p < P; ++p) a += __shfl(part, ri + p * m);
+            if (lane < m) ys = fmax(ys + sig * (dr * a - hs), 0.0);
             zs = zn;
             if ((k + 1) % check_every == 0 || k + 1 == iters) {
                 const double yi = (lane < m && act_i) ? ys * dr : 0.0;

@@ -81,42 +81,41 @@ def test_lp_certificate(cuda, which):
             assert lp[b] <= F[feas, 0].min() + 1e-9
 
 
-def test_lp_certificate_cuts_with_binding_constraints(cuda):
-    """Linear objective and linear constraints (exact planes): the LP value
-    rises above the plane-only box minimum and reaches the LP optimum."""
-    from gear_amd.crown import BoundedQCQP
+# 16 lanes per box; one wavefront per box (the dual step's row sums split
+# over 4 and 3 lane groups); one workgroup per box
+@pytest.mark.parametrize("n,m", [(12, 4), (24, 6), (40, 20), (70, 5)])
+def test_lp_certificate_cuts_with_binding_constraints(cuda, n, m):
+    """Linear objective and linear constraints, so the planes are exact
+    (lA[b] = c, lbias[b] = 0): the LP value rises above the plane-only box
+    minimum and reaches the LP optimum."""
     from gear_amd.lp import qcqp_lp_bound
-    from gear_amd.problems import QCQP
-    rng = np.random.default_rng(3)
-    n, m = 12, 4
-    Q = np.zeros((m + 1, n, n))
+    rng = np.random.default_rng(3 + n)
     c = rng.standard_normal((m + 1, n))
-    d = np.zeros(m + 1)
     rhs = np.concatenate([[0.0], -np.abs(rng.standard_normal(m)) * 0.5])
-    p = QCQP(Q=Q, c=c, d=d, rhs=rhs, xl=-np.ones(n), xu=np.ones(n))
-    L, U = random_bisection_boxes(p.xl, p.xu, 16, seed=1)
-    net = BoundedQCQP(p, cuda)
+    L, U = random_bisection_boxes(-np.ones(n), np.ones(n), 16, seed=1)
     xL = torch.as_tensor(L, device=cuda)
     xU = torch.as_tensor(U, device=cuda)
-    lb, lA, lbias = net.bound(xL, xU, return_A=True)
+    lA = torch.as_tensor(np.broadcast_to(c, (len(L), m + 1, n)).copy(), device=cuda)
+    lbias = torch.zeros(len(L), m + 1, dtype=torch.float64, device=cuda)
     thr = torch.as_tensor(np.concatenate([[np.inf], rhs[1:]]), device=cuda)
     lp = qcqp_lp_bound(lA, lbias, thr, xL, xU, iters=2000).cpu().numpy()
-    lAn, lbn = lA.cpu().numpy(), lbias.cpu().numpy()
-    rises = 0
+    rises = solved = 0
     for b in range(len(L)):
-        cc, c0, G = lAn[b, 0], lbn[b, 0], lAn[b, 1:]
-        h = rhs[1:] - lbn[b, 1:]
-        res = linprog(cc, A_ub=G, b_ub=h, bounds=list(zip(L[b], U[b])), method="highs")
-        base = O.lp_dual_value(cc, c0, G, h, L[b], U[b], np.zeros(m))
+        G = c[1:]
+        h = rhs[1:]
+        res = linprog(c[0], A_ub=G, b_ub=h, bounds=list(zip(L[b], U[b])), method="highs")
+        base = O.lp_dual_value(c[0], 0.0, G, h, L[b], U[b], np.zeros(m))
         if res.status == 2:
             continue
-        opt = res.fun + c0
+        solved += 1
+        opt = res.fun
         assert lp[b] <= opt + 1e-9 * (1 + abs(opt))
         assert opt - lp[b] <= 1e-3 * (1 + abs(opt)), (b, opt, lp[b])
         if lp[b] > base + 1e-6:
             rises += 1
-    print(f"boxes where the constraints raise the certificate: {rises}/{len(L)}", flush=True)
-    assert rises >= len(L) // 2
+    print(f"n={n} m={m}: the constraints raise the certificate in {rises} of {solved} "
+          f"feasible boxes", flush=True)
+    assert solved >= 4 and rises >= solved // 2
 
 
 def test_bab_with_lp_pruning_keeps_the_optimum(cuda):
