@@ -422,15 +422,41 @@ __global__ __launch_bounds__(kSortThreads) void sort_tile_kernel(
 // Each element goes to its group's start + its index in its run + its rank
 // in each other run of the group: the length of the run's prefix whose keys
 // come before it (strictly, or tied as well for the runs before its own).
-// A workgroup's 256 elements lie in one run, so it stages every run's
-// window pivots (the last key of each window of P keys, P >= 64, at most
-// 256 windows) in LDS: a search first counts the whole windows before the
-// element there, then searches one window in global memory, the WAYS - 1
-// searches interleaved so their loads are in flight together.
-constexpr int kMergePivots = 256;
-// four-way: 1,024-row tiles to 65,536 rows in three passes (an eight-way
-// pass took 20.6 us against 11 us, two passes against three: 41 us, 33.8)
-constexpr int kMergeLogWays = 2;
+// A workgroup's 256 elements are consecutive in one run (runs >= 1,024), so
+// their ranks in another run lie between the ranks of its first and last
+// element.  Wavefront u finds those two ranks in the u-th other run (a
+// 32-ary search per half-wavefront: 32 probes, a ballot, about log32 of the
+// run length steps), the workgroup copies the keys between them into LDS
+// (contiguous loads; kSegCap keys at most, else that run is searched in
+// global memory between the two ranks), and every element finds its rank
+// by a binary search there.
+constexpr int kMergeLogWays = 2;   // four-way: 1,024-row tiles to 65,536 rows in three passes
+constexpr int kSegCap = 1024;
+
+// Number of keys of run[0 .. len) that come before x (prefix length), by a
+// 32-ary search on the calling half-wavefront (hl = lane in the half; the
+// half's 32 lanes must all call it).
+__device__ __forceinline__ int64_t rank32(const double* __restrict__ run, int64_t len,
+                                          unsigned long long x, bool tie, int hl)
+{
+    int64_t a = 0, b = len;  // keys [0, a) come before x, keys [b, len) do not
+    while (a < b) {
+        const int64_t step = (b - a + 31) / 32;
+        const int64_t p = a + (int64_t)(hl + 1) * step - 1;
+        bool before = false;
+        if (p < b) {
+            const unsigned long long y = desc_key(run[p]);
+            before = y > x || (tie && y == x);
+        }
+        const unsigned long long mask = __ballot(before);
+        const int cnt = __popcll((threadIdx.x & 32) ? (mask >> 32) : (mask & 0xffffffffull));
+        const int64_t na = a + (int64_t)cnt * step;
+        b = min(b, na + step - 1);
+        a = na;
+        if (a > b) a = b;
+    }
+    return a;
+}
 
 template <int LOGW>
 __global__ __launch_bounds__(256) void merge_pass_kernel(int64_t N, int lg,
@@ -440,61 +466,71 @@ __global__ __launch_bounds__(256) void merge_pass_kernel(int64_t N, int lg,
                                                          int64_t* __restrict__ vOut)
 {
     constexpr int WAYS = 1 << LOGW, O = WAYS - 1;
-    __shared__ unsigned long long piv[WAYS][kMergePivots];
+    static_assert(O <= 4, "one wavefront per other run");
+    __shared__ unsigned long long seg[O][kSegCap];
+    __shared__ int64_t bnd[O][2];
     const int64_t t0 = (int64_t)blockIdx.x * 256;
     const int64_t run = (int64_t)1 << lg;
-    const int64_t r = t0 >> lg;                      // the block's run (runs >= 1,024)
+    const int64_t r = t0 >> lg;                      // the block's run
     const int q = (int)(r & (WAYS - 1));
     const int64_t g0 = (r - q) << lg;                // the group's start
-    const int np = run / 64 < kMergePivots ? (int)(run / 64) : kMergePivots;
-    const int64_t P = run / np;                      // window length, a power of two
-    for (int idx = threadIdx.x; idx < WAYS * np; idx += 256) {
-        const int o = idx / np, w = idx - o * np;
-        const int64_t at = g0 + o * run + (int64_t)(w + 1) * P - 1;
-        // a window reaching past N never counts as wholly before
-        piv[o][w] = (at < N && at < g0 + (o + 1) * run) ? desc_key(kIn[at]) : 0ull;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    int64_t lo[O], len[O];
+    bool tie[O];
+#pragma unroll
+    for (int u = 0; u < O; ++u) {
+        const int o = u < q ? u : u + 1;             // the other runs, in order
+        lo[u] = g0 + o * run;
+        len[u] = lo[u] < N ? min(run, N - lo[u]) : 0;
+        tie[u] = o < q;
+    }
+    if (w < O) {
+        // ranks of the block's first (lanes 0..31) and last (32..63) element
+        const int64_t te = (lane & 32) ? min(t0 + 255, N - 1) : t0;
+        const unsigned long long xe = desc_key(kIn[te]);
+        int64_t ln = 0, lw = 0;
+        bool tw = false;
+#pragma unroll
+        for (int u = 0; u < O; ++u)
+            if (u == w) {
+                ln = len[u];
+                lw = lo[u];
+                tw = tie[u];
+            }
+        const int64_t rk = rank32(kIn + lw, ln, xe, tw, lane & 31);
+        if ((lane & 31) == 0) bnd[w][lane >> 5] = rk;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < O; ++u) {
+        const int64_t s0 = bnd[u][0], sl = bnd[u][1] - s0;
+        if (sl <= kSegCap)
+            for (int i = threadIdx.x; i < sl; i += 256) seg[u][i] = desc_key(kIn[lo[u] + s0 + i]);
     }
     __syncthreads();
     const int64_t t = t0 + threadIdx.x;
     if (t >= N) return;
     const unsigned long long x = desc_key(kIn[t]);
-    int64_t lo[O], len[O], pos[O];
-    int ou[O];
-    bool tie[O];
-#pragma unroll
-    for (int u = 0; u < O; ++u) {
-        const int o = u < q ? u : u + 1;             // the other runs, in order
-        ou[u] = o;
-        lo[u] = g0 + o * run;
-        len[u] = lo[u] < N ? min(run, N - lo[u]) : 0;
-        tie[u] = o < q;
-        pos[u] = 0;
-    }
-    // whole windows before x (pivot keys are 0 past the run: never before)
-#pragma unroll
-    for (int u = 0; u < O; ++u) {
-        int w = 0;
-        for (int step = kMergePivots; step > 0; step >>= 1) {
-            if (w + step <= np) {
-                const unsigned long long y = piv[ou[u]][w + step - 1];
-                if (y > x || (tie[u] && y == x && y != 0ull)) w += step;
-            }
-        }
-        pos[u] = (int64_t)w * P;
-    }
-    // then inside the next window, the searches interleaved
-    for (int64_t step = P; step > 0; step >>= 1) {
-#pragma unroll
-        for (int u = 0; u < O; ++u) {
-            if (pos[u] + step <= len[u] && ((pos[u] & (P - 1)) + step) <= P) {
-                const unsigned long long y = desc_key(kIn[lo[u] + pos[u] + step - 1]);
-                if (y > x || (tie[u] && y == x)) pos[u] += step;
-            }
-        }
-    }
     int64_t dst = g0 + (t - (r << lg));
 #pragma unroll
-    for (int u = 0; u < O; ++u) dst += pos[u];
+    for (int u = 0; u < O; ++u) {
+        const int64_t s0 = bnd[u][0], sl = bnd[u][1] - s0;
+        int64_t a = 0, b = sl;   // rank inside [s0, s0 + sl)
+        if (sl <= kSegCap) {
+            while (a < b) {
+                const int64_t mid = (a + b) >> 1;
+                const unsigned long long y = seg[u][mid];
+                if (y > x || (tie[u] && y == x)) a = mid + 1; else b = mid;
+            }
+        } else {
+            while (a < b) {
+                const int64_t mid = (a + b) >> 1;
+                const unsigned long long y = desc_key(kIn[lo[u] + s0 + mid]);
+                if (y > x || (tie[u] && y == x)) a = mid + 1; else b = mid;
+            }
+        }
+        dst += s0 + a;
+    }
     kOut[dst] = kIn[t];
     vOut[dst] = vIn[t];
 }
