@@ -304,38 +304,87 @@ __device__ __forceinline__ unsigned long long desc_key(double x)
     return (u >> 63) ? ~u : (u | 0x8000000000000000ull);
 }
 
+// Number of keys of run[0 .. len) that come before x (prefix length), by a
+// 32-ary search on the calling half-wavefront (hl = lane in the half; the
+// half's 32 lanes must all call it).
+__device__ __forceinline__ int64_t rank32(const double* __restrict__ run, int64_t len,
+                                          unsigned long long x, bool tie, int hl)
+{
+    int64_t a = 0, b = len;  // keys [0, a) come before x, keys [b, len) do not
+    while (a < b) {
+        const int64_t step = (b - a + 31) / 32;
+        const int64_t p = a + (int64_t)(hl + 1) * step - 1;
+        bool before = false;
+        if (p < b) {
+            const unsigned long long y = desc_key(run[p]);
+            before = y > x || (tie && y == x);
+        }
+        const unsigned long long mask = __ballot(before);
+        const int cnt = __popcll((threadIdx.x & 32) ? (mask >> 32) : (mask & 0xffffffffull));
+        const int64_t na = a + (int64_t)cnt * step;
+        b = min(b, na + step - 1);
+        a = na;
+        if (a > b) a = b;
+    }
+    return a;
+}
+
+constexpr int kDescSegCap = 2048;
+
 // Stable merge of two descending runs (A before B on ties): element i of A
 // goes to i + #{B strictly before it}, element j of B to j + #{A before or
-// tied with it}; both counts by binary search in the other run.
-__global__ void merge_desc_kernel(int64_t nA, const double* __restrict__ kA,
-                                  const int64_t* __restrict__ vA, int64_t nB,
-                                  const double* __restrict__ kB, const int64_t* __restrict__ vB,
-                                  double* __restrict__ kO, int64_t* __restrict__ vO)
+// tied with it}.  A workgroup takes 256 consecutive elements of A or of B:
+// wavefront 0 finds the counts of its first and last element in the other
+// run (rank32), the keys between them are copied into LDS (kDescSegCap at
+// most, else searched in global memory) and every element's count is a
+// binary search there.
+__global__ __launch_bounds__(256) void merge_desc_kernel(int64_t nA, const double* __restrict__ kA,
+                                                        const int64_t* __restrict__ vA, int64_t nB,
+                                                        const double* __restrict__ kB,
+                                                        const int64_t* __restrict__ vB,
+                                                        double* __restrict__ kO,
+                                                        int64_t* __restrict__ vO)
 {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nA + nB) return;
-    if (t < nA) {
-        const unsigned long long x = desc_key(kA[t]);
-        int64_t lo = 0, hi = nB;            // count B with key > x
-        while (lo < hi) {
-            const int64_t mid = (lo + hi) >> 1;
-            if (desc_key(kB[mid]) > x) lo = mid + 1;
-            else hi = mid;
-        }
-        kO[t + lo] = kA[t];
-        vO[t + lo] = vA[t];
-    } else {
-        const int64_t j = t - nA;
-        const unsigned long long x = desc_key(kB[j]);
-        int64_t lo = 0, hi = nA;            // count A with key >= x
-        while (lo < hi) {
-            const int64_t mid = (lo + hi) >> 1;
-            if (desc_key(kA[mid]) >= x) lo = mid + 1;
-            else hi = mid;
-        }
-        kO[j + lo] = kB[j];
-        vO[j + lo] = vB[j];
+    __shared__ unsigned long long seg[kDescSegCap];
+    __shared__ int64_t bnd[2];
+    const int64_t bA = (nA + 255) / 256;
+    const bool inA = (int64_t)blockIdx.x < bA;
+    const int64_t base = (inA ? (int64_t)blockIdx.x : (int64_t)blockIdx.x - bA) * 256;
+    const int64_t nS = inA ? nA : nB, nT = inA ? nB : nA;
+    const double* kS = inA ? kA : kB;
+    const double* kT = inA ? kB : kA;
+    const bool tie = !inA;   // B's elements count A's equal keys as before them
+    if (threadIdx.x < 64) {
+        // ranks of the block's first (lanes 0..31) and last (32..63) element
+        const int lane = threadIdx.x;
+        const int64_t te = (lane & 32) ? min(base + 255, nS - 1) : base;
+        const int64_t rk = rank32(kT, nT, desc_key(kS[te]), tie, lane & 31);
+        if ((lane & 31) == 0) bnd[lane >> 5] = rk;
     }
+    __syncthreads();
+    const int64_t s0 = bnd[0], sl = bnd[1] - s0;
+    if (sl <= kDescSegCap)
+        for (int i = threadIdx.x; i < sl; i += 256) seg[i] = desc_key(kT[s0 + i]);
+    __syncthreads();
+    const int64_t t = base + threadIdx.x;
+    if (t >= nS) return;
+    const unsigned long long x = desc_key(kS[t]);
+    int64_t a = 0, b = sl;
+    if (sl <= kDescSegCap) {
+        while (a < b) {
+            const int64_t mid = (a + b) >> 1;
+            const unsigned long long y = seg[mid];
+            if (y > x || (tie && y == x)) a = mid + 1; else b = mid;
+        }
+    } else {
+        while (a < b) {
+            const int64_t mid = (a + b) >> 1;
+            const unsigned long long y = desc_key(kT[s0 + mid]);
+            if (y > x || (tie && y == x)) a = mid + 1; else b = mid;
+        }
+    }
+    kO[t + s0 + a] = kS[t];
+    vO[t + s0 + a] = inA ? vA[t] : vB[t];
 }
 
 // Inverse of desc_key on its image: the key (every NaN as one NaN, -0 as
@@ -432,31 +481,6 @@ __global__ __launch_bounds__(kSortThreads) void sort_tile_kernel(
 // by a binary search there.
 constexpr int kMergeLogWays = 2;   // four-way: 1,024-row tiles to 65,536 rows in three passes
 constexpr int kSegCap = 1024;
-
-// Number of keys of run[0 .. len) that come before x (prefix length), by a
-// 32-ary search on the calling half-wavefront (hl = lane in the half; the
-// half's 32 lanes must all call it).
-__device__ __forceinline__ int64_t rank32(const double* __restrict__ run, int64_t len,
-                                          unsigned long long x, bool tie, int hl)
-{
-    int64_t a = 0, b = len;  // keys [0, a) come before x, keys [b, len) do not
-    while (a < b) {
-        const int64_t step = (b - a + 31) / 32;
-        const int64_t p = a + (int64_t)(hl + 1) * step - 1;
-        bool before = false;
-        if (p < b) {
-            const unsigned long long y = desc_key(run[p]);
-            before = y > x || (tie && y == x);
-        }
-        const unsigned long long mask = __ballot(before);
-        const int cnt = __popcll((threadIdx.x & 32) ? (mask >> 32) : (mask & 0xffffffffull));
-        const int64_t na = a + (int64_t)cnt * step;
-        b = min(b, na + step - 1);
-        a = na;
-        if (a > b) a = b;
-    }
-    return a;
-}
 
 template <int LOGW>
 __global__ __launch_bounds__(256) void merge_pass_kernel(int64_t N, int lg,
@@ -724,7 +748,8 @@ extern "C" int gh_merge_desc_f64(int64_t nA, const double* kA, const int64_t* vA
     const int64_t N = nA + nB;
     if (N == 0) return GH_OK;
     if ((nA && (!kA || !vA)) || (nB && (!kB || !vB)) || !kO || !vO) return GH_EINVAL;
-    hipLaunchKernelGGL(merge_desc_kernel, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, stream,
+    hipLaunchKernelGGL(merge_desc_kernel,
+                       dim3((unsigned)((nA + 255) / 256 + (nB + 255) / 256)), dim3(256), 0, stream,
                        nA, kA, vA, nB, kB, vB, kO, vO);
     return status();
 }

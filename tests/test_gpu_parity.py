@@ -461,8 +461,14 @@ def test_merge_desc_equals_stable_sort(cuda):
     stable descending argsort of the concatenation (ties, NaN, +-inf, -0)."""
     from gear_amd._lib import lib, ptr, stream_ptr
     rng = np.random.default_rng(3)
-    for nA, nB in ((1000, 300), (0, 50), (70, 0), (1, 1), (5000, 4096)):
-        k = np.round(rng.standard_normal(nA + nB) * 3) / 2          # many ties
+    # the last two at the loop's sizes: many ties (segments past the LDS
+    # capacity, searched in global memory) and continuous keys (in LDS)
+    for nA, nB, ties in ((1000, 300, 1), (0, 50, 1), (70, 0, 1), (1, 1, 1), (5000, 4096, 1),
+                         (400000, 65536, 1), (300000, 65536, 0)):
+        print(f"case nA={nA} nB={nB} ties={ties}", flush=True)
+        k = rng.standard_normal(nA + nB) * 3
+        if ties:
+            k = np.round(k) / 2                                      # many ties
         k[rng.random(nA + nB) < 0.03] = np.nan
         k[rng.random(nA + nB) < 0.02] = np.inf
         k[rng.random(nA + nB) < 0.02] = -np.inf
