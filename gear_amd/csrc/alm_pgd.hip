@@ -17,44 +17,17 @@
 // A restart's rows are split over the W wavefronts of its workgroup; for
 // n <= 16 and S <= 8 one wavefront holds all of a restart's rows in
 // registers (alm_pgd_small_kernel).
-#include <type_traits>
-
 #include "common.h"
 #include "gearhip_internal.h"
 
 namespace {
 
-// The value lane (lane ^ 32) / (lane ^ 16) holds: v_permlane32_swap /
-// v_permlane16_swap exchange the two halves / the odd and even 16-lane rows
-// of a register, so with both operands v each lane finds its partner's value
-// in the operand it did not keep.
-__device__ __forceinline__ double xor32(double v)
-{
-    const long long b = __double_as_longlong(v);
-    const auto lo = __builtin_amdgcn_permlane32_swap((unsigned)b, (unsigned)b, false, false);
-    const auto hi =
-        __builtin_amdgcn_permlane32_swap((unsigned)(b >> 32), (unsigned)(b >> 32), false, false);
-    const int e = (threadIdx.x & 32) ? 0 : 1;
-    return __longlong_as_double(((long long)hi[e] << 32) | (unsigned)lo[e]);
-}
-__device__ __forceinline__ double xor16(double v)
-{
-    const long long b = __double_as_longlong(v);
-    const auto lo = __builtin_amdgcn_permlane16_swap((unsigned)b, (unsigned)b, false, false);
-    const auto hi =
-        __builtin_amdgcn_permlane16_swap((unsigned)(b >> 32), (unsigned)(b >> 32), false, false);
-    const int e = (threadIdx.x & 16) ? 0 : 1;
-    return __longlong_as_double(((long long)hi[e] << 32) | (unsigned)lo[e]);
-}
-// v of another lane of the same 16-lane row by a DPP control
-template <int CTRL>
-__device__ __forceinline__ double dpp(double v)
-{
-    const long long b = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xf, 0xf, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xf, 0xf, false);
-    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
-}
+using gh::xor32_f64;
+using gh::xor16_f64;
+using gh::dpp_f64;
+using gh::bcast16_f64;
+using gh::for_lanes16;
+
 constexpr int kRowRor8 = 0x128, kRowRor4 = 0x124;   // lane (i + 8) / (i + 4) mod 16
 constexpr int kQuadXor1 = 0xB1, kQuadXor2 = 0x4E;   // quad_perm [1,0,3,2] / [2,3,0,1]
 
@@ -66,12 +39,12 @@ constexpr int kQuadXor1 = 0xB1, kQuadXor2 = 0x4E;   // quad_perm [1,0,3,2] / [2,
 template <class Op>
 __device__ __forceinline__ double wreduce(double v, Op op)
 {
-    v = op(v, xor32(v));
-    v = op(v, xor16(v));
-    v = op(v, dpp<kRowRor8>(v));
-    v = op(v, dpp<kRowRor4>(v));
-    v = op(v, dpp<kQuadXor2>(v));
-    v = op(v, dpp<kQuadXor1>(v));
+    v = op(v, xor32_f64(v));
+    v = op(v, xor16_f64(v));
+    v = op(v, dpp_f64<kRowRor8>(v));
+    v = op(v, dpp_f64<kRowRor4>(v));
+    v = op(v, dpp_f64<kQuadXor2>(v));
+    v = op(v, dpp_f64<kQuadXor1>(v));
     return v;
 }
 __device__ __forceinline__ double wsum(double v)
@@ -327,40 +300,22 @@ alm_pgd_kernel(int N, int n, int S, const double *__restrict__ Qg, const double 
 // per-step scalars and needs no barrier.  Same iteration as above; only the
 // order of the sums differs.
 
-template <int I>
-__device__ __forceinline__ double bc16(double v)
-{
-    return __builtin_amdgcn_update_dpp(0.0, v, 0x150 + I, 0xf, 0xf, false);  // v_mov_b64_dpp
-}
-
-// f(integral_constant<int, J>) for J = 0 .. 15 while J < count (count uniform)
-template <int J = 0, class F>
-__device__ __forceinline__ void for_cols(int count, F &&f)
-{
-    if constexpr (J < 16) {
-        if (J < count) {
-            f(std::integral_constant<int, J>{});
-            for_cols<J + 1>(count, f);
-        }
-    }
-}
-
 // butterfly over the 16 lanes of a row (o = 8, 4, 2, 1), as in wreduce
 template <class Op>
 __device__ __forceinline__ double rreduce(double v, Op op)
 {
-    v = op(v, dpp<kRowRor8>(v));
-    v = op(v, dpp<kRowRor4>(v));
-    v = op(v, dpp<kQuadXor2>(v));
-    v = op(v, dpp<kQuadXor1>(v));
+    v = op(v, dpp_f64<kRowRor8>(v));
+    v = op(v, dpp_f64<kRowRor4>(v));
+    v = op(v, dpp_f64<kQuadXor2>(v));
+    v = op(v, dpp_f64<kQuadXor1>(v));
     return v;
 }
 // across the four row slots (o = 32, 16)
 template <class Op>
 __device__ __forceinline__ double sreduce(double v, Op op)
 {
-    v = op(v, xor32(v));
-    v = op(v, xor16(v));
+    v = op(v, xor32_f64(v));
+    v = op(v, xor16_f64(v));
     return v;
 }
 
@@ -412,9 +367,9 @@ alm_pgd_small_kernel(int N, int n, int S, const double *__restrict__ Qs,
     auto eval = [&](double (&y)[G], double (&f)[G]) {
 #pragma unroll
         for (int g = 0; g < G; ++g) y[g] = 0.0;
-        for_cols(n, [&](auto J) {
+        for_lanes16(n, [&](auto J) {
             constexpr int j = decltype(J)::value;
-            const double xj = bc16<j>(x);
+            const double xj = bcast16_f64<j>(x);
 #pragma unroll
             for (int g = 0; g < G; ++g) y[g] = fma(q[g][j], xj, y[g]);
         });

@@ -3,6 +3,7 @@
 // calls, so each arithmetic chain has exactly one meaning.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <type_traits>
 #include <stdint.h>
 
 #define GH_LANES 16   // lanes that cooperate on one box
@@ -93,6 +94,27 @@ __device__ __forceinline__ double xor32_f64(double v)
     r.x = (int)(low_half ? lo[1] : lo[0]);
     r.y = (int)(low_half ? hi[1] : hi[0]);
     return __builtin_bit_cast(double, r);
+}
+
+// Lane I of this lane's 16-lane row (DPP row_newbcast, one v_mov_b64_dpp).
+template <int I>
+__device__ __forceinline__ double bcast16_f64(double v)
+{
+    return __builtin_amdgcn_update_dpp(0.0, v, 0x150 + I, 0xF, 0xF, false);
+}
+
+// f(integral_constant<int, I>) for I = 0 .. 15 while I < count (count the
+// same in every lane): loops whose lane index must be a compile-time
+// constant (DPP controls, register arrays).
+template <int I = 0, class F>
+__device__ __forceinline__ void for_lanes16(int count, F &&f)
+{
+    if constexpr (I < 16) {
+        if (I < count) {
+            f(std::integral_constant<int, I>{});
+            for_lanes16<I + 1>(count, f);
+        }
+    }
 }
 
 // The red16 tree when partial l = q + 4v lives in lane row q (lane / 16)

@@ -24,12 +24,13 @@
 // (lp_dual_wave_kernel).  Larger: one
 // workgroup (256 threads) per box; the per-box vectors live in dynamic LDS
 // sized by (n, m), and G too when m n <= kGlds.
-#include <type_traits>
-
 #include "common.h"
 #include "gearhip_internal.h"
 
 namespace {
+
+using gh::bcast16_f64;
+using gh::for_lanes16;
 
 constexpr double U = 1.1102230246251565404236316680908203125e-16;  // 2^-53
 constexpr double ETA = 4.9406564584124654e-324;                     // 2^-1074
@@ -534,25 +535,6 @@ lp_dual_wave_kernel(int B, int m, int n, int S, const double *__restrict__ lA,
 constexpr int kG = 16;              // lanes per box
 constexpr int kGThreads = 256;      // 16 boxes per workgroup
 
-// lane I of this lane's 16-lane row
-template <int I>
-__device__ __forceinline__ double bc16(double v)
-{
-    return __builtin_amdgcn_update_dpp(0.0, v, 0x150 + I, 0xf, 0xf, false);  // v_mov_b64_dpp
-}
-
-// f(integral_constant<int, I>) for I = 0 .. 15 while I < count (count uniform)
-template <int I = 0, class F>
-__device__ __forceinline__ void for_lanes(int count, F &&f)
-{
-    if constexpr (I < kG) {
-        if (I < count) {
-            f(std::integral_constant<int, I>{});
-            for_lanes<I + 1>(count, f);
-        }
-    }
-}
-
 __device__ __forceinline__ double gsum16(double v)
 {
     for (int o = kG / 2; o >= 1; o >>= 1) v += __shfl_xor(v, o);
@@ -568,8 +550,8 @@ __device__ double rigorous_dual_g16(int m, int n, const double (&gc)[kG], double
     const int s = threadIdx.x & (kG - 1);
     const double infl = 1.0 + gamma_k(2.0 * (m + n) + 16.0);
     double r = 0.0, rho = 0.0, k = 0.0;
-    for_lanes(m, [&](auto I) {
-        const double yi = bc16<decltype(I)::value>(ys_eval), g = gc[decltype(I)::value];
+    for_lanes16(m, [&](auto I) {
+        const double yi = bcast16_f64<decltype(I)::value>(ys_eval), g = gc[decltype(I)::value];
         if (yi != 0.0 && g != 0.0) {
             const double p = g * yi;
             r += p;
@@ -655,10 +637,10 @@ lp_dual_g16_kernel(int B, int m, int n, int S, const double *__restrict__ lA,
     const double actd = act_i ? 1.0 : 0.0;
 #pragma unroll
     for (int i = 0; i < kG; ++i) gc[i] = 0.0;
-    for_lanes(m, [&](auto I) {
+    for_lanes16(m, [&](auto I) {
         constexpr int i = decltype(I)::value;
         const double g = col ? Gb[(size_t)i * n + s] : 0.0;
-        gc[i] = bc16<i>(actd) != 0.0 ? g : 0.0;
+        gc[i] = bcast16_f64<i>(actd) != 0.0 ? g : 0.0;
     });
     double best = rigorous_dual_g16(m, n, gc, 0.0, c0, cj, lj, uj, hi, act_i);
     double ybest = 0.0;
@@ -669,20 +651,20 @@ lp_dual_g16_kernel(int B, int m, int n, int S, const double *__restrict__ lA,
         double dc = 1.0, dr = 1.0;
         // hs = h - G l
         double gl = 0.0;
-        for_lanes(n, [&](auto J) {
-            gl += gr[decltype(J)::value] * bc16<decltype(J)::value>(lj);
+        for_lanes16(n, [&](auto J) {
+            gl += gr[decltype(J)::value] * bcast16_f64<decltype(J)::value>(lj);
         });
         double hs = row ? hi - gl : 0.0;
         for (int it = 0; it < ruiz_iters; ++it) {
             const double v = w * dc, y = dr;
             double rn = 0.0, cn = 0.0;
-            for_lanes(n, [&](auto J) {
+            for_lanes16(n, [&](auto J) {
                 constexpr int j = decltype(J)::value;
-                rn = fmax(rn, fabs(gr[j] * bc16<j>(v) * dr));
+                rn = fmax(rn, fabs(gr[j] * bcast16_f64<j>(v) * dr));
             });
-            for_lanes(m, [&](auto I) {
+            for_lanes16(m, [&](auto I) {
                 constexpr int i = decltype(I)::value;
-                cn = fmax(cn, fabs(gc[i] * bc16<i>(y) * w * dc));
+                cn = fmax(cn, fabs(gc[i] * bcast16_f64<i>(y) * w * dc));
             });
             if (row) dr = dr / (rn > 0 ? sqrt(rn) : 1.0);
             if (col) dc = dc / (cn > 0 ? sqrt(cn) : 1.0);
@@ -694,13 +676,13 @@ lp_dual_g16_kernel(int B, int m, int n, int S, const double *__restrict__ lA,
         double tau = 0.0, sig = 0.0;
         {
             double a = 0.0, q = 0.0;
-            for_lanes(m, [&](auto I) {
+            for_lanes16(m, [&](auto I) {
                 constexpr int i = decltype(I)::value;
-                a += fabs(gc[i] * bc16<i>(drr) * wdc);
+                a += fabs(gc[i] * bcast16_f64<i>(drr) * wdc);
             });
-            for_lanes(n, [&](auto J) {
+            for_lanes16(n, [&](auto J) {
                 constexpr int j = decltype(J)::value;
-                q += fabs(gr[j] * dr * bc16<j>(wdc));
+                q += fabs(gr[j] * dr * bcast16_f64<j>(wdc));
             });
             if (col) tau = 1.0 / fmax(a, 1e-12);
             if (row) sig = 1.0 / fmax(q, 1e-12);
@@ -710,18 +692,18 @@ lp_dual_g16_kernel(int B, int m, int n, int S, const double *__restrict__ lA,
             // gradient of column s from the rows' Dr_i ys_i; primal step
             const double yd = row ? dr * ys : 0.0;
             double g = 0.0;
-            for_lanes(m, [&](auto I) {
+            for_lanes16(m, [&](auto I) {
                 constexpr int i = decltype(I)::value;
-                g = fma(gc[i], bc16<i>(yd), g);
+                g = fma(gc[i], bcast16_f64<i>(yd), g);
             });
             g = cs + wdc * g;
             const double zn = col ? fmin(fmax(zs - tau * g, 0.0), zub) : 0.0;
             // dual step of row s from the columns' w_j Dc_j (2 zn_j - zs_j)
             const double v = col ? wdc * (2.0 * zn - zs) : 0.0;
             double a = 0.0;
-            for_lanes(n, [&](auto J) {
+            for_lanes16(n, [&](auto J) {
                 constexpr int j = decltype(J)::value;
-                a = fma(gr[j], bc16<j>(v), a);
+                a = fma(gr[j], bcast16_f64<j>(v), a);
             });
             if (row) ys = fmax(ys + sig * (dr * a - hs), 0.0);
             zs = zn;
