@@ -293,15 +293,13 @@ lp_dual_kernel(int B, int m, int n, int S, const double *__restrict__ lA,
 // and the scalings live in the wavefront's LDS, so an iteration needs no
 // workgroup barrier.  Same iteration and same rigorous evaluation as above.
 
+// sum over the wavefront as the xor butterfly (o = 32, 16, 8, 4, 2, 1) on
+// permlane swaps and DPP: every lane ends with the same value
 __device__ __forceinline__ double wsum64(double v)
 {
-    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o);
-    return v;
-}
-__device__ __forceinline__ double wmax64(double v)
-{
-    for (int o = 32; o >= 1; o >>= 1) v = fmax(v, __shfl_xor(v, o));
-    return v;
+    v = v + gh::xor32_f64(v);
+    v = v + gh::xor16_f64(v);
+    return gh::tree16(v);
 }
 
 // sum_k a[k * sa] * b[k] for k < len, in order, from LDS: eight entries of
@@ -535,11 +533,8 @@ lp_dual_wave_kernel(int B, int m, int n, int S, const double *__restrict__ lA,
 constexpr int kG = 16;              // lanes per box
 constexpr int kGThreads = 256;      // 16 boxes per workgroup
 
-__device__ __forceinline__ double gsum16(double v)
-{
-    for (int o = kG / 2; o >= 1; o >>= 1) v += __shfl_xor(v, o);
-    return v;
-}
+// sum over the box's 16 lanes (xor butterfly o = 8, 4, 2, 1 on DPP)
+__device__ __forceinline__ double gsum16(double v) { return gh::tree16(v); }
 
 // rigorous D(y) of the group's LP: lane s holds column s of G (gc), y_s (row
 // lanes) and its column's data
