@@ -52,21 +52,25 @@ __device__ __forceinline__ float up_f32(double v)
     return ((double)f < v) ? nextafterf(f, INFINITY) : f;
 }
 
+// Wavefront reductions as the xor butterfly (o = 32, 16, 8, 4, 2, 1) on
+// permlane swaps and DPP (no LDS); every lane ends with the same value.
 __device__ __forceinline__ double wave_sum(double v)
 {
-    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o);
-    return v;
+    v = v + gh::xor32_f64(v);
+    v = v + gh::xor16_f64(v);
+    return gh::tree16(v);
 }
 __device__ __forceinline__ double wave_max(double v)
 {
-    for (int o = 32; o >= 1; o >>= 1) v = fmax(v, __shfl_xor(v, o));
-    return v;
+    v = fmax(v, gh::xor32_f64(v));
+    v = fmax(v, gh::xor16_f64(v));
+    v = fmax(v, gh::dpp_f64<0x128>(v));   // row_ror:8 (lane ^ 8)
+    v = fmax(v, gh::dpp_f64<0x124>(v));   // row_ror:4 (a lane holding lane ^ 4's value)
+    v = fmax(v, gh::dpp_f64<0x4E>(v));    // quad_perm [2,3,0,1]
+    return fmax(v, gh::dpp_f64<0xB1>(v)); // quad_perm [1,0,3,2]
 }
-__device__ __forceinline__ int wave_or(int v)
-{
-    for (int o = 32; o >= 1; o >>= 1) v |= __shfl_xor(v, o);
-    return v;
-}
+// 1 when any lane's flag (0 or 1) is set
+__device__ __forceinline__ int wave_any(int v) { return __ballot(v != 0) != 0ull ? 1 : 0; }
 
 // Q32 = fl32(Q), absQ = |Q32|, rowabs[s][i] >= sum_j |Q32[s][i][j]|.
 __global__ void __launch_bounds__(64)
@@ -207,7 +211,7 @@ large_relax_kernel(int B, int n, int S, const float *__restrict__ C, const float
     ga = wave_sum(ga);
     bm = wave_sum(bm);
     am = wave_sum(am);
-    tie = wave_or(tie);
+    tie = wave_any(tie);
     if (threadIdx.x == 0) {
         double *o = sums + sb * 4;
         o[0] = gs;
