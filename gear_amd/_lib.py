@@ -56,8 +56,8 @@ SIGNATURES = {
     "gh_split_append_workspace_bytes": [_I],
     "gh_pipe_probe": [_I, _I, _I, _I, _P, _P],
     "gh_frontier_sort_f64": [_L, _I, _P, _P, _P, _P, _P, _P, _P, _P],
-    "gh_split_append_f64": [_I, _I, _I, _I, _P, _P, _P, _P, _I, _P, _P, _P, _P, _P, _P, _L, _L,
-                            _P, _L, _P, _P, _P, _P],
+    "gh_split_append_f64": [_I, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _L, _L, _P,
+                            _L, _P, _P, _P, _P],
     "gh_qcqp_crown_rows_f64": [_I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I, _P],
     "gh_qcqp_crown_rows_f32s": [_I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I, _P],
     "gh_dag_plan_create": [_I, _P, ctypes.POINTER(_P)],

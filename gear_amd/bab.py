@@ -73,14 +73,14 @@ def reordered_step(f, bound, thr, nb, n, S, planes=True, lp=None, clip=False,
     accepts a row index, so without clip and LP the picked domains are
     bound and split where the frontier stores them (no gather)."""
     if in_place and not clip and lp is None and f.in_place_room(nb):
-        idx, _ = f.pick_rows(nb, bounds=False)
+        idx, pkey = f.pick_rows(nb)
         L, U = f.storage()
         if planes:
             lb, lA, lbias = bound(L, U, return_A=True, plane_rows=1, rows=idx)
         else:
             lb, lA, lbias = bound(L, U, rows=idx), None, None
-        # never below the stored keys: split_append takes the maximum itself
-        return f.split_append(lb, L, U, thr, lA, lbias, sync=sync, rows=idx, floor_stored=True)
+        lb = torch.maximum(lb, pkey)
+        return f.split_append(lb, L, U, thr, lA, lbias, sync=sync, rows=idx)
     pL, pU, pkey = f.pick_out(nb)
     if planes:
         # the LP and the cuts read every row's plane, the child keys the

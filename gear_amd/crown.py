@@ -455,22 +455,19 @@ class DeviceFrontier:
                                        stream_ptr()), "gh_gather_rows_f64")
         return pL, pU, plb
 
-    def pick_rows(self, batch, bounds=True):
+    def pick_rows(self, batch):
         """Slot ids and bound rows (idx, lb) of the last `batch` logical rows,
         which leave the frontier as in pick_out; their boxes are read in place
         (rows idx of the frontier's storage, self.storage()) by the bound and
         split_append, which avoids gathering them.  idx is a view of the
         order: the next append overwrites it (split_append reads each entry
-        before it writes the same slot).  bounds=False: lb is None (for
-        split_append(floor_stored=True), which reads the stored rows)."""
+        before it writes the same slot)."""
         nb = min(batch, self.count)
         s = self.count - nb
         self.count = s
         self._sorted = min(self._sorted, s)
         idx = self.order[s:s + nb]
         self._picked = idx
-        if not bounds:
-            return idx, None
         plb = torch.empty(nb, self.S, dtype=torch.float64, device=self.device)
         check(lib().gh_gather_rows_f64(nb, ptr(idx), self.S, ptr(self._lb), ptr(plb), 0, None,
                                        None, 0, None, None, stream_ptr()), "gh_gather_rows_f64")
@@ -495,8 +492,7 @@ class DeviceFrontier:
             self._ws = torch.empty(need, dtype=torch.uint8, device=self.device)
         return self._ws
 
-    def split_append(self, plb, pL, pU, thr, lA=None, lbias=None, sync=True, rows=None,
-                     floor_stored=False):
+    def split_append(self, plb, pL, pU, thr, lA=None, lbias=None, sync=True, rows=None):
         """Widest branching, bisection, child keys from the parents'
         objective planes, prune and append for K parents in one call
         (gh_split_append_f64): the frontier branch_split_keys followed by
@@ -504,12 +500,8 @@ class DeviceFrontier:
         the children.  lA (K, R, n) / lbias (K, R) may be None (the keys are
         the parents' bounds).  rows (K,) int64: parent p's box is row rows[p]
         of pL, pU (e.g. this frontier's own storage after pick_rows; the
-        append must then not compact it, see in_place_room).  floor_stored
-        (with rows): plb is first raised in place to the rows' stored bound
-        rows, torch.maximum(plb, stored) (never below the key a picked domain
-        was stored with), which spares pick_rows' gather and the maximum.
-        Returns the survivor count when sync is set, else None (as
-        filter_append)."""
+        append must then not compact it, see in_place_room).  Returns the
+        survivor count when sync is set, else None (as filter_append)."""
         if self._pay is not None:
             raise ValueError("a frontier with a payload appends through filter_append")
         K = plb.shape[0] if rows is not None else pL.shape[0]
@@ -529,9 +521,6 @@ class DeviceFrontier:
             raise ValueError("parents must be (K, n) boxes")
         if plb.shape != (K, self.S):
             raise ValueError("parent bounds must be (K, S)")
-        if floor_stored and (rows is None or not plb.is_contiguous()
-                             or pL.data_ptr() != self._L.data_ptr()):
-            raise ValueError("floor_stored raises contiguous plb to this frontier's rows")
         if self.count + B > self.cap:
             raise RuntimeError("frontier capacity exceeded")
         self._room(B)
@@ -539,7 +528,7 @@ class DeviceFrontier:
         ws = self.workspace(B, lib().gh_split_append_workspace_bytes(K))
         check(lib().gh_split_append_f64(K, self.n, self.S, R, ptr(pL.contiguous()),
                                         ptr(pU.contiguous()), ptr(rows), ptr(plb.contiguous()),
-                                        int(bool(floor_stored)), ptr(lA),
+                                        ptr(lA),
                                         ptr(lbias), ptr(thr.contiguous()), ptr(self._L),
                                         ptr(self._U), ptr(self._lb), self.hw, self.phys,
                                         ptr(self.order[self.count:self.count + B]), self.dead,

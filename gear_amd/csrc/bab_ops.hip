@@ -387,14 +387,6 @@ __global__ __launch_bounds__(256) void merge_desc_kernel(int64_t nA, const doubl
     vO[t + s0 + a] = inA ? vA[t] : vB[t];
 }
 
-// The larger of a and b as torch.maximum gives it: a NaN operand is the
-// result (a first), otherwise the hardware maximum (v_max_f64, which puts
-// -0 below +0) of two numbers.
-__device__ __forceinline__ double torch_max(double a, double b)
-{
-    return isnan(a) ? a : (isnan(b) ? b : fmax(a, b));
-}
-
 // Inverse of desc_key on its image: the key (every NaN as one NaN, -0 as
 // +0 -- the order desc_key gives is unchanged).
 __device__ __forceinline__ double from_desc_key(unsigned long long dk)
@@ -1063,7 +1055,7 @@ __device__ __forceinline__ void split_block_counts(int nw, const int* kc0, const
 __global__ void __launch_bounds__(1024)
 split_keep_wave_kernel(int K, int n, int S, int R, const double *__restrict__ pL,
                        const double *__restrict__ pU, const int64_t *__restrict__ prows,
-                       double *__restrict__ plb, const double *__restrict__ floor_rows,
+                  const double *__restrict__ plb,
                        const double *__restrict__ lA, const double *__restrict__ lbias,
                        const double *__restrict__ thr, double *__restrict__ ckey,
                        int32_t *__restrict__ sidx, int32_t *__restrict__ keep,
@@ -1078,17 +1070,10 @@ split_keep_wave_kernel(int K, int n, int S, int R, const double *__restrict__ pL
     if (p < K) {
         const size_t br = prows ? (size_t)prows[p] : (size_t)p;   // the parent's box row
         const double *l = pL + br * n, *u = pU + br * n;
-        double *prow = plb + (size_t)p * S;
+        const double *prow = plb + (size_t)p * S;
         const double *a = lA ? lA + (size_t)p * R * n : nullptr;
         const double b = lA ? lbias[(size_t)p * R] : 0.0;
-        double p0 = prow[0];
-        if (floor_rows) {
-            // never below the stored row; each lane rewrites the entries it
-            // reads again (s = lane mod 64), and p0 is the same either way
-            const double *fr = floor_rows + br * S;
-            for (int s = lane; s < S; s += 64) prow[s] = torch_max(prow[s], fr[s]);
-            p0 = torch_max(p0, fr[0]);
-        }
+        const double p0 = prow[0];
         const int idx = first_widest(n, lane, l, u);
         for (int c = 0; c < 2; ++c) {
             int fl;
@@ -1129,7 +1114,7 @@ __device__ __forceinline__ bool group_any(bool x, int lane)
 __global__ void __launch_bounds__(256)
 split_keep16_kernel(int K, int n, int S, int R, const double *__restrict__ pL,
                     const double *__restrict__ pU, const int64_t *__restrict__ prows,
-                    double *__restrict__ plb, const double *__restrict__ floor_rows,
+                  const double *__restrict__ plb,
                     const double *__restrict__ lA, const double *__restrict__ lbias,
                     const double *__restrict__ thr, double *__restrict__ ckey,
                     int32_t *__restrict__ sidx, int32_t *__restrict__ keep,
@@ -1149,9 +1134,8 @@ split_keep16_kernel(int K, int n, int S, int R, const double *__restrict__ pL,
     if (valid) {
         const size_t br = prows ? (size_t)prows[p] : (size_t)p;   // the parent's box row
         const double *l = pL + br * n, *u = pU + br * n;
-        double *prow = plb + (size_t)p * S;
+        const double *prow = plb + (size_t)p * S;
         const double *a = planes ? lA + (size_t)p * R * n : nullptr;
-        const double *fr = floor_rows ? floor_rows + br * S : nullptr;
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             const int i = gl + 16 * k;
@@ -1163,11 +1147,6 @@ split_keep16_kernel(int K, int n, int S, int R, const double *__restrict__ pL,
             if (i < S) {
                 pr[k] = prow[i];
                 th[k] = thr[i];
-                if (fr) {
-                    // never below the stored row; the copy pass reads it back
-                    pr[k] = torch_max(pr[k], fr[i]);
-                    prow[i] = pr[k];
-                }
             }
         }
         if (planes) b = lbias[(size_t)p * R];
@@ -1422,14 +1401,13 @@ extern "C" size_t gh_split_append_workspace_bytes(int K)
 }
 
 extern "C" int gh_split_append_f64(int K, int n, int S, int R, const double *pL, const double *pU,
-                                   const int64_t *prows, double *plb, int floor_stored,
-                                   const double *lA, const double *lbias, const double *thr,
-                                   double *FL, double *FU, double *Flb, int64_t base, int64_t cap,
-                                   int64_t *order_out, int64_t dead, int64_t *total,
-                                   double *disc_min, void *ws, hipStream_t stream)
+                                   const int64_t *prows, const double *plb, const double *lA,
+                                   const double *lbias, const double *thr, double *FL, double *FU,
+                                   double *Flb, int64_t base, int64_t cap, int64_t *order_out,
+                                   int64_t dead, int64_t *total, double *disc_min, void *ws,
+                                   hipStream_t stream)
 {
     if (K < 0 || n < 1 || S < 1 || base < 0 || !total || !disc_min) return GH_EINVAL;
-    if (floor_stored && !prows) return GH_EINVAL;
     if (lA && (R < 1 || R > S || !lbias)) return GH_EINVAL;
     if (K == 0) {
         hipLaunchKernelGGL(set_i64_kernel, dim3(1), dim3(1), 0, stream, total, (int64_t)0);
@@ -1442,17 +1420,12 @@ extern "C" int gh_split_append_f64(int K, int n, int S, int R, const double *pL,
     int32_t *sidx = (int32_t *)(ckey + 2 * (size_t)K);
     int32_t *keep = sidx + K;
     int32_t *cnt = keep + 2 * (size_t)K;
-    // the stored rows of the parents: Flb at their slot ids (rows below
-    // base; the children go to rows base and up)
-    const double *floor_rows = floor_stored ? Flb : nullptr;
     if (n <= 64 && S <= 64)
         hipLaunchKernelGGL(split_keep16_kernel, dim3(nblk), dim3(256), 0, stream, K, n, S, R, pL,
-                           pU, prows, plb, floor_rows, lA, lbias, thr, ckey, sidx, keep, cnt,
-                           disc_min);
+                           pU, prows, plb, lA, lbias, thr, ckey, sidx, keep, cnt, disc_min);
     else
         hipLaunchKernelGGL(split_keep_wave_kernel, dim3(nblk), dim3(1024), 0, stream, K, n, S, R,
-                           pL, pU, prows, plb, floor_rows, lA, lbias, thr, ckey, sidx, keep, cnt,
-                           disc_min);
+                           pL, pU, prows, plb, lA, lbias, thr, ckey, sidx, keep, cnt, disc_min);
     hipLaunchKernelGGL(filter_scan_kernel, dim3(1), dim3(1024), 0, stream, 2 * nblk, cnt, total);
     if (n <= 64 && S <= 64)
         hipLaunchKernelGGL(split_copy16_kernel, dim3(nblk), dim3(256), 0, stream, K, n, S, pL, pU,

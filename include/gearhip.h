@@ -349,17 +349,12 @@ size_t gh_split_append_workspace_bytes(int K);
  * from its parent.  Parent p's box is row prows[p] of pL, pU (row p when
  * prows is NULL), so the parents can be read in place in the frontier as
  * long as the append does not overwrite them; plb, lA, lbias are indexed
- * by p.  floor_stored (needs prows): the parents' bound rows are first
- * raised to their stored rows, plb[p] = torch.maximum(plb[p],
- * Flb[prows[p]]) in place (compare_with_old_bounds, bounding.py:361-363),
- * so a caller that bounds picked domains in place needs neither the
- * stored rows gathered nor a separate maximum.  ws:
- * gh_split_append_workspace_bytes(K) bytes.  The caller guarantees
- * base + 2K <= cap.  Frontiers whose domains carry a payload use the
- * separate calls.
+ * by p.  ws: gh_split_append_workspace_bytes(K) bytes.  The caller
+ * guarantees base + 2K <= cap.  Frontiers whose domains carry a
+ * payload use the separate calls.
  */
 int gh_split_append_f64(int K, int n, int S, int R, const double *pL, const double *pU,
-                        const int64_t *prows, double *plb, int floor_stored, const double *lA,
+                        const int64_t *prows, const double *plb, const double *lA,
                         const double *lbias,
                         const double *thr, double *FL, double *FU, double *Flb, int64_t base,
                         int64_t cap, int64_t *order_out, int64_t dead, int64_t *total,

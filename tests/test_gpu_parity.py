@@ -338,64 +338,6 @@ def test_split_append_equals_separate_calls(cuda, K, n, m, planes):
     assert a.compact_order() == b.compact_order() == 13 + kept
 
 
-# n, S <= 64 (16 lanes per parent) and S > 64 (a wavefront per parent)
-@pytest.mark.parametrize("K,n,m", [(300, 12, 4), (257, 50, 20), (70, 8, 70)])
-def test_split_append_floor_equals_maximum(cuda, K, n, m):
-    """split_append(floor_stored=True) on picked rows raises the parents'
-    bounds to their stored rows exactly as torch.maximum does (NaN on either
-    side, +0 against -0 both ways, infinities) and leaves the frontier
-    bit for bit as the gathered stored rows, torch.maximum and
-    split_append without the floor do."""
-    from gear_amd.crown import DeviceFrontier
-    print(f"case K={K} n={n} m={m}", flush=True)
-    rng = np.random.default_rng(K + n)
-    S = m + 1
-    L = -rng.random((K, n)) - 0.1
-    U = rng.random((K, n)) + 0.1
-    stored = rng.standard_normal((K, S))
-    new = stored + rng.standard_normal((K, S)) * 0.5
-    special = [(-0.0, 0.0), (0.0, -0.0), (np.nan, 1.0), (1.0, np.nan), (np.inf, -np.inf),
-               (-np.inf, 2.0), (3.0, 3.0), (np.nan, np.nan), (-np.inf, -np.inf)]
-    for j, (a, b) in enumerate(special):
-        for r in range(j, K, len(special)):
-            c = (r * 7 + j) % S
-            new[r, c], stored[r, c] = a, b
-    lA = rng.standard_normal((K, 1, n))
-    lbias = rng.standard_normal((K, 1))
-    thr = np.full(S, np.inf)
-    thr[0] = float(np.nanmedian(stored[:, 0]))
-    if S > 1:
-        thr[1] = 1.5
-    d = lambda a: _dev(a, cuda)
-    fronts, raised = [], []
-    for floor in (True, False):
-        f = DeviceFrontier(n, S, 4 * K + 64, device=cuda, slack=4 * K)
-        f.push(d(L), d(U), d(stored))
-        idx, pk = f.pick_rows(K, bounds=not floor)
-        FL, FU = f.storage()
-        plb = d(new)
-        if floor:
-            f.split_append(plb, FL, FU, d(thr), d(lA), d(lbias), sync=False, rows=idx,
-                           floor_stored=True)
-        else:
-            plb = torch.maximum(plb, pk)
-            f.split_append(plb, FL, FU, d(thr), d(lA), d(lbias), sync=False, rows=idx)
-        torch.cuda.synchronize()
-        raised.append(plb.cpu().numpy())
-        fronts.append(f)
-    assert _bits_equal(raised[0], raised[1])
-    a, b = fronts
-    kept = int(a.total.item())
-    assert kept == int(b.total.item())
-    assert a.count == b.count and a.hw == b.hw
-    assert torch.equal(a.order[:a.count], b.order[:b.count])
-    rows = K + kept           # the parents' rows and the survivors (past them: unwritten)
-    assert _bits_equal(a._L[:rows].cpu().numpy(), b._L[:rows].cpu().numpy())
-    assert _bits_equal(a._U[:rows].cpu().numpy(), b._U[:rows].cpu().numpy())
-    assert _bits_equal(a._lb[:rows].cpu().numpy(), b._lb[:rows].cpu().numpy())
-    assert _bits_equal(a.disc_min.cpu().numpy(), b.disc_min.cpu().numpy())
-
-
 @pytest.mark.parametrize("precision", ["fp64", "fp32s"])
 def test_bound_rows_equals_gathered(cuda, precision):
     """Bounding boxes in place through a row index (gh_qcqp_crown_rows_*)
