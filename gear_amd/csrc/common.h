@@ -103,18 +103,23 @@ __device__ __forceinline__ double bcast16_f64(double v)
     return __builtin_amdgcn_update_dpp(0.0, v, 0x150 + I, 0xF, 0xF, false);
 }
 
-// f(integral_constant<int, I>) for I = 0 .. 15 while I < count (count the
-// same in every lane): loops whose lane index must be a compile-time
-// constant (DPP controls, register arrays).
-template <int I = 0, class F>
-__device__ __forceinline__ void for_lanes16(int count, F &&f)
+// f(integral_constant<int, I>) for I = 0 .. LIMIT - 1 while I < count (count
+// the same in every lane): loops whose lane index must be a compile-time
+// constant (DPP controls, register arrays of LIMIT entries).
+template <int LIMIT, int I = 0, class F>
+__device__ __forceinline__ void for_lanes_upto(int count, F &&f)
 {
-    if constexpr (I < 16) {
+    if constexpr (I < LIMIT) {
         if (I < count) {
             f(std::integral_constant<int, I>{});
-            for_lanes16<I + 1>(count, f);
+            for_lanes_upto<LIMIT, I + 1>(count, f);
         }
     }
+}
+template <class F>
+__device__ __forceinline__ void for_lanes16(int count, F &&f)
+{
+    for_lanes_upto<16>(count, f);
 }
 
 // The red16 tree when partial l = q + 4v lives in lane row q (lane / 16)

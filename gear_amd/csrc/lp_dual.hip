@@ -30,7 +30,6 @@
 namespace {
 
 using gh::bcast16_f64;
-using gh::for_lanes16;
 
 constexpr double U = 1.1102230246251565404236316680908203125e-16;  // 2^-53
 constexpr double ETA = 4.9406564584124654e-324;                     // 2^-1074
@@ -538,14 +537,15 @@ __device__ __forceinline__ double gsum16(double v) { return gh::tree16(v); }
 
 // rigorous D(y) of the group's LP: lane s holds column s of G (gc), y_s (row
 // lanes) and its column's data
-__device__ double rigorous_dual_g16(int m, int n, const double (&gc)[kG], double ys_eval,
+template <int NM>
+__device__ double rigorous_dual_g16(int m, int n, const double (&gc)[NM], double ys_eval,
                                     double c0, double cj, double lj, double uj, double hi,
                                     bool act_i)
 {
     const int s = threadIdx.x & (kG - 1);
     const double infl = 1.0 + gamma_k(2.0 * (m + n) + 16.0);
     double r = 0.0, rho = 0.0, k = 0.0;
-    for_lanes16(m, [&](auto I) {
+    gh::for_lanes_upto<NM>(m, [&](auto I) {
         const double yi = bcast16_f64<decltype(I)::value>(ys_eval), g = gc[decltype(I)::value];
         if (yi != 0.0 && g != 0.0) {
             const double p = g * yi;
@@ -597,6 +597,7 @@ __device__ double rigorous_dual_g16(int m, int n, const double (&gc)[kG], double
     return (bad == 0.0 && isfinite(lb)) ? lb : -INFINITY;
 }
 
+template <int NM>
 __global__ void __launch_bounds__(kGThreads)
 lp_dual_g16_kernel(int B, int m, int n, int S, const double *__restrict__ lA,
                    const double *__restrict__ lbias, const double *__restrict__ thr,
@@ -614,25 +615,25 @@ lp_dual_g16_kernel(int B, int m, int n, int S, const double *__restrict__ lA,
     const double lj = col ? xL[(size_t)b * n + s] : 0.0;
     const double uj = col ? xU[(size_t)b * n + s] : 0.0;
     // row s (threshold, activity, entries) and column s of G, inactive rows zeroed
-    double gr[kG], gc[kG];
+    double gr[NM], gc[NM];
     double hi = 1.0;
     bool act_i = false;
 #pragma unroll
-    for (int j = 0; j < kG; ++j) gr[j] = (row && j < n) ? Gb[(size_t)s * n + j] : 0.0;
+    for (int j = 0; j < NM; ++j) gr[j] = (row && j < n) ? Gb[(size_t)s * n + j] : 0.0;
     if (row) {
         const double h0 = thr[s + 1] - lbias[(size_t)b * S + s + 1];
         bool a = isfinite(h0);
 #pragma unroll
-        for (int j = 0; j < kG; ++j) a = a && isfinite(gr[j]);
+        for (int j = 0; j < NM; ++j) a = a && isfinite(gr[j]);
         act_i = a;
         hi = a ? h0 : 1.0;
     }
 #pragma unroll
-    for (int j = 0; j < kG; ++j) gr[j] = act_i ? gr[j] : 0.0;
+    for (int j = 0; j < NM; ++j) gr[j] = act_i ? gr[j] : 0.0;
     const double actd = act_i ? 1.0 : 0.0;
 #pragma unroll
-    for (int i = 0; i < kG; ++i) gc[i] = 0.0;
-    for_lanes16(m, [&](auto I) {
+    for (int i = 0; i < NM; ++i) gc[i] = 0.0;
+    gh::for_lanes_upto<NM>(m, [&](auto I) {
         constexpr int i = decltype(I)::value;
         const double g = col ? Gb[(size_t)i * n + s] : 0.0;
         gc[i] = bcast16_f64<i>(actd) != 0.0 ? g : 0.0;
@@ -646,18 +647,18 @@ lp_dual_g16_kernel(int B, int m, int n, int S, const double *__restrict__ lA,
         double dc = 1.0, dr = 1.0;
         // hs = h - G l
         double gl = 0.0;
-        for_lanes16(n, [&](auto J) {
+        gh::for_lanes_upto<NM>(n, [&](auto J) {
             gl += gr[decltype(J)::value] * bcast16_f64<decltype(J)::value>(lj);
         });
         double hs = row ? hi - gl : 0.0;
         for (int it = 0; it < ruiz_iters; ++it) {
             const double v = w * dc, y = dr;
             double rn = 0.0, cn = 0.0;
-            for_lanes16(n, [&](auto J) {
+            gh::for_lanes_upto<NM>(n, [&](auto J) {
                 constexpr int j = decltype(J)::value;
                 rn = fmax(rn, fabs(gr[j] * bcast16_f64<j>(v) * dr));
             });
-            for_lanes16(m, [&](auto I) {
+            gh::for_lanes_upto<NM>(m, [&](auto I) {
                 constexpr int i = decltype(I)::value;
                 cn = fmax(cn, fabs(gc[i] * bcast16_f64<i>(y) * w * dc));
             });
@@ -671,11 +672,11 @@ lp_dual_g16_kernel(int B, int m, int n, int S, const double *__restrict__ lA,
         double tau = 0.0, sig = 0.0;
         {
             double a = 0.0, q = 0.0;
-            for_lanes16(m, [&](auto I) {
+            gh::for_lanes_upto<NM>(m, [&](auto I) {
                 constexpr int i = decltype(I)::value;
                 a += fabs(gc[i] * bcast16_f64<i>(drr) * wdc);
             });
-            for_lanes16(n, [&](auto J) {
+            gh::for_lanes_upto<NM>(n, [&](auto J) {
                 constexpr int j = decltype(J)::value;
                 q += fabs(gr[j] * dr * bcast16_f64<j>(wdc));
             });
@@ -687,7 +688,7 @@ lp_dual_g16_kernel(int B, int m, int n, int S, const double *__restrict__ lA,
             // gradient of column s from the rows' Dr_i ys_i; primal step
             const double yd = row ? dr * ys : 0.0;
             double g = 0.0;
-            for_lanes16(m, [&](auto I) {
+            gh::for_lanes_upto<NM>(m, [&](auto I) {
                 constexpr int i = decltype(I)::value;
                 g = fma(gc[i], bcast16_f64<i>(yd), g);
             });
@@ -696,7 +697,7 @@ lp_dual_g16_kernel(int B, int m, int n, int S, const double *__restrict__ lA,
             // dual step of row s from the columns' w_j Dc_j (2 zn_j - zs_j)
             const double v = col ? wdc * (2.0 * zn - zs) : 0.0;
             double a = 0.0;
-            for_lanes16(n, [&](auto J) {
+            gh::for_lanes_upto<NM>(n, [&](auto J) {
                 constexpr int j = decltype(J)::value;
                 a = fma(gr[j], bcast16_f64<j>(v), a);
             });
@@ -731,9 +732,16 @@ extern "C" int gh_lp_dual_bound_f64(int B, int n, int S, const double *lA, const
     const int m = S - 1;
     if (n > kMaxN || m > kMaxM) return GH_EUNSUPPORTED;
     if (n <= kG && m <= kG) {
+        // register arrays of 8 entries for n, m <= 8 (fewer VGPRs, more
+        // wavefronts per SIMD), 16 above
         const int per = kGThreads / kG;
-        hipLaunchKernelGGL(lp_dual_g16_kernel, dim3((B + per - 1) / per), dim3(kGThreads), 0,
-                           stream, B, m, n, S, lA, lbias, thr, xL, xU, iters, 10, 50, lb, y);
+        const dim3 grid((B + per - 1) / per);
+        if (n <= 8 && m <= 8)
+            hipLaunchKernelGGL(lp_dual_g16_kernel<8>, grid, dim3(kGThreads), 0, stream, B, m, n,
+                               S, lA, lbias, thr, xL, xU, iters, 10, 50, lb, y);
+        else
+            hipLaunchKernelGGL(lp_dual_g16_kernel<16>, grid, dim3(kGThreads), 0, stream, B, m, n,
+                               S, lA, lbias, thr, xL, xU, iters, 10, 50, lb, y);
         return hipGetLastError() == hipSuccess ? GH_OK : GH_ELAUNCH;
     }
     if (n <= 64 && m <= 64) {
