@@ -63,8 +63,6 @@ def parse():
     ap.add_argument("--no-clip", action="store_true", help="skip the clip_type 'complete' section")
     ap.add_argument("--clip-only", action="store_true",
                     help="only the clip_type 'complete' section (kernels and time to gap)")
-    ap.add_argument("--pipe-probe", action="store_true",
-                    help="time FP64 MFMA waves, FP64 FMA waves and both on shared SIMDs, then exit")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device 5 times (a known-size "
                          "reference for the HBM byte counters) and exit")
@@ -80,32 +78,6 @@ def copy_probe(mib):
         b.copy_(a)
     torch.cuda.synchronize()
     print(f"copy probe: {mib} MiB read + {mib} MiB written per launch", flush=True)
-
-
-def pipe_probe():
-    """Do FP64 MFMA and FP64 VALU work of one SIMD overlap?  One workgroup of
-    8 waves per CU: 4 waves of 4 x 2,000 MFMA (16x16x4 f64), 4 waves of
-    8 x 4,000 FP64 FMAs, timed alone and together (gh_pipe_probe)."""
-    import torch
-    from gear_amd._lib import check, lib, ptr, stream_ptr
-    blocks = torch.cuda.get_device_properties(0).multi_processor_count
-    out = torch.empty(blocks * 512, dtype=torch.float64, device="cuda:0")
-    res = {}
-    for rep in range(2):
-        for mode, name in ((1, "mfma_only"), (2, "fma_only"), (0, "both")):
-            check(lib().gh_pipe_probe(mode, blocks, 2000, 4000, ptr(out), stream_ptr()),
-                  "gh_pipe_probe")
-            e0 = torch.cuda.Event(enable_timing=True)
-            e1 = torch.cuda.Event(enable_timing=True)
-            e0.record()
-            for _ in range(5):
-                check(lib().gh_pipe_probe(mode, blocks, 2000, 4000, ptr(out), stream_ptr()),
-                      "gh_pipe_probe")
-            e1.record()
-            torch.cuda.synchronize()
-            res[name] = round(e0.elapsed_time(e1) / 5, 4)
-    res["sum_of_alone"] = round(res["mfma_only"] + res["fma_only"], 4)
-    print(json.dumps({"pipe_probe_ms": res, "workgroups": blocks}), flush=True)
 
 
 def bench_uneven_dag(reps=20, cpu=True):
@@ -420,9 +392,6 @@ def main():
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
         sys.exit(2)
-    if args.pipe_probe:
-        pipe_probe()
-        return
     if args.copy_probe:
         copy_probe(args.copy_probe)
         return

@@ -76,7 +76,8 @@ def clip_domains(lA, lbias, thr, x_L, x_U, out=None, stream=None):
     x_U = x_U.contiguous()
     oL, oU = out if out is not None else (torch.empty_like(x_L), torch.empty_like(x_U))
     state = torch.empty(B, dtype=torch.int32, device=x_L.device)
-    check(lib().gh_clip_domains_f64(B, P, n, S, ptr(lA.contiguous()), ptr(lbias.contiguous()),
-                                    ptr(thr.contiguous()), ptr(x_L), ptr(x_U), ptr(oL), ptr(oU),
+    lA, lbias, thr = lA.contiguous(), lbias.contiguous(), thr.contiguous()
+    check(lib().gh_clip_domains_f64(B, P, n, S, ptr(lA), ptr(lbias),
+                                    ptr(thr), ptr(x_L), ptr(x_U), ptr(oL), ptr(oU),
                                     ptr(state), stream_ptr(stream)), "gh_clip_domains_f64")
     return oL, oU, state

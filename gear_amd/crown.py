@@ -207,8 +207,9 @@ def input_split_branching(net, dom_lb, x_L, x_U, lA, thresholds, branching_metho
     if branching_method != "naive" or split_depth != 1:
         raise NotImplementedError("only naive (widest) branching at depth 1 is on this path")
     B, n = x_L.shape
+    x_L, x_U = x_L.contiguous(), x_U.contiguous()
     idx = torch.empty(B, dtype=torch.int32, device=x_L.device)
-    check(lib().gh_branch_widest_f64(B, n, ptr(x_L.contiguous()), ptr(x_U.contiguous()),
+    check(lib().gh_branch_widest_f64(B, n, ptr(x_L), ptr(x_U),
                                      ptr(idx), stream_ptr()), "gh_branch_widest_f64")
     return idx.long().unsqueeze(1)
 
@@ -220,10 +221,11 @@ def input_split_parallel(x_L, x_U, shape=None, split_depth=1, i_idx=None, split_
     if split_depth != 1 or split_partitions != 2 or split_hint is not None:
         raise NotImplementedError("depth-1 bisection only")
     B, n = x_L.shape
+    x_L, x_U = x_L.contiguous(), x_U.contiguous()
     idx = i_idx[:, 0].to(torch.int32).contiguous()
     oL = torch.empty(2 * B, n, dtype=x_L.dtype, device=x_L.device)
     oU = torch.empty_like(oL)
-    check(lib().gh_split_f64(B, n, ptr(x_L.contiguous()), ptr(x_U.contiguous()), ptr(idx),
+    check(lib().gh_split_f64(B, n, ptr(x_L), ptr(x_U), ptr(idx),
                              ptr(oL), ptr(oU), stream_ptr()), "gh_split_f64")
     return oL, oU, 1
 
@@ -239,9 +241,10 @@ def child_keys(plb, lA, lbias, cL, cU, stream=None):
     if lA.shape != (K, R, n) or lbias.shape != (K, R) or not lA.is_contiguous() \
             or not lbias.is_contiguous():
         raise ValueError("planes must be contiguous (K, R, n) and (K, R)")
+    plb, cL, cU = plb.contiguous(), cL.contiguous(), cU.contiguous()
     keys = torch.empty(2 * K, S, dtype=torch.float64, device=plb.device)
-    check(lib().gh_child_keys_f64(K, n, S, R, ptr(plb.contiguous()), ptr(lA), ptr(lbias),
-                                  ptr(cL.contiguous()), ptr(cU.contiguous()), ptr(keys),
+    check(lib().gh_child_keys_f64(K, n, S, R, ptr(plb), ptr(lA), ptr(lbias),
+                                  ptr(cL), ptr(cU), ptr(keys),
                                   stream_ptr(stream)), "gh_child_keys_f64")
     return keys
 
@@ -261,8 +264,9 @@ def branch_split_keys(plb, pL, pU, lA=None, lbias=None, stream=None):
     cL = torch.empty(2 * K, n, dtype=torch.float64, device=pL.device)
     cU = torch.empty_like(cL)
     keys = torch.empty(2 * K, S, dtype=torch.float64, device=pL.device)
-    check(lib().gh_branch_split_keys_f64(K, n, S, R, ptr(pL.contiguous()), ptr(pU.contiguous()),
-                                         ptr(plb.contiguous()), ptr(lA), ptr(lbias), ptr(cL),
+    pL, pU, plb = pL.contiguous(), pU.contiguous(), plb.contiguous()
+    check(lib().gh_branch_split_keys_f64(K, n, S, R, ptr(pL), ptr(pU),
+                                         ptr(plb), ptr(lA), ptr(lbias), ptr(cL),
                                          ptr(cU), ptr(keys), stream_ptr(stream)),
           "gh_branch_split_keys_f64")
     return cL, cU, keys
@@ -379,11 +383,12 @@ class DeviceFrontier:
         the live records in logical order when the slack has run out."""
         if self.hw + B <= self.phys:
             return
+        # unsynced appends leave dead slots in the order: drop them first, or
+        # the copy below would turn the dead record into ordinary rows
+        self.compact_order()
         live = self.order[:self.count]
         old = (self._L, self._U, self._lb, self._pay)
-        unsynced = self.unsynced
         self._alloc(self.phys)
-        self.unsynced = unsynced
         for src, dst in zip(old, (self._L, self._U, self._lb, self._pay)):
             if src is not None:
                 torch.index_select(src, 0, live, out=dst[:self.count])
@@ -526,10 +531,11 @@ class DeviceFrontier:
         self._room(B)
         self._last_B = B
         ws = self.workspace(B, lib().gh_split_append_workspace_bytes(K))
-        check(lib().gh_split_append_f64(K, self.n, self.S, R, ptr(pL.contiguous()),
-                                        ptr(pU.contiguous()), ptr(rows), ptr(plb.contiguous()),
+        pL, pU, plb, thr = pL.contiguous(), pU.contiguous(), plb.contiguous(), thr.contiguous()
+        check(lib().gh_split_append_f64(K, self.n, self.S, R, ptr(pL),
+                                        ptr(pU), ptr(rows), ptr(plb),
                                         ptr(lA),
-                                        ptr(lbias), ptr(thr.contiguous()), ptr(self._L),
+                                        ptr(lbias), ptr(thr), ptr(self._L),
                                         ptr(self._U), ptr(self._lb), self.hw, self.phys,
                                         ptr(self.order[self.count:self.count + B]), self.dead,
                                         ptr(self.total), ptr(self.disc_min), ptr(ws),

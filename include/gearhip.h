@@ -426,18 +426,6 @@ int gh_gather_rows_f64(int64_t N, const int64_t *idx, int w0, const double *s0, 
                        int w1, const double *s1, double *d1, int w2, const double *s2,
                        double *d2, hipStream_t stream);
 
-/* ------------------------------------------------ measurement */
-
-/*
- * Microbenchmark of the FP64 pipes (no counterpart in the reference; used
- * by bench.py --pipe-probe to measure whether FP64 MFMA and FP64 VALU work
- * of one SIMD overlap): `blocks` workgroups of 512 threads; waves 0-3 run
- * n_mfma x 4 v_mfma_f64_16x16x4f64, waves 4-7 n_fma x 8 FP64 FMAs; mode 1
- * runs only the MFMA waves, mode 2 only the FMA waves, mode 0 both.  out
- * holds blocks * 512 doubles (results, so nothing is optimized away).
- */
-int gh_pipe_probe(int mode, int blocks, int n_mfma, int n_fma, double *out, hipStream_t stream);
-
 /* ------------------------------------------------ expression programs */
 
 /*

@@ -193,3 +193,27 @@ def test_lp_stops_at_the_objective_threshold(cuda, n, m):
             assert abs(o - early[b]) <= 1e-10 * (1 + abs(o)), b
     print(f"n={n} m={m}: {int((~below).sum())} of {len(L)} boxes stopped at thr0 {thr0:.6f}",
           flush=True)
+
+
+def test_noncontiguous_inputs_equal_contiguous(cuda):
+    """Strided views (transposed storage for the boxes, a row slice of the
+    planes) give the same certificates as contiguous copies: each copy the
+    binding makes stays alive until the kernel is queued."""
+    from gear_amd.crown import BoundedQCQP
+    from gear_amd.lp import qcqp_lp_bound
+    p = qcqp_dense(10, 3, seed=4)
+    L, U = random_bisection_boxes(p.xl, p.xu, 64, seed=5)
+    net = BoundedQCQP(p, cuda)
+    xL = torch.as_tensor(L, device=cuda)
+    xU = torch.as_tensor(U, device=cuda)
+    lb, lA, lbias = net.bound(xL, xU, return_A=True)
+    thr = torch.as_tensor(np.concatenate([[np.inf], p.rhs[1:] + 1e-6]), device=cuda)
+    ref = qcqp_lp_bound(lA, lbias, thr, xL, xU, iters=300)
+    tL = xL.t().contiguous().t()        # same values, column-major storage
+    tU = xU.t().contiguous().t()
+    assert not tL.is_contiguous() and not tU.is_contiguous()
+    wide = torch.cat([lA, lA], dim=1)[:, :lA.shape[1]]  # a strided view of the same planes
+    assert not wide.is_contiguous()
+    got = qcqp_lp_bound(wide, lbias, thr, tL, tU, iters=300)
+    torch.cuda.synchronize()
+    assert torch.equal(got, ref)

@@ -399,6 +399,41 @@ def test_unsynced_steps_equal_synced(cuda):
     assert _bits_equal(L0, L1) and _bits_equal(U0, U1) and _bits_equal(lb0, lb1)
 
 
+def test_compaction_with_unsynced_appends(cuda):
+    """A compaction that runs while unsynced appends still hold dead slots
+    keeps exactly the survivors (the dead record never becomes a row): after
+    appends without a count, picks from the end and a compaction, the
+    frontier holds the oracle-pruned survivors in order, bit for bit."""
+    from gear_amd.crown import DeviceFrontier
+    rng = np.random.default_rng(11)
+    n, S, B, P = 5, 3, 200, 150
+    thr = np.array([0.0, 0.5, np.inf])
+    f = DeviceFrontier(n, S, cap=2000, device=cuda, slack=100)
+    order = []                       # logical order: a survivor (L, U, lb) or None (dead)
+    compacted = False
+    for step in range(14):
+        L = rng.standard_normal((B, n))
+        U = L + np.abs(rng.standard_normal((B, n)))
+        lb = rng.standard_normal((B, S))
+        keep, kept, _ = O.prune(lb, thr, L, U)
+        assert 0 < kept < B
+        hw0 = f.hw
+        f.filter_append(_dev(lb, cuda), _dev(thr, cuda), _dev(L, cuda), _dev(U, cuda),
+                        sync=False)
+        compacted |= f.hw < hw0
+        order += [(L[k], U[k], lb[k]) for k in np.flatnonzero(keep)] + [None] * (B - kept)
+        f.pick_out(P)
+        order = order[:-P]
+    assert compacted
+    c = f.compact_order()
+    live = [r for r in order if r is not None]
+    gL, gU, glb = (t.cpu().numpy() for t in f.rows())
+    assert c == len(live) == len(gL)
+    assert _bits_equal(gL, np.array([r[0] for r in live]))
+    assert _bits_equal(gU, np.array([r[1] for r in live]))
+    assert _bits_equal(glb, np.array([r[2] for r in live]))
+
+
 @pytest.mark.parametrize("n,m,B", [(50, 20, 16384 + 37), (17, 4, 20000), (55, 3, 16400),
                                    (41, 2, 16500), (10, 3, 17000)])
 def test_qcqp_crown_f64_staged_bit_exact(cuda, n, m, B):
