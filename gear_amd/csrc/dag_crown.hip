@@ -18,6 +18,7 @@
 #include "common.h"
 #include "dag_tape.h"
 #include "gearhip_internal.h"
+#include "sshape.h"
 #include "../../include/gearhip_math.h"
 
 namespace {
@@ -102,7 +103,12 @@ __device__ void ibp(int op, bool self, double la, double ua, double lb_, double 
         l = ua < 0 ? ua * ua : (la > 0 ? la * la : 0.0);
         u = fmax(la * la, ua * ua);
         break;
-    case GH_OP_POW: {
+    case GH_OP_POW: {  // s_shaped.py:1088-1130
+        if (c != floor(c)) {  // non-integer: increasing on the domain x >= 0
+            l = gh_powf(la > 0 ? la : 0.0, c);
+            u = gh_powf(ua, c);
+            break;
+        }
         double pl = gh_powi(la, (int)c), pu = gh_powi(ua, (int)c);
         if (((int)c) % 2 == 0) {
             l = (la <= 0 && ua >= 0) ? 0.0 : fmin(pl, pu);
@@ -130,6 +136,12 @@ __device__ void ibp(int op, bool self, double la, double ua, double lb_, double 
         l = crosses(la, ua, 3 * PI / 2) ? -1.0 : fmin(sl, su);
         break;
     }
+    case GH_OP_COS: {  // the extrema of cos at 0 and pi (trigonometric.py:509-514)
+        double sl = gh_cos(la), su = gh_cos(ua);
+        u = crosses(la, ua, 0.0) ? 1.0 : fmax(sl, su);
+        l = crosses(la, ua, PI) ? -1.0 : fmin(sl, su);
+        break;
+    }
     default: l = -INFINITY; u = INFINITY;
     }
     if (isnan(l)) l = -INFINITY;
@@ -142,7 +154,7 @@ __device__ void ibp(int op, bool self, double la, double ua, double lb_, double 
 // R: unary (lw, lb, uw, ub); MUL (alpha_l, beta_l, gamma_l, alpha_u,
 // beta_u, gamma_u).  Returns the undefined sides (1 lower, 2 upper).
 __device__ int relax(int op, double l, double u, double yl, double yu, double c, double il,
-                     double iu, double R[6])
+                     double iu, const double *T, double R[6])
 {
     for (int k = 0; k < 6; ++k) R[k] = 0.0;
     switch (op) {
@@ -174,55 +186,21 @@ __device__ int relax(int op, double l, double u, double yl, double yu, double c,
         return 0;
     }
     case GH_OP_POW: {
+        if (c != floor(c)) return gh::ss::relax_pow_frac(c, l, u, R);
         int p = (int)c;
         if (!isfinite(l) || !isfinite(u)) return (p % 2 == 0) ? 2 : 3;
+        if (p % 2 == 1) return gh::ss::relax_pow_odd(T, p, l, u, R);
         u = fmax(u, l + 1e-8);  // s_shaped.py:795
-        if (p % 2 == 0) {  // s_shaped.py:940-972
-            double yl_ = gh_powi(l, p), yu_ = gh_powi(u, p);
-            double eps = fmax(fabs(yl_), fabs(yu_)) * 1e-6 + 1e-12;
-            double k = (yu_ - yl_) / fmax(u - l, 1e-8);
-            R[2] = k;
-            R[3] = -l * k + (yl_ + eps);
-            double mid = l >= 0 ? l : (u <= 0 ? u : 0.0);
-            double dk = p * gh_powi(mid, p - 1);
-            R[0] = dk;
-            R[1] = -mid * dk + (gh_powi(mid, p) - eps);
-        } else {  // x^3, inflection at 0 (s_shaped.py:325-549)
-            double fl = l * l * l, fu = u * u * u;
-            double ks = (fu - fl) / (u - l);
-            if (l >= 0) {
-                R[2] = ks;
-                R[3] = -l * ks + fl;
-                double m = (l + u) / 2, dk = 3 * m * m;
-                R[0] = dk;
-                R[1] = -m * dk + m * m * m;
-            } else if (u <= 0) {
-                R[0] = ks;
-                R[1] = -l * ks + fl;
-                double m = (l + u) / 2, dk = 3 * m * m;
-                R[2] = dk;
-                R[3] = -m * dk + m * m * m;
-            } else {
-                double d = -l / 2;  // tangent at d passes through (l, l^3)
-                if (d <= u) {
-                    double dk = 3 * d * d;
-                    R[0] = dk;
-                    R[1] = -l * dk + fl;
-                } else {
-                    R[0] = ks;
-                    R[1] = -l * ks + fl;
-                }
-                double e = -u / 2;  // tangent at e passes through (u, u^3)
-                if (e >= l) {
-                    double ek = 3 * e * e;
-                    R[2] = ek;
-                    R[3] = -u * ek + fu;
-                } else {
-                    R[2] = ks;
-                    R[3] = -l * ks + fl;
-                }
-            }
-        }
+        // even power: bound_relax_even_power (s_shaped.py:940-972)
+        double yl_ = gh_powi(l, p), yu_ = gh_powi(u, p);
+        double eps = fmax(fabs(yl_), fabs(yu_)) * 1e-6 + 1e-12;
+        double k = (yu_ - yl_) / fmax(u - l, 1e-8);
+        R[2] = k;
+        R[3] = -l * k + (yl_ + eps);
+        double mid = l >= 0 ? l : (u <= 0 ? u : 0.0);
+        double dk = p * gh_powi(mid, p - 1);
+        R[0] = dk;
+        R[1] = -mid * dk + (gh_powi(mid, p) - eps);
         return 0;
     }
     case GH_OP_EXP: {  // convex_concave.py:569-625
@@ -281,28 +259,8 @@ __device__ int relax(int op, double l, double u, double yl, double yu, double c,
         }
         return 0;
     }
-    case GH_OP_SIN: {  // trigonometric.py:72-85 roundoff margin
-        if (!isfinite(l) || !isfinite(u)) return 3;
-        double margin = 8 * EPS64 * (1 + fmax(fabs(l), fabs(u)));
-        double kpi_l = floor(l / PI), kpi_u = floor(u / PI);
-        if (kpi_l == kpi_u && u - l < PI) {
-            // one piece: sin is concave where floor(x / pi) is even
-            double sl = gh_sin(l), su = gh_sin(u);
-            double ks = (u > l) ? (su - sl) / (u - l) : gh_cos(l);
-            double m = (l + u) / 2, km = gh_cos(m), bm = -m * km + gh_sin(m);
-            double bs = -l * ks + sl;
-            if (((long long)kpi_l) % 2 == 0) {
-                R[0] = ks; R[1] = bs; R[2] = km; R[3] = bm;
-            } else {
-                R[0] = km; R[1] = bm; R[2] = ks; R[3] = bs;
-            }
-        } else {
-            R[0] = 0.0; R[1] = il; R[2] = 0.0; R[3] = iu;
-        }
-        R[1] -= margin;
-        R[3] += margin;
-        return 0;
-    }
+    case GH_OP_SIN: return gh::ss::relax_sin(T, l, u, R);
+    case GH_OP_COS: return gh::ss::relax_cos(T, l, u, R);
     default: return 3;
     }
 }
@@ -511,7 +469,8 @@ __device__ void crown_pass(const DagPass *passes, const DagStep *steps, const in
 // Bound one row of one program over 64 boxes (one per lane).
 __device__ void run_job(const DagJob &J, const DagFwd *fwd, const DagPass *passes,
                         const DagStep *steps, const int32_t *cin, const int32_t *gin,
-                        const gh_dag_batch &bt, int64_t count, int64_t chunk, Lane W)
+                        const double *tabs, const gh_dag_batch &bt, int64_t count,
+                        int64_t chunk, Lane W)
 {
     const int lane = threadIdx.x;
     const int64_t box = chunk * kLanes + lane;
@@ -535,7 +494,7 @@ __device__ void run_job(const DagJob &J, const DagFwd *fwd, const DagPass *passe
         W[V + k] = iu;
         if (rs >= 0) {
             double R[6];
-            int ud = relax(op, la, ua, lb_, ub_, f.c, il, iu, R);
+            int ud = relax(op, la, ua, lb_, ub_, f.c, il, iu, tabs + uni(f.tab), R);
             if (op == GH_OP_MUL) {
                 for (int q = 0; q < 6; ++q) W[rs + q] = R[q];
                 W[rs + 6] = (double)ud;
@@ -601,7 +560,8 @@ __global__ void __launch_bounds__(64)
 dag_crown_kernel(const DagJob *__restrict__ jobs, int64_t njobs, const DagFwd *__restrict__ fwd,
                  const DagPass *__restrict__ passes,
                  const DagStep *__restrict__ steps, const int32_t *__restrict__ cin,
-                 const int32_t *__restrict__ gin, const gh_dag_batch *__restrict__ batches,
+                 const int32_t *__restrict__ gin, const double *__restrict__ tabs,
+                 const gh_dag_batch *__restrict__ batches,
                  const int64_t *__restrict__ job_start, int64_t *counter, double *ws,
                  int64_t slots)
 {
@@ -624,7 +584,7 @@ dag_crown_kernel(const DagJob *__restrict__ jobs, int64_t njobs, const DagFwd *_
         const DagJob J = jobs[lo];
         const gh_dag_batch bt = batches[J.prog];
         const int64_t count = *bt.count;
-        run_job(J, fwd, passes, steps, cin, gin, bt, count, it - job_start[lo], W);
+        run_job(J, fwd, passes, steps, cin, gin, tabs, bt, count, it - job_start[lo], W);
     }
 }
 
@@ -651,6 +611,6 @@ extern "C" int gh_dag_crown_f64(const gh_dag_plan *plan, const gh_dag_batch *bat
                        plan->njobs, batches, job_start, counter);
     hipLaunchKernelGGL(dag_crown_kernel, dim3(waves), dim3(kLanes), 0, stream, plan->jobs,
                        plan->njobs, plan->fwd, plan->passes, plan->steps, plan->cin, plan->gin,
-                       batches, job_start, counter, slots, plan->max_slots);
+                       plan->tabs, batches, job_start, counter, slots, plan->max_slots);
     return hipGetLastError() == hipSuccess ? GH_OK : GH_ELAUNCH;
 }

@@ -12,11 +12,15 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <vector>
 
 #include "dag_tape.h"
 #include "gearhip_internal.h"
+#include "sshape.h"
 
 namespace {
 
@@ -28,7 +32,32 @@ using gh::DagStep;
 bool is_relaxed(int op)
 {
     return op == GH_OP_MUL || op == GH_OP_SQR || op == GH_OP_POW || op == GH_OP_EXP ||
-           op == GH_OP_LOG || op == GH_OP_RECIP || op == GH_OP_SIN;
+           op == GH_OP_LOG || op == GH_OP_RECIP || op == GH_OP_SIN || op == GH_OP_COS;
+}
+
+bool odd_power(const gh_node &nd)
+{
+    return nd.op == GH_OP_POW && nd.c == std::floor(nd.c) && ((int)nd.c) % 2 == 1;
+}
+
+// Tangent tables, built once per process on the host (they depend only on
+// the function): key 0 sin / cos, key p an odd power p.
+const std::vector<double> &host_table(int key)
+{
+    static std::mutex mu;
+    static std::map<int, std::vector<double>> cache;
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+    std::vector<double> &T = cache[key];
+    if (key == 0) {
+        T.resize(gh::ss::SIN_BLOCK);
+        gh::ss::build_sin_tables(T.data());
+    } else {
+        T.resize(gh::ss::POW_BLOCK);
+        gh::ss::build_pow_tables(key, T.data());
+    }
+    return T;
 }
 
 bool is_binary(int op) { return op == GH_OP_ADD || op == GH_OP_SUB || op == GH_OP_MUL; }
@@ -42,6 +71,20 @@ struct Builder {
     std::vector<DagStep> steps;
     std::vector<int32_t> cin;
     std::vector<int32_t> gin;
+    // tangent tables the tape's sin / cos / odd-power nodes read: table key
+    // -> offset (doubles) in `tabs`
+    std::vector<double> tabs;
+    std::map<int, int32_t> tab_off;
+    int32_t table(int key)
+    {
+        auto it = tab_off.find(key);
+        if (it != tab_off.end()) return it->second;
+        const std::vector<double> &T = host_table(key);
+        int32_t off = (int32_t)tabs.size();
+        tabs.insert(tabs.end(), T.begin(), T.end());
+        tab_off[key] = off;
+        return off;
+    }
     // per program: state words, I/O words and backward steps per box
     std::vector<int64_t> prog_words, prog_io, prog_steps;
 
@@ -208,12 +251,13 @@ struct Builder {
         std::vector<char> needs(Vg, 0);
         for (int v = n; v < Vg; ++v) {
             const gh_node &nd = G.nodes[v];
-            if (nd.op < GH_OP_CONST || nd.op > GH_OP_SIN) return GH_EINVAL;
+            if (nd.op < GH_OP_CONST || nd.op > GH_OP_COS) return GH_EINVAL;
             if (nd.op != GH_OP_CONST && (nd.a < 0 || nd.a >= v)) return GH_EINVAL;
             if (is_binary(nd.op) && (nd.b < 0 || nd.b >= v)) return GH_EINVAL;
             if (nd.op == GH_OP_POW) {
-                int e = (int)nd.c;
-                if ((double)e != nd.c || (e != 3 && e != 4 && e != 6)) return GH_EUNSUPPORTED;
+                // integers 3 .. 31, or non-integers > 0 (x ** 2 is SQR)
+                if (!(nd.c > 0) || nd.c > 31) return GH_EUNSUPPORTED;
+                if (nd.c == std::floor(nd.c) && nd.c < 3) return GH_EUNSUPPORTED;
             }
             if (!is_relaxed(nd.op)) continue;
             if (nd.a >= n) needs[nd.a] = 1;
@@ -287,6 +331,8 @@ struct Builder {
                 f.rs = rs[k];
                 f.c = N[k].c;
                 f.pass = -1;
+                if (N[k].op == GH_OP_SIN || N[k].op == GH_OP_COS) f.tab = table(0);
+                else if (odd_power(G.nodes[N[k].gid])) f.tab = table((int)N[k].c);
                 words += 4 + (is_binary(f.op) ? 2 : 0) + (rs[k] < 0 ? 0 : (f.op == GH_OP_MUL ? 7 : 5));
                 if (needs[N[k].gid] && N[k].op != GH_OP_CONST) {
                     f.pass = add_pass(N, ni, k, 1, abase, &apeak, rs);
@@ -340,6 +386,7 @@ extern "C" int gh_dag_plan_create(int nprog, const gh_prog *progs, gh_dag_plan *
     std::vector<char> buf;
     size_t o_jobs = append(buf, B.jobs), o_fwd = append(buf, B.fwd), o_pass = append(buf, B.passes);
     size_t o_steps = append(buf, B.steps), o_cin = append(buf, B.cin), o_gin = append(buf, B.gin);
+    size_t o_tabs = append(buf, B.tabs);
     buf.resize(std::max<size_t>(buf.size(), 256));
     auto *P = new gh_dag_plan{};
     if (hipGetDevice(&P->device) != hipSuccess || hipMalloc(&P->dbuf, buf.size()) != hipSuccess) {
@@ -358,6 +405,7 @@ extern "C" int gh_dag_plan_create(int nprog, const gh_prog *progs, gh_dag_plan *
     P->steps = reinterpret_cast<const DagStep *>(d + o_steps);
     P->cin = reinterpret_cast<const int32_t *>(d + o_cin);
     P->gin = reinterpret_cast<const int32_t *>(d + o_gin);
+    P->tabs = reinterpret_cast<const double *>(d + o_tabs);
     P->njobs = (int64_t)B.jobs.size();
     P->nfwd = (int64_t)B.fwd.size();
     P->npass = (int64_t)B.passes.size();

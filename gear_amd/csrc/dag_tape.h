@@ -54,7 +54,7 @@ struct DagStep {      // 32 bytes, one per visited ancestor
 struct DagFwd {       // 32 bytes, one per operation node
     int32_t op, a, b, rs;
     int32_t pass;     // CROWN pass of this node, -1 for none
-    int32_t pad;
+    int32_t tab;      // sin / cos / odd power: offset of its tangent tables
     double c;
 };
 
@@ -91,6 +91,7 @@ struct gh_dag_plan {
     const gh::DagStep *steps;
     const int32_t *cin;         // (node << 16) | A slot
     const int32_t *gin;
+    const double *tabs;         // tangent tables (gear_amd/csrc/sshape.h)
     // per program, per box: bytes of per-lane state the tape reads and
     // writes (the A propagation and relaxation data), and box / bound I/O
     std::vector<int64_t> state_bytes, io_bytes, visits;

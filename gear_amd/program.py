@@ -9,10 +9,10 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-INPUT, CONST, ADD, SUB, NEG, MULC, ADDC, MUL, SQR, POW, EXP, LOG, RECIP, SIN = range(14)
+INPUT, CONST, ADD, SUB, NEG, MULC, ADDC, MUL, SQR, POW, EXP, LOG, RECIP, SIN, COS = range(15)
 OP_NAMES = ["input", "const", "add", "sub", "neg", "mulc", "addc", "mul", "sqr", "pow", "exp",
-            "log", "recip", "sin"]
-RELAXED = {MUL, SQR, POW, EXP, LOG, RECIP, SIN}
+            "log", "recip", "sin", "cos"]
+RELAXED = {MUL, SQR, POW, EXP, LOG, RECIP, SIN, COS}
 BINARY = {ADD, SUB, MUL}
 
 NODE_DTYPE = np.dtype([("op", "<i4"), ("a", "<i4"), ("b", "<i4"), ("pad", "<i4"), ("c", "<f8")])
@@ -61,12 +61,15 @@ class Program:
         return self._node(SQR, a)
 
     def pow(self, a, p):
+        """a ** p for a constant p: integers >= 1 (x ** 2 is SQR, as
+        convert_sqr lowers it) and non-integers p > 0 (on a >= 0)."""
+        p = float(p)
         if p == 2:
             return self.sqr(a)
         if p == 1:
             return a
-        if p not in (3, 4, 6):
-            raise ValueError("integer exponents 2, 3, 4, 6")
+        if not (p > 0) or p > 31 or (p == int(p) and p < 3):
+            raise ValueError("exponents: integers 1 .. 31 or non-integers p > 0")
         return self._node(POW, a, c=p)
 
     def exp(self, a):
@@ -80,6 +83,9 @@ class Program:
 
     def sin(self, a):
         return self._node(SIN, a)
+
+    def cos(self, a):
+        return self._node(COS, a)
 
     def div(self, a, b):
         """a / b lowered as MUL(a, RECIP(b)) (optimize_graph.py:103-122)."""
@@ -149,6 +155,8 @@ class Program:
                     y = 1.0 / A
                 elif op == SIN:
                     y = np.sin(A)
+                elif op == COS:
+                    y = np.cos(A)
                 else:
                     raise ValueError(op)
                 Y[k] = y

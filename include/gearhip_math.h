@@ -1,7 +1,7 @@
 /*
  * Deterministic FP64 elementary functions of the expression-program bound:
- * exp, expm1, log, sin, cos and small integer powers, built only from
- * IEEE +, -, *, /, fma, floor and exact scaling (ldexp / frexp).
+ * exp, expm1, log, sin, cos, asin, integer and non-integer powers, built only
+ * from IEEE +, -, *, /, fma, sqrt, floor and exact scaling (ldexp / frexp).
  *
  * Every operation is correctly rounded on both the host and gfx950, so a
  * host build (gcc, -ffp-contract=off) and a device build give the same bits
@@ -13,7 +13,7 @@
  *
  * Accuracy (measured against 80-bit reference values on seeded sweeps):
  * exp, sin, cos within 2 ulp; log within 2.5 ulp; expm1 within 3 ulp;
- * integer powers up to x^6 within 3.5 ulp.
+ * integer powers up to x^6 within 3.5 ulp; asin within 2.5 ulp.
  * sin / cos use a three-part Cody-Waite reduction by pi/2 and return NaN for
  * |x| >= 2^50, where no double-precision reduction of that kind is accurate
  * (callers treat NaN as a vacuous bound).
@@ -22,7 +22,9 @@
  * scaled by 2^k; expm1 -- Taylor series of degree 16 on |x| < ln2/2, else
  * exp(x) - 1; log -- x = 2^e m, m in [sqrt(1/2), sqrt(2)), log m =
  * 2 atanh(s), s = (m - 1) / (m + 1), odd series to s^25; sin / cos -- Taylor
- * series of degree 21 / 22 on |r| <= pi/4 after the reduction.
+ * series of degree 21 / 22 on |r| <= pi/4 after the reduction; asin -- the
+ * Maclaurin series on |x| <= 1/2 and the half-angle identity above; x^p for
+ * non-integer p -- exp(p log x) (sqrt for p = 1/2).
  */
 #ifndef GEARHIP_MATH_H
 #define GEARHIP_MATH_H
@@ -225,6 +227,69 @@ GH_MATH_FN double gh_powi(double x, int p)
         return p < 0 ? 1.0 / r : r;
     }
     }
+}
+
+/* asin x on |x| <= 1/2: x + x z P(z), z = x^2, the Maclaurin series
+ * sum_k (2k)! / (4^k (k!)^2 (2k + 1)) x^(2k+1) to k = 26 (tail < 1e-18
+ * relative at |x| = 1/2) */
+GH_MATH_FN double gh_asin_kernel(double x)
+{
+    double z = x * x;
+    double p = 0.0020776610325181676;
+    p = fma(p, z, 0.0022014739737101384);
+    p = fma(p, z, 0.002338091892111975);
+    p = fma(p, z, 0.0024894486782468836);
+    p = fma(p, z, 0.00265787063820729);
+    p = fma(p, z, 0.002846178401108942);
+    p = fma(p, z, 0.0030578216492580306);
+    p = fma(p, z, 0.003297059503473485);
+    p = fma(p, z, 0.0035692053938259347);
+    p = fma(p, z, 0.003880964558837669);
+    p = fma(p, z, 0.004240907093679363);
+    p = fma(p, z, 0.004660143486915096);
+    p = fma(p, z, 0.005153309682319905);
+    p = fma(p, z, 0.005740037670841924);
+    p = fma(p, z, 0.006447210311889649);
+    p = fma(p, z, 0.0073125258735988454);
+    p = fma(p, z, 0.008390335809616815);
+    p = fma(p, z, 0.009761609529194078);
+    p = fma(p, z, 0.011551800896139705);
+    p = fma(p, z, 0.01396484375);
+    p = fma(p, z, 0.017352764423076924);
+    p = fma(p, z, 0.022372159090909092);
+    p = fma(p, z, 0.030381944444444444);
+    p = fma(p, z, 0.044642857142857144);
+    p = fma(p, z, 0.075);
+    p = fma(p, z, 0.16666666666666666);
+    return fma(x * z, p, x);
+}
+
+/* asin x: the series for |x| <= 1/2, else asin |x| = pi/2 - 2 asin s with
+ * s = sqrt((1 - |x|) / 2) <= 1/2 (the half-angle identity); NaN outside
+ * [-1, 1] */
+GH_MATH_FN double gh_asin(double x)
+{
+    if (isnan(x) || fabs(x) > 1.0) return NAN;
+    double a = fabs(x);
+    double r;
+    if (a <= 0.5) {
+        r = gh_asin_kernel(a);
+    } else {
+        double s = sqrt((1.0 - a) * 0.5);
+        r = fma(-2.0, gh_asin_kernel(s), GH_PIO2_1) + GH_PIO2_2;
+    }
+    return x < 0 ? -r : r;
+}
+
+/* x^p for a non-integer p > 0 and x >= 0: sqrt for p = 1/2, else
+ * exp(p log x); 0 at x = 0, NaN below 0 */
+GH_MATH_FN double gh_powf(double x, double p)
+{
+    if (isnan(x) || x < 0) return NAN;
+    if (x == 0) return 0.0;
+    if (p == 0.5) return sqrt(x);
+    if (isinf(x)) return INFINITY;
+    return gh_exp(p * gh_log(x));
 }
 
 #endif /* GEARHIP_MATH_H */

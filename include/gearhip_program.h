@@ -7,7 +7,7 @@
  * The node vocabulary is the subset of the reference's operator set that
  * nonlinear programs lower to (auto_LiRPA/operators: add_sub.py, bivariate.py,
  * activations.py BoundSqr, s_shaped.py BoundPow, convex_concave.py BoundExp /
- * BoundLog / BoundReciprocal, trigonometric.py BoundSin); division is
+ * BoundLog / BoundReciprocal, trigonometric.py BoundSin / BoundCos); division is
  * MUL(a, RECIP(b)) as optimize_graph.py:103-122 lowers it.
  */
 #ifndef GEARHIP_PROGRAM_H
@@ -25,11 +25,12 @@ enum {
     GH_OP_ADDC = 6,  /* a + c */
     GH_OP_MUL = 7,   /* a * b, both perturbed */
     GH_OP_SQR = 8,   /* a * a */
-    GH_OP_POW = 9,   /* a ** c, c in {3, 4} */
+    GH_OP_POW = 9,   /* a ** c: c an integer >= 3, or a non-integer c > 0 (on a >= 0) */
     GH_OP_EXP = 10,
     GH_OP_LOG = 11,
     GH_OP_RECIP = 12, /* 1 / a */
-    GH_OP_SIN = 13
+    GH_OP_SIN = 13,
+    GH_OP_COS = 14
 };
 
 typedef struct {

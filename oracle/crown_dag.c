@@ -9,8 +9,8 @@
  * Program: nodes in topological order; nodes 0..n-1 are the inputs x_i (the
  * reference's Gather / slice of the input vector), then operations
  *   CONST c, ADD a b, SUB a b, NEG a, MULC a c (a * constant), ADDC a c
- *   (a + constant), MUL a b (both perturbed), SQR a, POW a p (p = 3, 4),
- *   EXP a, LOG a, RECIP a, SIN a
+ *   (a + constant), MUL a b (both perturbed), SQR a, POW a p (an integer
+ *   p >= 3 or a non-integer p > 0), EXP a, LOG a, RECIP a, SIN a, COS a
  * (division is lowered to MUL(a, RECIP(b)) as optimize_graph.py:103-122
  * does; x ** 2 to SQR as convert_sqr does, optimize_graph.py:124).
  * Row r of the specification is node out[r], C = +1, lower bounds.
@@ -38,12 +38,8 @@
  * pass vacuous, and the row falls back to its interval bound
  * (backward_bound.py:295-463, 826-877).
  *
- * Deviations, stated: the reference's S-shaped x^3 and sin relaxations read
- * tangent points from precomputed tables (s_shaped.py:694-770,
- * trigonometric.py:60-69, 323-440); here x^3 uses the exact tangent points
- * (-l/2, -u/2) those tables approximate, and sin uses secant / midpoint
- * tangent on a convex or concave piece and constant planes at its interval
- * bounds when the interval crosses an inflection point.  Both are sound.
+ * The relaxations of sin, cos, odd integer powers and non-integer powers,
+ * with their tangent tables, are in sshaped.c.
  */
 #include <math.h>
 #include <stdint.h>
@@ -55,6 +51,7 @@
 
 #include "crown_dag.h"
 #include "gearhip_math.h"
+#include "sshaped.h"
 
 #define EPS64 2.220446049250313080847e-16
 #define EPSILON 1e-12  /* auto_LiRPA operators/base.py:32 */
@@ -82,7 +79,7 @@ static double mulz(double a, double d) { return a == 0 ? 0.0 : a * d; }
 static int is_relaxed(int op)
 {
     return op == GH_OP_MUL || op == GH_OP_SQR || op == GH_OP_POW || op == GH_OP_EXP ||
-           op == GH_OP_LOG || op == GH_OP_RECIP || op == GH_OP_SIN;
+           op == GH_OP_LOG || op == GH_OP_RECIP || op == GH_OP_SIN || op == GH_OP_COS;
 }
 
 /* ---------------------------------------------------------------- IBP */
@@ -135,7 +132,12 @@ static void ibp_node(const gh_node *nd, const double *L, const double *U, double
         l = ua < 0 ? ua * ua : (la > 0 ? la * la : 0.0);
         u = fmax(la * la, ua * ua);
         break;
-    case GH_OP_POW: {
+    case GH_OP_POW: {  /* s_shaped.py:1088-1130 */
+        if (c != floor(c)) {  /* non-integer: increasing on the domain x >= 0 */
+            l = gh_powf(la > 0 ? la : 0.0, c);
+            u = gh_powf(ua, c);
+            break;
+        }
         double pl = gh_powi(la, (int)c), pu = gh_powi(ua, (int)c);
         if (((int)c) % 2 == 0) {
             l = (la <= 0 && ua >= 0) ? 0.0 : fmin(pl, pu);
@@ -154,7 +156,13 @@ static void ibp_node(const gh_node *nd, const double *L, const double *U, double
     case GH_OP_SIN: { /* trigonometric.py:445-467 */
         double sl = gh_sin(la), su = gh_sin(ua);
         u = crosses(la, ua, M_PI / 2) ? 1.0 : fmax(sl, su);
-        l = crosses(la, ua, 3 * M_PI / 2) ? -1.0 : fmin(sl, su);
+        l = crosses(la, ua, M_PI * 3 / 2) ? -1.0 : fmin(sl, su);
+        break;
+    }
+    case GH_OP_COS: { /* the same with the extrema of cos (trigonometric.py:509-514) */
+        double sl = gh_cos(la), su = gh_cos(ua);
+        u = crosses(la, ua, 0.0) ? 1.0 : fmax(sl, su);
+        l = crosses(la, ua, M_PI) ? -1.0 : fmin(sl, su);
         break;
     }
     default: l = -INFINITY; u = INFINITY;
@@ -166,6 +174,11 @@ static void ibp_node(const gh_node *nd, const double *L, const double *U, double
 }
 
 /* -------------------------------------------------------- relaxations */
+/* GHO_SIN_LEGACY_INDEX (sshaped.h) or 0; set by gho_dag_set_flags */
+static int dag_flags;
+
+void gho_dag_set_flags(int flags) { dag_flags = flags; }
+
 /* R[0..5]: unary (lw, lb, uw, ub); MUL (al, bl, gl, au, bu, gu).
  * returns bit 1: lower side undefined, bit 2: upper side undefined. */
 static int relax_node(const gh_node *nd, const double *L, const double *U, double *R)
@@ -199,6 +212,7 @@ static int relax_node(const gh_node *nd, const double *L, const double *U, doubl
         return 0;
     }
     case GH_OP_POW: {
+        if (nd->c != floor(nd->c)) return gho_relax_pow_frac(nd->c, l, u, R);
         int p = (int)nd->c;
         if (!isfinite(l) || !isfinite(u)) return (p % 2 == 0) ? 2 : 3;
         u = fmax(u, l + 1e-8);  /* s_shaped.py:795 */
@@ -210,27 +224,9 @@ static int relax_node(const gh_node *nd, const double *L, const double *U, doubl
             double mid = l >= 0 ? l : (u <= 0 ? u : 0.0);
             double dk = p * gh_powi(mid, p - 1);
             R[0] = dk; R[1] = -mid * dk + (gh_powi(mid, p) - eps);
-        } else {  /* x^3: inverse S (s_shaped.py:325-549 with inflection 0) */
-            double fl = l * l * l, fu = u * u * u;
-            double ks = (fu - fl) / (u - l);
-            if (l >= 0) {  /* convex: upper secant, lower tangent at the midpoint */
-                R[2] = ks; R[3] = -l * ks + fl;
-                double m = (l + u) / 2, dk = 3 * m * m;
-                R[0] = dk; R[1] = -m * dk + m * m * m;
-            } else if (u <= 0) {  /* concave: lower secant, upper tangent */
-                R[0] = ks; R[1] = -l * ks + fl;
-                double m = (l + u) / 2, dk = 3 * m * m;
-                R[2] = dk; R[3] = -m * dk + m * m * m;
-            } else {
-                double d = -l / 2;  /* tangent at d passes through (l, l^3) */
-                if (d <= u) { double dk = 3 * d * d; R[0] = dk; R[1] = -l * dk + fl; }
-                else { R[0] = ks; R[1] = -l * ks + fl; }
-                double e = -u / 2;  /* tangent at e passes through (u, u^3) */
-                if (e >= l) { double ek = 3 * e * e; R[2] = ek; R[3] = -u * ek + fu; }
-                else { R[2] = ks; R[3] = -l * ks + fl; }
-            }
+            return 0;
         }
-        return 0;
+        return gho_relax_pow_odd(p, l, u, R);
     }
     case GH_OP_EXP: { /* convex_concave.py:569-625 */
         if (!isfinite(l) || !isfinite(u)) return 3;
@@ -280,28 +276,8 @@ static int relax_node(const gh_node *nd, const double *L, const double *U, doubl
         }
         return 0;
     }
-    case GH_OP_SIN: { /* trigonometric.py:72-85 roundoff margin */
-        if (!isfinite(l) || !isfinite(u)) return 3;
-        double margin = 8 * EPS64 * (1 + fmax(fabs(l), fabs(u)));
-        double kpi_l = floor(l / M_PI), kpi_u = floor(u / M_PI);
-        if (kpi_l == kpi_u && u - l < M_PI) {
-            /* one piece: sin is concave where floor(x/pi) is even */
-            double sl = gh_sin(l), su = gh_sin(u);
-            double ks = (u > l) ? (su - sl) / (u - l) : gh_cos(l);
-            double m = (l + u) / 2, km = gh_cos(m), bm = -m * km + gh_sin(m);
-            double bs = -l * ks + sl;
-            if (((long)kpi_l) % 2 == 0) { R[0] = ks; R[1] = bs; R[2] = km; R[3] = bm; }
-            else { R[0] = km; R[1] = bm; R[2] = ks; R[3] = bs; }
-        } else {
-            double lo, hi;
-            gh_node t = *nd;
-            ibp_node(&t, L, U, &lo, &hi);
-            R[0] = 0.0; R[1] = lo; R[2] = 0.0; R[3] = hi;
-        }
-        R[1] -= margin;
-        R[3] += margin;
-        return 0;
-    }
+    case GH_OP_SIN: return gho_relax_sin(l, u, dag_flags, R);
+    case GH_OP_COS: return gho_relax_cos(l, u, dag_flags, R);
     default: return 3;
     }
 }
@@ -578,6 +554,14 @@ int gho_dag_crown(const gh_prog *P, int B, const double *xL, const double *xU, d
     int V = P->V, n = P->n;
     uint8_t *needs = (uint8_t *)malloc((size_t)V);
     gho_dag_needs(P, needs);
+    /* the tangent tables are built once, before the threads read them */
+    int pow_mask = 0;
+    for (int v = n; v < V; ++v) {
+        const gh_node *nd = &P->nodes[v];
+        if (nd->op == GH_OP_POW && nd->c == floor(nd->c) && ((int)nd->c) % 2 == 1 && nd->c < 32)
+            pow_mask |= 1 << (int)nd->c;
+    }
+    gho_sshape_init(pow_mask);
 #pragma omp parallel
     {
         dag_ws w;
@@ -635,6 +619,7 @@ int gho_dag_eval(const gh_prog *P, int N_, const double *X, double *F)
             case GH_OP_LOG: y = log(a); break;
             case GH_OP_RECIP: y = 1.0 / a; break;
             case GH_OP_SIN: y = sin(a); break;
+            case GH_OP_COS: y = cos(a); break;
             default: y = NAN;
             }
             Y[v] = y;
@@ -646,7 +631,7 @@ int gho_dag_eval(const gh_prog *P, int N_, const double *X, double *F)
 }
 
 /* The shared elementary functions (gearhip_math.h) over an array, for their
- * accuracy tests: fn 0 exp, 1 expm1, 2 log, 3 sin, 4 cos, 10 + p x^p. */
+ * accuracy tests: fn 0 exp, 1 expm1, 2 log, 3 sin, 4 cos, 5 asin, 10 + p x^p. */
 int gho_math(int fn, int N, const double *x, double *y)
 {
     for (int k = 0; k < N; ++k) {
@@ -657,6 +642,7 @@ int gho_math(int fn, int N, const double *x, double *y)
         case 2: r = gh_log(v); break;
         case 3: r = gh_sin(v); break;
         case 4: r = gh_cos(v); break;
+        case 5: r = gh_asin(v); break;
         default:
             if (fn < 10) return 1;
             r = gh_powi(v, fn - 10);

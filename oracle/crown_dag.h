@@ -9,4 +9,5 @@ int gho_dag_ibp(const gh_prog *P, int B, const double *xL, const double *xU, dou
 int gho_dag_eval(const gh_prog *P, int N, const double *X, double *F);
 void gho_dag_needs(const gh_prog *P, uint8_t *needs);
 int gho_math(int fn, int N, const double *x, double *y);
+void gho_dag_set_flags(int flags);
 #endif

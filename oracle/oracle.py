@@ -4,7 +4,8 @@ Python face of the C restatements in oracle/crown_qcqp.c (FP64 CROWN of a
 QCQP with the reference's rounding enclosures, widest branching, bisection,
 prune) and oracle/crown_dag.c (FP64 CROWN of expression programs: per-node
 intermediate bounds, the reference's relaxations of mul / sqr / pow / exp /
-log / reciprocal / sin, interval fallback for rows with no relaxation).
+log / reciprocal / sin / cos -- the S-shaped ones with their tangent tables in
+oracle/sshaped.c -- interval fallback for rows with no relaxation).
 Only tests/, the smoke check and the benchmark's CPU-baseline leg may import
 this module; the product path never does.
 
@@ -15,12 +16,13 @@ enclosure margin; a single bilinear term gives the exact McCormick minimum),
 brute force on tiny inputs, and the reference's own explicit expectations
 for split / branch / prune (CV/tests/test_input_split_split.py and
 branching_domains.py semantics).  The program oracle is pinned in
-tests/test_oracle_dag.py: the reference's golden CROWN bounds of x^4 and x^6
-(solver/tests/data_64/pow_test_data, extracted to tests/golden/), closed
-forms of the exp / log / reciprocal / square planes, and sampled soundness of
-every operator and of random programs.  x^3 (exact tangent points instead of
-the reference's tables) and sin (secant / tangent on one convexity piece)
-are pinned by soundness only.
+tests/test_oracle_dag.py: the reference's golden CROWN bounds of x^4, x^6,
+non-integer powers and sin (solver/tests/data_64/pow_test_data and
+s_shape_test_data, extracted to tests/golden/), the tangent tables against
+the tangency conditions they solve, closed forms of the exp / log /
+reciprocal / square / cubic / cos planes, cancellation invariants of the
+intermediate bounds, and sampled soundness of every operator and of random
+programs.
 """
 import ctypes
 import os
@@ -135,18 +137,56 @@ def _dag_lib():
         h.gho_dag_crown.restype = ctypes.c_int
         h.gho_dag_ibp.argtypes = [ctypes.POINTER(_GhProg), ctypes.c_int, _D, _D, _D, _D]
         h.gho_dag_ibp.restype = ctypes.c_int
+        h.gho_dag_set_flags.argtypes = [ctypes.c_int]
+        h.gho_dag_set_flags.restype = None
+        h.gho_relax_unary.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_int, _D, _D, _D,
+                                      ctypes.c_int]
+        h.gho_relax_unary.restype = ctypes.c_int
+        h.gho_sshape_tables.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p]
+        h.gho_sshape_tables.restype = ctypes.c_int
         h._dag_ready = True
     return h
 
 
-def dag_crown(prog, xL, xU, threads=1):
+SIN_LEGACY_INDEX = 1   # sshaped.h GHO_SIN_LEGACY_INDEX
+
+
+def dag_crown(prog, xL, xU, threads=1, flags=0):
     """lb (B, R): FP64 oracle CROWN lower bound of every row of an
-    expression program (gear_amd.program.Program) over boxes xL, xU (B, n)."""
+    expression program (gear_amd.program.Program) over boxes xL, xU (B, n).
+    flags=SIN_LEGACY_INDEX rounds sin's right-endpoint table index down, as
+    the reference did when it wrote its golden sin bounds."""
     xL, xU = _c(np.atleast_2d(xL)), _c(np.atleast_2d(xU))
     st, keep = _prog_struct(prog)
     lb = np.empty((xL.shape[0], prog.R))
-    _dag_lib().gho_dag_crown(ctypes.byref(st), xL.shape[0], xL, xU, lb, threads)
+    h = _dag_lib()
+    h.gho_dag_set_flags(int(flags))
+    try:
+        h.gho_dag_crown(ctypes.byref(st), xL.shape[0], xL, xU, lb, threads)
+    finally:
+        h.gho_dag_set_flags(0)
     return lb
+
+
+def relax_unary(op, c, l, u, flags=0):
+    """(N, 4) relaxation planes (lw, lb, uw, ub) of sin / cos / an odd or
+    non-integer power over intervals [l, u], and the undefined sides."""
+    l, u = _c(np.atleast_1d(l)), _c(np.atleast_1d(u))
+    R = np.zeros((l.size, 4))
+    ud = _dag_lib().gho_relax_unary(int(op), float(c), l.size, l, u, R, int(flags))
+    return R, ud
+
+
+def sshape_tables(which, p=0):
+    """The tangent tables: which 0 -> sin (d_lower, d_upper) (2, 162);
+    1 -> sin endpoint tables (9, 1001): x, then (d, cycles) of xl_lower,
+    xl_upper, xu_lower, xu_upper; 2 -> odd power p (d_lower, d_upper)
+    (2, 50005)."""
+    h = _dag_lib()
+    n = h.gho_sshape_tables(which, p, None)
+    out = np.empty(n)
+    h.gho_sshape_tables(which, p, out.ctypes.data_as(ctypes.c_void_p))
+    return out.reshape(9 if which == 1 else 2, -1)
 
 
 def dag_ibp(prog, xL, xU):
@@ -160,8 +200,8 @@ def dag_ibp(prog, xL, xU):
 
 def math_fn(name, x):
     """The shared elementary functions of include/gearhip_math.h on an
-    array (name: exp, expm1, log, sin, cos, pow2 .. pow6)."""
-    codes = {"exp": 0, "expm1": 1, "log": 2, "sin": 3, "cos": 4}
+    array (name: exp, expm1, log, sin, cos, asin, pow2 .. pow6)."""
+    codes = {"exp": 0, "expm1": 1, "log": 2, "sin": 3, "cos": 4, "asin": 5}
     code = codes[name] if name in codes else 10 + int(name[3:])
     x = _c(x)
     y = np.empty_like(x)

@@ -99,6 +99,40 @@ def test_edge_program(cuda):
     _check(lb, O.dag_crown(P, L, U, threads=8), "edge program")
 
 
+def _sshape_program():
+    """sin / cos over wide and narrow intervals, odd and non-integer powers,
+    the operators whose relaxations read tangent tables."""
+    P = Program(3)
+    P.output(P.sin(P.mulc(0, 7.0)))                      # many periods
+    P.output(P.neg(P.sin(P.addc(1, 0.4))))               # upper planes
+    P.output(P.cos(P.mulc(2, 2.0)))
+    P.output(P.neg(P.cos(P.add(0, 1))))
+    P.output(P.pow(P.addc(0, -0.2), 5))                  # odd power 5
+    P.output(P.neg(P.pow(P.mulc(1, 3.0), 3)))
+    P.output(P.pow(P.addc(P.sqr(2), 0.1), 0.5))          # non-integer powers
+    P.output(P.neg(P.pow(P.addc(P.sqr(0), 0.01), 2.5)))
+    P.output(P.pow(P.addc(2, 3.5), 1.7))                 # domain cut at x >= 0
+    P.output(P.mul(P.sin(0), P.cos(P.mul(1, 2))))        # products of trig terms
+    P.output(P.add(P.pow(P.sin(2), 3), P.cos(P.pow(0, 3))))
+    return P
+
+
+@pytest.mark.parametrize("scale", [0.02, 1.0, 6.0])
+def test_sshape_program(cuda, scale):
+    """Tables and every branch of the sin / cos / odd / non-integer power
+    relaxations: the kernel equals the oracle value for value, on boxes from
+    1e-2 wide (one convexity piece) to several periods."""
+    print(f"case sshape program scale {scale}", flush=True)
+    P = _sshape_program()
+    rng = np.random.default_rng(int(scale * 100))
+    c = rng.uniform(-4, 4, (700, 3))
+    w = rng.uniform(0.1, 1.0, (700, 3)) * scale
+    L, U = c - w / 2, c + w / 2
+    L[:3] = U[:3]                                        # point boxes
+    lb = _bound([P], [(L, U)], cuda)[0]
+    _check(lb, O.dag_crown(P, L, U, threads=8), f"sshape program x{scale}")
+
+
 def test_uneven_instances_subset(cuda):
     insts = uneven_nlp_instances()[:24]
     rng = np.random.default_rng(0)

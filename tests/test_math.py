@@ -37,6 +37,8 @@ def _pts(rng, lo, hi, n, log=False):
     ("expm1", mpmath.expm1, 3.0, [(-50, 50, False), (-0.4, 0.4, False), (-12, -1, True)]),
     ("sin", mpmath.sin, 2.0, [(-10, 10, False), (-1e4, 1e4, False), (-12, -1, True)]),
     ("cos", mpmath.cos, 2.0, [(-10, 10, False), (-1e4, 1e4, False), (-12, -1, True)]),
+    ("asin", mpmath.asin, 2.5, [(-1, 1, False), (-0.5, 0.5, False), (0.49, 0.51, False),
+                                (0.999, 1.0, False), (-12, -1, True)]),
 ])
 def test_ulp_error(name, ref, bound, ranges):
     rng = np.random.default_rng(len(name))
@@ -60,6 +62,8 @@ def test_special_values():
     assert O.math_fn("expm1", [-50.0])[0] == -1.0
     assert np.isnan(O.math_fn("sin", [2.0 ** 51]))[0]
     assert O.math_fn("sin", [0.0])[0] == 0.0 and O.math_fn("cos", [0.0])[0] == 1.0
+    assert O.math_fn("asin", [1.0])[0] == math.pi / 2 and O.math_fn("asin", [-1.0])[0] == -math.pi / 2
+    assert np.isnan(O.math_fn("asin", [1.0 + 2 ** -52]))[0]
 
 
 @pytest.mark.parametrize("p", [2, 3, 4, 5, 6])

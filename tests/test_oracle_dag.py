@@ -1,6 +1,9 @@
 """The expression-program oracle pinned by: the reference's golden CROWN
-bounds of even powers, closed forms of the exp / log / reciprocal / square
-planes, per-operator and whole-program sampled soundness."""
+bounds of even powers, non-integer powers and sin; the tangent tables
+against the tangency conditions they approximate; closed forms of the exp /
+log / reciprocal / square / odd-power / cos planes; cancellation
+invariants that only CROWN intermediate bounds satisfy; per-operator and
+whole-program sampled soundness."""
 import json
 import math
 import os
@@ -82,11 +85,218 @@ def test_square_tangent_closed_form():
         assert lb <= exact and exact - lb < 1e-13 * (1 + abs(l) + abs(u)) ** 2
 
 
+def _sshape():
+    return json.load(open(os.path.join(HERE, "golden", "sshape_crown.json")))
+
+
+def _bounds(build, L, U, flags=0):
+    """(lower, upper) CROWN bounds of a one-input expression over intervals."""
+    lo = O.dag_crown(_single(build), np.asarray(L)[:, None], np.asarray(U)[:, None],
+                     flags=flags)[:, 0]
+    hi = -O.dag_crown(_single(lambda P: P.neg(build(P))), np.asarray(L)[:, None],
+                      np.asarray(U)[:, None], flags=flags)[:, 0]
+    return lo, hi
+
+
+def _close(a, b, tol=1e-9):
+    return np.abs(np.asarray(a) - np.asarray(b)) <= tol * np.maximum(1.0, np.abs(b))
+
+
+@pytest.mark.parametrize("grid", [0, 1])
+def test_sin_matches_reference_golden(grid):
+    """The reference's golden CROWN bounds of sin on 19 intervals of width pi
+    and 19 of width pi / 2 (solver/tests/data_64/s_shape_test_data entries 0
+    and 2), with the right-endpoint table index rounded down as the reference
+    rounded it when it wrote that file: 19 / 19 lower and upper bounds within
+    1e-9 (the difference is the roundoff margin added since, 8 eps (1 +
+    max |x|) <= 6e-14, plus the concretization enclosure)."""
+    g = _sshape()["sin"][grid]
+    lo, hi = _bounds(lambda P: P.sin(0), g["input_lb"], g["input_ub"], flags=O.SIN_LEGACY_INDEX)
+    assert _close(lo, g["lb"]).sum() == 19 and _close(hi, g["ub"]).sum() == 19
+    assert np.all(lo <= np.array(g["lb"]) + 1e-15) and np.all(hi >= np.array(g["ub"]) - 1e-15)
+    assert np.max(np.abs(lo - g["lb"])) < 1e-12 and np.max(np.abs(hi - g["ub"])) < 1e-12
+
+
+@pytest.mark.parametrize("grid", [0, 1])
+def test_sin_current_index_rounds_up(grid):
+    """The reference now reads the right-endpoint tangents one table entry
+    up (trigonometric.py:339-343), so they are valid up to the interval's
+    upper end.  On the golden grids its bounds equal the golden ones except
+    where a right-endpoint tangent is chosen: every entry of the pi / 2 grid,
+    16 / 19 lower and upper bounds of the pi grid; the other six move by at
+    most 2.2e-3.  Both forms are sound on a dense sample of every interval."""
+    g = _sshape()["sin"][grid]
+    lo, hi = _bounds(lambda P: P.sin(0), g["input_lb"], g["input_ub"])
+    lo0, hi0 = _bounds(lambda P: P.sin(0), g["input_lb"], g["input_ub"],
+                       flags=O.SIN_LEGACY_INDEX)
+    same = _close(lo, g["lb"]).sum(), _close(hi, g["ub"]).sum()
+    assert same == ((19, 19) if grid == 1 else (16, 16))
+    assert np.max(np.abs(lo0 - lo)) < 2.5e-3 and np.max(np.abs(hi - hi0)) < 2.5e-3
+    for k, (a, b) in enumerate(zip(g["input_lb"], g["input_ub"])):
+        f = np.sin(np.linspace(a, b, 20001))
+        assert f.min() >= max(lo[k], lo0[k]) - 1e-12 and f.max() <= min(hi[k], hi0[k]) + 1e-12
+
+
+def test_sin_d_tables_solve_the_tangency_condition():
+    """BoundSShaped.precompute_relaxation for sin (x_limit pi / 2): entry i
+    of d_lower is the tangent point d <= 0 whose tangent passes through
+    (0.01 i, sin(0.01 i)) -- the bisection's last point on the sound side,
+    as close to the root found here by Brent's method as the condition
+    number of the tangency equation allows (a few eps / |g'(root)|); d_upper
+    the same for the points (-0.01 i, sin(-0.01 i)) with d >= 0."""
+    from scipy.optimize import brentq
+    dl, du = O.sshape_tables(0)
+    assert dl.shape == (162,) and du.shape == (162,)
+    eps = 2.0 ** -52
+    for i in range(1, 162):
+        x = 0.01 * i
+        g = lambda d: math.cos(d) * (x - d) + math.sin(d) - math.sin(x)
+        root = brentq(g, -2.0 * x - 1, -1e-300, xtol=1e-18)
+        tol = 16 * eps / abs(math.sin(root) * (x - root)) + 4 * eps
+        assert abs(dl[i] - root) < tol, i
+        assert g(dl[i]) <= 4 * eps                     # sound side, up to libm rounding
+        h = lambda d: math.cos(d) * (-x - d) + math.sin(d) - math.sin(-x)
+        root = brentq(h, 1e-300, 2.0 * x + 1, xtol=1e-18)
+        tol = 16 * eps / abs(math.sin(root) * (x + root)) + 4 * eps
+        assert abs(du[i] - root) < tol, i
+        assert h(du[i]) >= -4 * eps                    # the line is above sin(-x)
+
+
+def test_sin_endpoint_tables_are_tight_tangents():
+    """get_lower_left_bound & co over linspace(0, 2 pi, 1001): each table
+    tangent is valid (its line stays on the right side of sin over the
+    whole period it is meant for) and the bisection's last step was the
+    tight one (the point 2^-20 pi / 2 closer to the boundary is not)."""
+    T = O.sshape_tables(1)
+    x = T[0]
+    assert x[0] == 0.0 and x[-1] == 2 * math.pi and np.all(np.diff(x) > 0)
+    step = (math.pi / 2) / 2 ** 20
+    for which, (d, cyc) in enumerate(((T[1], T[2]), (T[3], T[4]))):
+        for i in range(0, 1001, 7):
+            xl = x[i] - cyc[i]
+            t = d[i]
+            if (which == 0 and xl >= math.pi) or (which == 1 and xl >= 2 * math.pi):
+                assert t == xl
+                continue
+            lo_end, hi_end = (math.pi, 1.5 * math.pi) if which == 0 else (2 * math.pi, 2.5 * math.pi)
+            assert lo_end <= t <= hi_end
+            margin = lambda tt: math.sin(xl) - (math.cos(tt) * (xl - tt) + math.sin(tt))
+            if which == 0:
+                # lower: the tangent at t is below sin at xl; one step left it is not
+                assert margin(t) > 0 and (t == hi_end or margin(t - 2 * step) <= 1e-15)
+            else:
+                assert margin(t) <= 0 and (t == hi_end or margin(t - 2 * step) > -1e-15)
+
+
+def test_cos_closed_form_on_convexity_pieces():
+    """cos on one concave piece (|x| < pi/2) and on one convex piece: the
+    secant and the midpoint tangent, shifted as BoundCos shifts sin's
+    relaxation (trigonometric.py:509-527)."""
+    for l, u in [(-0.5, 1.2), (-1.2, -0.1), (0.2, 1.4)]:
+        R, ud = O.relax_unary(14, 0.0, l, u)
+        assert ud == 0
+        ks = (math.cos(u) - math.cos(l)) / (u - l)
+        m = (l + u) / 2
+        # concave piece: lower = secant, upper = tangent at the midpoint
+        assert R[0, 0] == pytest.approx(ks, abs=1e-13)
+        assert R[0, 1] == pytest.approx(math.cos(l) - ks * l, abs=1e-13)
+        assert R[0, 2] == pytest.approx(-math.sin(m), abs=1e-13)
+        assert R[0, 3] == pytest.approx(math.cos(m) + math.sin(m) * m, abs=1e-13)
+    for l, u in [(1.8, 4.4), (2.0, 3.0)]:
+        R, ud = O.relax_unary(14, 0.0, l, u)
+        ks = (math.cos(u) - math.cos(l)) / (u - l)
+        m = (l + u) / 2
+        # convex piece: upper = secant, lower = tangent at the midpoint
+        assert R[0, 2] == pytest.approx(ks, abs=1e-13)
+        assert R[0, 3] == pytest.approx(math.cos(l) - ks * l, abs=1e-13)
+        assert R[0, 0] == pytest.approx(-math.sin(m), abs=1e-13)
+        assert R[0, 1] == pytest.approx(math.cos(m) + math.sin(m) * m, abs=1e-13)
+
+
+def test_pow_tables_are_the_cubic_tangent_points():
+    """BoundPow.precompute_relaxation for x^3 (x_limit 500, 50,005 entries):
+    the tangent at d meets x^3 again at -2 d, so the exact tangent point for
+    the grid point 0.01 i is -0.01 i / 2 (d_upper) and +0.01 i / 2
+    (d_lower); the bisection ends within an ulp or two of it."""
+    dl, du = O.sshape_tables(2, 3)
+    i = np.arange(50005)
+    x = 0.01 * i
+    assert np.all(np.abs(du - (-x / 2)) <= 4e-16 * (1 + x))
+    assert np.all(np.abs(dl - (x / 2)) <= 4e-16 * (1 + x))
+
+
+@pytest.mark.parametrize("l,u", [(-0.7, 0.4), (-0.2, 0.9), (-3.0, 2.0), (0.5, 2.0), (-2.0, -0.5)])
+def test_cubic_planes_closed_form(l, u):
+    """x^3 planes (s_shaped.py:325-549, inverse S shape): on an interval
+    through 0, tangents through the grid points at or above the ends
+    (0.01 (floor(|end| / 0.01) + 1)), unless the smoothness rule takes the
+    midpoint tangent; on one side of 0 the secant and the midpoint tangent."""
+    R, ud = O.relax_unary(9, 3.0, l, u)
+    assert ud == 0
+    lw, lb, uw, ub = R[0]
+    m = (l + u) / 2
+    tang = lambda d: (3 * d * d, -2 * d ** 3)
+    ks = (u ** 3 - l ** 3) / (u - l)
+    sec = (ks, l ** 3 - ks * l)
+    if l < 0 < u:
+        gl = 0.01 * (math.floor(-l / 0.01) + 1)
+        gu = 0.01 * (math.floor(u / 0.01) + 1)
+        d_lo, d_up = gl / 2, -gu / 2
+        want_l = tang(m) if d_lo < m else (sec if ks > 3 * u * u else tang(d_lo))
+        want_u = tang(m) if d_up >= m else (sec if ks > 3 * l * l else tang(d_up))
+    elif l >= 0:
+        want_l, want_u = tang(m), sec
+    else:
+        want_l, want_u = sec, tang(m)
+    assert (lw, lb) == pytest.approx(want_l, abs=1e-12)
+    assert (uw, ub) == pytest.approx(want_u, abs=1e-12)
+
+
+def test_noninteger_pow_matches_reference_golden():
+    """x^p for non-integer p on the six boxes of the reference's pow test
+    (solver/tests/data_64/pow_test_data entries 0-11): IBP within 1e-14,
+    CROWN within 1e-9, and the CROWN bounds outward of the golden ones."""
+    for c in _sshape()["pow_noninteger"]:
+        p, a, b = c["exponent"], c["x_low"], c["x_high"]
+        lo, hi = _bounds(lambda P: P.pow(0, p), [a], [b])
+        assert _close(lo, c["crown"][0]).all() and _close(hi, c["crown"][1]).all(), c
+        assert lo[0] <= c["crown"][0] + 1e-15 and hi[0] >= c["crown"][1] - 1e-15
+        P = _single(lambda P: P.pow(0, p))
+        ilo, ihi = O.dag_ibp(P, np.array([[a]], float), np.array([[b]], float))
+        assert abs(ilo[0, 0] - c["ibp"][0]) <= 1e-14 * (1 + abs(c["ibp"][0]))
+        assert abs(ihi[0, 0] - c["ibp"][1]) <= 1e-14 * (1 + abs(c["ibp"][1]))
+
+
+def test_cancellation_needs_crown_intermediates():
+    """Intermediate bounds come from CROWN intersected with IBP: the input
+    of sqr in sqr(x0 + x1 - x1) is bounded like x0 itself (IBP alone would
+    add the width of x1 twice), and exp(x - x) is bounded by 1 up to the
+    enclosure margins."""
+    rng = np.random.default_rng(7)
+    for _ in range(40):
+        L = rng.uniform(-2, 1, 2)
+        U = L + rng.uniform(0.1, 3, 2)
+        P = Program(2)
+        P.output(P.sqr(P.sub(P.add(0, 1), 1)))
+        P.output(P.sqr(0))
+        lb = O.dag_crown(P, L[None], U[None])[0]
+        assert abs(lb[0] - lb[1]) <= 1e-12 * (1 + np.max(np.abs(np.r_[L, U]))) ** 2
+        P = Program(1)
+        P.output(P.exp(P.sub(0, 0)))
+        P.output(P.neg(P.exp(P.sub(0, 0))))
+        lb = O.dag_crown(P, L[:1, None], U[:1, None])[0]
+        assert lb[0] <= 1.0 and 1.0 - lb[0] < 1e-13 * (1 + abs(L[0]) + abs(U[0]))
+        assert -lb[1] >= 1.0 and -lb[1] - 1.0 < 1e-13 * (1 + abs(L[0]) + abs(U[0]))
+
+
 OPS = {
     "exp": (lambda P: P.exp(0), 1, False), "log": (lambda P: P.log(0), 1, True),
     "recip": (lambda P: P.recip(0), 1, True), "sqr": (lambda P: P.sqr(0), 1, False),
     "pow3": (lambda P: P.pow(0, 3), 1, False), "pow4": (lambda P: P.pow(0, 4), 1, False),
     "sin": (lambda P: P.sin(P.mulc(0, 3.0)), 1, False), "mul": (lambda P: P.mul(0, 1), 2, False),
+    "cos": (lambda P: P.cos(P.mulc(0, 3.0)), 1, False), "pow5": (lambda P: P.pow(0, 5), 1, False),
+    "pow_half": (lambda P: P.pow(0, 0.5), 1, True), "pow_2p5": (lambda P: P.pow(0, 2.5), 1, True),
+    "pow_0p3": (lambda P: P.pow(0, 0.3), 1, True), "sin_wide": (lambda P: P.sin(P.mulc(0, 9.0)), 1, False),
     "div": (lambda P: P.div(0, P.addc(P.sqr(1), 1.0)), 2, False),
     "sin_of_product": (lambda P: P.sin(P.mul(0, 1)), 2, False),
     "log1p_sq": (lambda P: P.log(P.addc(P.sqr(0), 1.0)), 1, False),
