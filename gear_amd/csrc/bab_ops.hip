@@ -3,6 +3,8 @@
 // The predicates and the arithmetic follow the reference step functions
 // cited in include/gearhip.h; the oracle (oracle/crown_qcqp.c) restates the
 // same rules and the tests compare the two bit for bit.
+#include <mutex>
+
 #include "common.h"
 #include "gearhip_internal.h"
 
@@ -1400,12 +1402,13 @@ extern "C" size_t gh_split_append_workspace_bytes(int K)
     return sizeof(double) * 2 * (size_t)K + sizeof(int32_t) * (3 * (size_t)K + 2 * nblk) + 256;
 }
 
-extern "C" int gh_split_append_f64(int K, int n, int S, int R, const double *pL, const double *pU,
-                                   const int64_t *prows, const double *plb, const double *lA,
-                                   const double *lbias, const double *thr, double *FL, double *FU,
-                                   double *Flb, int64_t base, int64_t cap, int64_t *order_out,
-                                   int64_t dead, int64_t *total, double *disc_min, void *ws,
-                                   hipStream_t stream)
+namespace {
+
+int split_append_impl(int K, int n, int S, int R, const double *pL, const double *pU,
+                      const int64_t *prows, const double *plb, const double *lA,
+                      const double *lbias, const double *thr, double *FL, double *FU, double *Flb,
+                      int64_t base, int64_t cap, int64_t *order_out, int64_t dead, int64_t *total,
+                      double *disc_min, void *ws, hipStream_t stream)
 {
     if (K < 0 || n < 1 || S < 1 || base < 0 || !total || !disc_min) return GH_EINVAL;
     if (lA && (R < 1 || R > S || !lbias)) return GH_EINVAL;
@@ -1437,3 +1440,72 @@ extern "C" int gh_split_append_f64(int K, int n, int S, int R, const double *pL,
                            dead);
     return hipGetLastError() == hipSuccess ? GH_OK : GH_ELAUNCH;
 }
+
+// Scratch of the allocation-free split-append: the workspace layout above
+// for the largest K it accepts, in a file-scope device array (one copy per
+// GPU, held until the process ends).  Host code never names the array: it
+// uses the address hipGetSymbolAddress returns for the current device.
+constexpr int kNoAllocMaxParents = 1 << 17;
+constexpr size_t kNoAllocBytes =
+    sizeof(double) * 2 * (size_t)kNoAllocMaxParents +
+    sizeof(int32_t) * (3 * (size_t)kNoAllocMaxParents +
+                       2 * (((size_t)kNoAllocMaxParents + kSplitBlock - 1) / kSplitBlock)) +
+    256;
+
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices];
+
+// The current device's address of `scratch`, looked up once per device.
+int scratch_address(void **out)
+{
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess) return GH_ELAUNCH;
+    if (dev < 0 || dev >= kMaxDevices) return GH_EUNSUPPORTED;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return GH_ELAUNCH;
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return GH_OK;
+}
+
+}  // namespace
+
+extern "C" int gh_split_append_f64(int K, int n, int S, int R, const double *pL, const double *pU,
+                                   const int64_t *prows, const double *plb, const double *lA,
+                                   const double *lbias, const double *thr, double *FL, double *FU,
+                                   double *Flb, int64_t base, int64_t cap, int64_t *order_out,
+                                   int64_t dead, int64_t *total, double *disc_min, void *ws,
+                                   hipStream_t stream)
+{
+    return split_append_impl(K, n, S, R, pL, pU, prows, plb, lA, lbias, thr, FL, FU, Flb, base,
+                             cap, order_out, dead, total, disc_min, ws, stream);
+}
+
+extern "C" int gh_split_append_f64_noalloc(int K, int n, int S, int R, const double *pL,
+                                           const double *pU, const int64_t *prows,
+                                           const double *plb, const double *lA,
+                                           const double *lbias, const double *thr, double *FL,
+                                           double *FU, double *Flb, int64_t base, int64_t cap,
+                                           int64_t *order_out, int64_t dead, int64_t *total,
+                                           double *disc_min, hipStream_t stream)
+{
+    if (K > kNoAllocMaxParents) return GH_EUNSUPPORTED;
+    if (K < 0) return GH_EINVAL;
+    void *ws = nullptr;
+    int rc = scratch_address(&ws);
+    if (rc != GH_OK) return rc;
+    return split_append_impl(K, n, S, R, pL, pU, prows, plb, lA, lbias, thr, FL, FU, Flb, base,
+                             cap, order_out, dead, total, disc_min, ws, stream);
+}
+
+extern "C" int gh_split_append_noalloc_max_parents(void) { return kNoAllocMaxParents; }

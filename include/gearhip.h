@@ -426,6 +426,27 @@ int gh_gather_rows_f64(int64_t N, const int64_t *idx, int w0, const double *s0, 
                        int w1, const double *s1, double *d1, int w2, const double *s2,
                        double *d2, hipStream_t stream);
 
+/*
+ * gh_split_append_f64 without a workspace argument, for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  No counterpart in the reference.  The
+ * scratch lives in a fixed device array of the library (3.7 MB per GPU,
+ * held until the process ends), so K may be at most
+ * gh_split_append_noalloc_max_parents() (131,072 parents, 262,144
+ * children); larger K returns GH_EUNSUPPORTED and launches nothing.  The
+ * call allocates nothing.  Every call on one GPU shares that GPU's scratch:
+ * calls that overlap on one GPU (other streams or threads) race and must be
+ * serialized by the caller.  Results equal gh_split_append_f64's bit for
+ * bit.
+ */
+int gh_split_append_f64_noalloc(int K, int n, int S, int R, const double *pL, const double *pU,
+                                const int64_t *prows, const double *plb, const double *lA,
+                                const double *lbias, const double *thr, double *FL, double *FU,
+                                double *Flb, int64_t base, int64_t cap, int64_t *order_out,
+                                int64_t dead, int64_t *total, double *disc_min,
+                                hipStream_t stream);
+int gh_split_append_noalloc_max_parents(void);
+
 /* ------------------------------------------------ expression programs */
 
 /*
