@@ -120,7 +120,15 @@ class BaBConfig:
     constr_viol_tol: float = 1e-6
     max_iters: int = 1000
     time_limit: float = 60.0
-    alm_interval: int = 0          # 0: ALM only at the root
+    # ALM-PGD in the loop (in_bab of the GEAR presets, nlp_in_bab.py:
+    # 1078-1231): every alm_interval iterations on the alm_boxes least
+    # verified domains, the interval doubling (to alm_max_interval) after
+    # alm_interval_patience calls without a better incumbent and reset by
+    # one; 0: ALM only at the root
+    alm_interval: int = 5
+    alm_max_interval: int = 20
+    alm_interval_patience: int = 3
+    alm_boxes: int = 64
     sort_interval: int = 1         # sort the frontier every k iterations (0: never)
     # LP dual certificate on the bounded domains' CROWN planes: PDHG
     # iterations, 0 to skip (bounding.py:489-520, dual_lp_pdhg_iters)
@@ -306,6 +314,7 @@ class QCQPBranchAndBound:
         cfg = self.cfg
         t0 = time.time()
         self.root()
+        every, fails = cfg.alm_interval, 0
         self.stats.stop = "iterations"
         while len(self.frontier) > 0 and self.stats.iterations < cfg.max_iters:
             if is_primal_dual_converged(self.P, self.D, cfg.abs_tol, cfg.rel_tol):
@@ -314,12 +323,20 @@ class QCQPBranchAndBound:
             if time.time() - t0 > cfg.time_limit:
                 self.stats.stop = "time"
                 break
-            if cfg.alm_interval and self.stats.iterations % cfg.alm_interval == 0 and \
-                    self.stats.iterations > 0:
-                k = min(len(self.frontier), 64)
+            if cfg.alm_interval and self.stats.iterations > 0 and \
+                    self.stats.iterations % every == 0:
+                k = min(len(self.frontier), cfg.alm_boxes)
                 s = len(self.frontier) - k
                 wL, wU = self.frontier.rows(s, s + k)[:2]
-                self.run_alm(wL, wU, restarts=8)
+                P0 = self.P
+                self.run_alm(wL, wU)
+                # adaptive interval (nlp_in_bab.py:1200-1215)
+                if self.P < P0:
+                    every, fails = cfg.alm_interval, 0
+                else:
+                    fails += 1
+                    if fails >= cfg.alm_interval_patience:
+                        every, fails = min(every * 2, cfg.alm_max_interval), 0
             if self.step() == 0:
                 # no room for one batch of children: the frontier is full
                 self.stats.stop = "frontier_full"

@@ -73,7 +73,7 @@ class ServeConfig:
     step_flops: float = 4e9       # bound work per step that sets the batch size
     max_batch: int = 8192
     min_batch: int = 16
-    alm_restarts: int = 32
+    alm_restarts: int = None        # None: the ALM's shared restart budget
     alm: ALMConfig = field(default_factory=ALMConfig)
     # ALM on the least verified frontier boxes every alm_interval steps, and
     # every alm_interval_noinc steps while there is no incumbent
@@ -81,7 +81,7 @@ class ServeConfig:
     alm_interval: int = 50
     alm_interval_noinc: int = 5
     alm_boxes: int = 8
-    alm_box_restarts: int = 4
+    alm_box_restarts: int = None    # None: the ALM's shared restart budget
     # FP64 LP dual certificate on the bounded domains' FP32-sound planes,
     # merged in solving mode (bounding.py:489-520, 698-723); PDHG
     # iterations, 0 to skip

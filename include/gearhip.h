@@ -139,30 +139,46 @@ int gh_qcqp_crown_f32s_large(int B, int n, int S, const void *plan, const double
                              size_t ws_bytes, hipStream_t stream);
 
 /*
- * ALM-PGD primal search, fused: N independent restarts of the reference's
+ * ALM-PGD primal search, fused: N restarts of the reference's
  * ALMPGDSolver.solve_with_restarts (complete_verifier/nlp_solver/
- * alm.py:1268-2065; inner Adam :551-788, AdamClipping_NLP.step :134-160,
- * ALMConfig :163-268) on a QCQP, one wavefront per restart, every outer and
- * inner iteration in one launch.
+ * alm.py:1268-2065; inner Adam _solve_subproblem_adam :551-788,
+ * AdamClipping_NLP.step :134-160, ALMConfig :163-268) on a QCQP, every
+ * outer and inner iteration in one launch.  Each restart runs as the
+ * reference runs one restart of one domain (batch 1, 1 restart): its inner
+ * loop ends at its own stationarity (gradient norm <= eta_k, :764-776) or
+ * L plateau (:678-699); its outer loop at its own objective patience or
+ * first-feasibility patience (:1800-1849); the multipliers, rho and the
+ * tolerances eps_k / eta_k follow :1851-1914 (update_multipliers_always
+ * selects the root-mode rule, else the gated BaB-mode rule).  The reference
+ * couples restarts only through those early exits, whose .all() over the
+ * batch is not reproduced.
  *   plan: gh_alm_plan_bytes(n, S) bytes, filled once by gh_alm_prepare from
  *   Q (S, n, n) (the symmetric parts (Q + Q^T) / 2);
  *   Cm (S, n), d (S), rhs (S): objective / constraint data (rhs[0] unused);
  *   lo, hi (N, n): each restart's box; x (N, n): starting points in, last
  *   iterates out; best_x (N, n), best_obj (N): each restart's best feasible
  *   point (violation <= feas_tol) and objective (+inf if none);
- *   cfg: HOST pointer; lr0 = inner_lr * the largest box width over all N.
+ *   cfg: HOST pointer;
+ *   trace (may be NULL): N x outer_max_iter records of
+ *   gh_alm_trace_width(S) doubles: rho, eps_k, eta_k after the update,
+ *   the violation, inner steps taken, a flag (0 updated, 1 stopped at this
+ *   iteration before the update, 2 not run), the last inner gradient norm,
+ *   then mu (S - 1) after the update and g = f - rhs (S - 1) before it.
  * Supported: n <= 1024, S <= 1024.
  */
 typedef struct {
     double rho0, kappa, max_rho, inner_lr, feas_tol, beta1, beta2, adam_eps;
-    double lr0;
-    int32_t outer_max_iter, inner_steps, scale_eps_with_bias, pad;
+    double alpha, beta, eps_final, eta_final;
+    double inner_lr_outer_decay, inner_lr_min_factor, obj_improvement_tol, inner_plateau_rtol;
+    int32_t outer_max_iter, inner_steps, scale_eps_with_bias, update_multipliers_always;
+    int32_t obj_patience, first_feas_patience, inner_plateau_check, inner_plateau_patience;
 } gh_alm_config;
 size_t gh_alm_plan_bytes(int n, int S);
+int gh_alm_trace_width(int S);
 int gh_alm_prepare(int n, int S, const double *Q, void *plan, hipStream_t stream);
 int gh_alm_pgd_f64(int N, int n, int S, const void *plan, const double *Cm, const double *d,
                    const double *rhs, const double *lo, const double *hi, double *x,
-                   double *best_x, double *best_obj, const gh_alm_config *cfg,
+                   double *best_x, double *best_obj, const gh_alm_config *cfg, double *trace,
                    hipStream_t stream);
 
 /*

@@ -366,3 +366,125 @@ def child_keys(plb, lA, lbias, cL, cU):
     ok = np.all(cL <= cU, axis=1) & np.isfinite(pm)
     keys[ok, 0] = np.maximum(keys[ok, 0], pm[ok])
     return keys
+
+
+# ------------------------------------------------------------- ALM schedule
+def _rpow(rho, a):
+    return np.sqrt(rho) if a == 0.5 else rho ** a
+
+
+def alm_schedule_check(cfg, trace, S, rtol_pow=2e-15):
+    """Replays the outer schedule of ALM-PGD restarts from their trace and
+    lists every disagreement (empty when the kernel followed it).
+
+    trace (N, outer_max_iter + 1, 8 + 2 (S - 1)): record 0 the start, record
+    k + 1 after outer iteration k: rho, eps_k, eta_k after the update, the
+    violation, the objective, inner steps, flag (0 updated, 1 stopped before
+    the update, 2 not run), the last inner gradient norm, mu after the
+    update, g = f - rhs before it.  From the recorded objective and g alone
+    it restates, per restart as the reference runs one restart of one
+    domain (complete_verifier/nlp_solver/alm.py):
+      * start: rho0, eps = rho^-alpha, eta = rho^-beta, mu = 0; the start
+        point is the first best point (:1380-1390, 1505-1522);
+      * the violation max(0, max_s g_s) (:1633-1645);
+      * best point: feasible first, else least violated (:1650-1700);
+      * exits: objective patience, then first-feasibility patience
+        (:1800-1849);
+      * BaB mode (update_multipliers_always false): where the violation is
+        within eps_k, mu <- max(mu + rho g, 0) and eps_k, eta_k divided by
+        rho; else rho <- min(kappa rho, max_rho) when the violation exceeds
+        1e-3 and eps_k, eta_k reset to rho^-alpha, rho^-beta (:1884-1907);
+        root mode: mu updated every iteration, rho raised when the
+        violation fell by less than 1.25x and exceeds feas_tol (:1857-1883);
+        both floored at eps_final, eta_final (:1913-1914);
+      * an inner loop that ends early ended at its stationarity test
+        (gradient norm <= eta_k of that iteration, :764-776) or at a plateau
+        check (:678-699).
+    rho and mu must match exactly; eps_k, eta_k within rtol_pow (x^0.8 is a
+    library power)."""
+    tr = np.asarray(trace, dtype=np.float64)
+    m = S - 1
+    probs = []
+    tol = cfg.feas_tol
+    for r in range(tr.shape[0]):
+        R = tr[r]
+        rho = cfg.rho0
+        eps, eta = 1.0 / _rpow(rho, cfg.alpha), 1.0 / _rpow(rho, cfg.beta)
+        mu = np.zeros(m)
+        rec = R[0]
+        g = rec[8 + m:8 + 2 * m]
+        viol = max(0.0, float(g.max())) if m else 0.0
+        if rec[3] != viol:
+            probs.append((r, 0, "start violation", rec[3], viol))
+        best, best_viol, best_feas = rec[4], viol, viol <= tol
+        prev_best, glob, prev_viol, no_imp, plat = np.inf, np.inf, np.inf, 0, -1
+        for name, got, want in (("rho", rec[0], rho), ("eps", rec[1], eps), ("eta", rec[2], eta)):
+            if abs(got - want) > rtol_pow * abs(want):
+                probs.append((r, 0, name, got, want))
+        stopped = False
+        for k in range(cfg.outer_max_iter):
+            rec = R[k + 1]
+            flag = int(rec[6])
+            if stopped:
+                if flag != 2:
+                    probs.append((r, k, "ran after its stop", flag, 2))
+                continue
+            if flag == 2:
+                probs.append((r, k, "not run before a stop", flag, 0))
+                break
+            g = rec[8 + m:8 + 2 * m]
+            phi = rec[4]
+            viol = max(0.0, float(g.max())) if m else 0.0
+            if rec[3] != viol:
+                probs.append((r, k, "violation", rec[3], viol))
+            steps, gnorm = int(rec[5]), rec[7]
+            if steps < cfg.inner_steps:
+                plateau_ok = (cfg.inner_plateau_check > 0 and steps % cfg.inner_plateau_check == 0
+                              and steps >= cfg.inner_plateau_check * (cfg.inner_plateau_patience + 1))
+                if not (gnorm <= eta or plateau_ok):
+                    probs.append((r, k, "early inner exit", steps, gnorm))
+            feas = viol <= tol
+            improved = feas and (not best_feas or phi < best)
+            better = (not feas) and (not best_feas) and viol < best_viol
+            if improved or better:
+                best, best_viol = phi, viol
+                best_feas = best_feas or improved
+            stop = False
+            if cfg.obj_patience >= 0:
+                no_imp = 0 if best < prev_best - cfg.obj_improvement_tol else no_imp + 1
+                prev_best = min(prev_best, best)
+                stop = no_imp >= cfg.obj_patience and best_feas
+            if not stop and cfg.first_feas_patience >= 0 and best_feas:
+                if plat < 0 or best < glob - cfg.obj_improvement_tol:
+                    plat, glob = k, best
+                elif k - plat >= cfg.first_feas_patience:
+                    stop = True
+            if stop != (flag == 1):
+                probs.append((r, k, "stop", flag, stop))
+            if not stop:
+                if cfg.update_multipliers_always:
+                    mu = np.maximum(mu + rho * g, 0.0)
+                    if k >= 1:
+                        ratio = prev_viol / max(viol, 1e-12)
+                        if ratio < 1.25 and viol > tol:
+                            rho = min(cfg.kappa * rho, cfg.max_rho)
+                    eps, eta = 1.0 / _rpow(rho, cfg.alpha), 1.0 / _rpow(rho, cfg.beta)
+                    prev_viol = viol
+                elif viol <= eps:
+                    mu = np.maximum(mu + rho * g, 0.0)
+                    eps, eta = eps / rho, eta / rho
+                else:
+                    if viol > 1e-3:
+                        rho = min(cfg.kappa * rho, cfg.max_rho)
+                    eps, eta = 1.0 / _rpow(rho, cfg.alpha), 1.0 / _rpow(rho, cfg.beta)
+                eps, eta = max(eps, cfg.eps_final), max(eta, cfg.eta_final)
+            else:
+                stopped = True
+            if rec[0] != rho:
+                probs.append((r, k, "rho", rec[0], rho))
+            for name, got, want in (("eps", rec[1], eps), ("eta", rec[2], eta)):
+                if abs(got - want) > rtol_pow * abs(want):
+                    probs.append((r, k, name, got, want))
+            if m and not np.array_equal(rec[8:8 + m], mu):
+                probs.append((r, k, "mu", rec[8:8 + m], mu))
+    return probs

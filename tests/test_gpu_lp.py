@@ -120,8 +120,10 @@ def test_lp_certificate_cuts_with_binding_constraints(cuda, n, m):
 
 def test_bab_with_lp_pruning_keeps_the_optimum(cuda):
     """B&B with the LP certificate in the prune closes the gap (the plain
-    CROWN prune does not within the same iteration budget), keeps the same
-    incumbent, never certifies above it, and bounds fewer boxes."""
+    CROWN prune does not within the same iteration budget), never certifies
+    above either run's incumbent, and bounds fewer boxes (the in-loop ALM
+    searches different boxes in the two runs, so their incumbents agree only
+    to the ALM's accuracy)."""
     from gear_amd.bab import BaBConfig, QCQPBranchAndBound
     from gear_amd.problems import QCQP
     rng = np.random.default_rng(8)
@@ -142,8 +144,8 @@ def test_bab_with_lp_pruning_keeps_the_optimum(cuda):
               flush=True)
     from gear_amd.bab import is_primal_dual_converged
     (P0, D0, n0), (P1, D1, n1) = out[0], out[300]
-    assert D1 <= P1 + 1e-9 and D0 <= P0 + 1e-9
-    assert P1 == P0
+    assert D1 <= min(P0, P1) + 1e-9 and D0 <= min(P0, P1) + 1e-9
+    assert abs(P1 - P0) <= 1e-3 * (1 + abs(P0))
     assert is_primal_dual_converged(P1, D1, 1e-5, 1e-4)
     assert D0 <= D1 + 1e-9 * max(1.0, abs(P0))
     assert n1 <= n0
