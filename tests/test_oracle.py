@@ -80,6 +80,35 @@ def test_single_bilinear_term_is_mccormick(box):
     assert lb <= min(x * y for x in (a, b) for y in (c, d))
 
 
+@pytest.mark.parametrize("seed", range(6))
+def test_dense_row_bilinear_uses_exact_linear_interval(seed):
+    """f = x_i (q^T x) with q_i = 0 and 19 other nonzero q_j: the first
+    Linear layer's interval of y = q^T x is exact, [sum q_j c_j - sum |q_j|
+    r_j, + ...] (IBP of one linear form over a box), and y does not depend
+    on x_i, so the CROWN bound is the McCormick rectangle minimum on
+    [l_i, u_i] x [y_lo, y_hi], computed here in rational arithmetic -- a
+    wrong centre or a radius without |q_j| moves it far outside the margin."""
+    rng = np.random.default_rng(seed)
+    n, i = 20, seed % 20
+    q = rng.standard_normal(n)
+    q[i] = 0.0
+    Q = np.zeros((1, n, n))
+    Q[0, i] = q
+    xl = rng.uniform(-2, 0.5, n)
+    xu = xl + rng.uniform(0.1, 2.5, n)
+    p = QCQP(Q, np.zeros((1, n)), np.zeros(1), np.zeros(1), xl, xu)
+    lb = O.qcqp_crown(p.Q, p.c, p.d, xl[None], xu[None])[0, 0]
+    F = Fraction
+    c = [(F(a) + F(b)) / 2 for a, b in zip(xl, xu)]
+    r = [(F(b) - F(a)) / 2 for a, b in zip(xl, xu)]
+    yc = sum(F(qj) * cj for qj, cj in zip(q, c))
+    yr = sum(abs(F(qj)) * rj for qj, rj in zip(q, r))
+    exact = _mccormick_box_min(xl[i], xu[i], yc - yr, yc + yr)
+    scale = 1 + float(sum(abs(F(qj)) for qj in q)) * 4 * float(max(map(abs, np.r_[xl, xu]))) ** 2
+    assert lb <= float(exact)
+    assert float(exact) - lb <= 1e-13 * scale, (lb, float(exact))
+
+
 def test_brute_force_tiny():
     """n = 2: the bound is below the minimum over a dense grid."""
     p = _problem(2, 2, 7, box=2.0)

@@ -289,6 +289,99 @@ def test_cancellation_needs_crown_intermediates():
         assert -lb[1] >= 1.0 and -lb[1] - 1.0 < 1e-13 * (1 + abs(L[0]) + abs(U[0]))
 
 
+def _linear_form(P, node):
+    """Exact (coefficients, constant) of a linear program node, in Fractions."""
+    from fractions import Fraction as F
+    from gear_amd.program import ADD, ADDC, CONST, INPUT, MULC, NEG, SUB
+    forms = []
+    for op, a, b, c in P.nodes:
+        if op == INPUT:
+            f = ([F(int(j == a)) for j in range(P.n)], F(0))
+        elif op == CONST:
+            f = ([F(0)] * P.n, F(c))
+        elif op in (ADD, SUB):
+            s = 1 if op == ADD else -1
+            f = ([x + s * y for x, y in zip(forms[a][0], forms[b][0])], forms[a][1] + s * forms[b][1])
+        elif op == NEG:
+            f = ([-x for x in forms[a][0]], -forms[a][1])
+        elif op == MULC:
+            f = ([F(c) * x for x in forms[a][0]], F(c) * forms[a][1])
+        elif op == ADDC:
+            f = (list(forms[a][0]), forms[a][1] + F(c))
+        else:
+            raise ValueError(op)
+        forms.append(f)
+    return forms[node]
+
+
+def _encloses(P, L, U):
+    """The program's rows are bounded below by lb and above by -lb(-row)
+    enclosing the exact range of their linear forms."""
+    from fractions import Fraction as F
+    lo = O.dag_crown(P, L[None], U[None])[0]
+    Q = Program(P.n)
+    Q.nodes = list(P.nodes)
+    Q.out = [Q.neg(o) for o in P.out]
+    hi = -O.dag_crown(Q, L[None], U[None])[0]
+    for r, o in enumerate(P.out):
+        coef, const = _linear_form(P, o)
+        mn = const + sum(min(a * F(l), a * F(h)) for a, l, h in zip(coef, L, U))
+        mx = const + sum(max(a * F(l), a * F(h)) for a, l, h in zip(coef, L, U))
+        assert np.isfinite(lo[r]) and np.isfinite(hi[r])
+        assert F(float(lo[r])) <= mn and F(float(hi[r])) >= mx, (r, lo[r], float(mn))
+    return lo, hi
+
+
+def test_enclosure_add_sub_chain():
+    """(x0 + x1) - x2 at x = (2^56, 1, 2^56): the exact value 1 is enclosed
+    (solver/tests/test_fp_enclosure.py:193-203)."""
+    P = Program(3)
+    P.output(P.sub(P.add(0, 1), 2))
+    x = np.array([2.0 ** 56, 1.0, 2.0 ** 56])
+    lo, hi = _encloses(P, x, x)
+    assert lo[0] <= 1.0 <= hi[0]
+
+
+@pytest.mark.parametrize("box", [(1.0, 1.0), (0.5, 1.5)])
+def test_enclosure_linear_fold(box):
+    """1e16 x - x - 1e16 x = -x: the products by inexact constants and the
+    merges of the cancelling contributions are charged, so [-hi, -lo] is
+    enclosed (test_fp_enclosure.py:206-235)."""
+    P = Program(1)
+    P.output(P.add(P.add(P.mulc(0, 1e16), P.neg(0)), P.mulc(0, -1e16)))
+    lo, hi = _encloses(P, np.array([box[0]]), np.array([box[1]]))
+    assert lo[0] <= -box[1] and hi[0] >= -box[0]
+
+
+@pytest.mark.parametrize("seed", range(40))
+def test_enclosure_random_cancellations(seed):
+    """Random linear programs mixing +-2^20..2^53 coefficients with O(1)
+    ones: lower and upper bounds enclose the exact range of every row, at a
+    point box and a box of half-width 1, and stay finite
+    (test_fp_enclosure.py:238-370)."""
+    import random
+    rng = random.Random(seed)
+    d = rng.randint(1, 3)
+    P = Program(d)
+    nodes = list(range(d))
+    for _ in range(rng.randint(3, 9)):
+        op = rng.choice(["add", "sub", "mulc", "addc", "neg"])
+        a, b = rng.choice(nodes), rng.choice(nodes)
+        big = rng.choice((1, -1)) * 2.0 ** rng.randint(20, 53) * (1 + rng.random())
+        c = big if rng.random() < 0.5 else rng.uniform(-3, 3)
+        nodes.append({"add": lambda: P.add(a, b), "sub": lambda: P.sub(a, b),
+                      "mulc": lambda: P.mulc(a, c), "addc": lambda: P.addc(a, c),
+                      "neg": lambda: P.neg(a)}[op]())
+    for v in nodes[-3:]:
+        if v >= d:
+            P.output(v)
+    if not P.out:
+        P.output(P.add(0, 0))
+    cen = np.array([rng.uniform(-4, 4) for _ in range(d)])
+    _encloses(P, cen, cen)
+    _encloses(P, cen - 1, cen + 1)
+
+
 OPS = {
     "exp": (lambda P: P.exp(0), 1, False), "log": (lambda P: P.log(0), 1, True),
     "recip": (lambda P: P.recip(0), 1, True), "sqr": (lambda P: P.sqr(0), 1, False),
