@@ -6,9 +6,11 @@ Workload: seeded nonconvex QCQP, n = 50, m = 20, coefficients N(0,1)/sqrt(n),
 x in [-1, 1]^50; the loop bounds (CROWN, FP64), branches on the widest
 variable, prunes against the ALM-PGD incumbent and the constraint rows, and
 appends survivors to a device frontier of at most 2^18 domains.  One step =
-one B&B iteration: `batch` parents picked from the frontier -> 2 * batch
-children bounded.  The loop starts from the root bisected (widest variable
-first) into `batch` leaves; the root ALM-PGD search runs once before that.
+one B&B iteration of the reordered loop: the picked domains (the largest
+power of two up to 32,768 whose growth fits the frontier over the run) are
+bounded, bisected, keyed and filtered into the frontier.  The loop starts
+from the root bisected (widest variable first) into one step's worth of
+leaves; the root ALM-PGD search runs once before that.
 
 Prints one JSON line (see the keys below).  The CPU baseline times the FP64
 oracle on the host cores over a bounded sample of the same children; the
@@ -41,8 +43,9 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--batch", type=int, default=16384,
-                    help="half the domains bounded per step (the reference's batch_size)")
+    ap.add_argument("--batch", type=int, default=0,
+                    help="domains picked and bounded per step (0: the largest power of two "
+                         "up to 32,768 whose growth over the run fits the 2^18 frontier)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-alm", action="store_true")
@@ -164,6 +167,65 @@ def bench_uneven_dag(reps=20, cpu=True):
                                "cores": cores, "kind": "port",
                                "sample": f"first 32 boxes of each of the 256 instances "
                                          f"({nb} boxes, {tc:.2f} s wall), FP64 oracle"}
+    return out
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def bench_nlp_config1(reps=5):
+    """The n = 8, m = 3 NLP (x^T Q x + sum c_i exp(0.3 x_i) + 4 bilinear
+    terms, three quadratic constraints) over 1,024 boxes of seeded random
+    bisection of [-2, 2]^8 (default_rng(2)): the FP64 CPU restatement timed
+    on all host cores and on one thread (best of 5 after one warm-up), and
+    the HIP expression-program bound of the same boxes (identical bounds,
+    tests/test_gpu_dag.py)."""
+    import torch
+
+    from gear_amd.dag import DagPlan
+    from gear_amd.problems import random_bisection_boxes
+    from gear_amd.program import nlp_n8_m3_problem
+    from oracle import oracle as O
+
+    P = nlp_n8_m3_problem()
+    L, U = random_bisection_boxes(P.xl, P.xu, 1024, seed=2)
+    out = {"config": {"workload": "nlp n=8 m=3 (exp, bilinear, quadratic constraints), 1024 "
+                                  "boxes of seeded random bisection of [-2, 2]^8",
+                      "boxes": 1024, "rows": P.R},
+           "host": {"cpu_model": cpu_model(), "nproc": os.cpu_count()}}
+    cpu = {}
+    for threads in (os.cpu_count() or 1, 1):
+        O.dag_crown(P, L, U, threads=threads)
+        best = float("inf")
+        for _ in range(reps):
+            t = time.perf_counter()
+            O.dag_crown(P, L, U, threads=threads)
+            best = min(best, time.perf_counter() - t)
+        cpu[f"threads_{threads}"] = {"value": round(1024 / best, 1), "unit": "subdomain bounds/s",
+                                     "ms_per_call": round(best * 1e3, 3)}
+    out["cpu_fp64_oracle"] = cpu
+    dev = torch.device("cuda:0")
+    plan = DagPlan([P], device=dev)
+    xL, xU = [torch.as_tensor(L, device=dev)], [torch.as_tensor(U, device=dev)]
+    plan.bound(xL, xU)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(20):
+        plan.bound(xL, xU)
+    e1.record()
+    e1.synchronize()
+    ms = e0.elapsed_time(e1) / 20
+    out["gpu_fp64"] = {"value": round(1024 / (ms * 1e-3), 1), "unit": "subdomain bounds/s",
+                       "ms_per_call": round(ms, 4),
+                       "note": "one call bounds the 1,024 boxes (plan made once)"}
     return out
 
 
@@ -420,14 +482,23 @@ def main():
     torch.cuda.set_device(dev)
     p = qcqp_n50_m20_problem()
     n, S = p.n, p.S
-    cap = 1 << 20
-    # a reordered step bounds 2 batch domains and appends at most 4 batch children
-    need = 2 * args.batch * (args.warmup + args.steps + 1)
+    cap = 1 << 18
+    # a reordered step picks `picks` domains, bounds them and appends at most
+    # 2 picks children: the frontier grows by at most `picks` per step
+    picks = args.batch
+    if picks <= 0:
+        picks = 32768
+        while picks > 1024 and picks * (args.warmup + args.steps + 1) > cap:
+            picks //= 2
+    need = picks * (args.warmup + args.steps + 1)
     if need > cap:
-        raise SystemExit(f"2 batch * (warmup + steps + 1) = {need} exceeds the frontier cap {cap}")
-    bb = QCQPBranchAndBound(p, BaBConfig(batch=args.batch, cap=cap), device=dev)
+        raise SystemExit(f"picks * (warmup + steps + 1) = {need} exceeds the frontier cap {cap}")
+    args.batch = picks // 2      # BaBConfig.batch: half the domains picked per step
+    # physical rows: cap and as many again, so the run's appends need no compaction
+    bb = QCQPBranchAndBound(p, BaBConfig(batch=args.batch, cap=cap, frontier_slack=cap),
+                            device=dev)
     t_alm = time.time()
-    # presplit start: 2 batch leaves so every step bounds a full 2 batch
+    # presplit start: `picks` leaves so every step bounds a full batch
     bb.presplit(2 * args.batch, alm=not args.no_alm)
     torch.cuda.synchronize()
     t_alm = time.time() - t_alm
@@ -484,8 +555,8 @@ def main():
     value = boxes / wall
 
     # the same loop with FP32-sound bounds (fresh tree, same start)
-    bb32 = QCQPBranchAndBound(p, BaBConfig(batch=args.batch, cap=cap), device=dev,
-                              precision="fp32s")
+    bb32 = QCQPBranchAndBound(p, BaBConfig(batch=args.batch, cap=cap, frontier_slack=cap),
+                              device=dev, precision="fp32s")
     if bb.x_best is not None:
         bb32.set_incumbent(bb.P, bb.x_best)
     bb32.presplit(2 * args.batch, alm=False)
@@ -519,7 +590,7 @@ def main():
         "data": "synthetic (seeded QCQP generator, default_rng(3))",
         "config": {"workload": "qcqp_n50_m20 batched B&B, reordered step: sort -> bound the "
                                "picked domains (CROWN + planes) -> widest branch -> bisect -> "
-                               "child keys -> prune -> append, frontier cap 2^20",
+                               "child keys -> prune -> append, frontier cap 2^18",
                    "n": n, "m": S - 1, "domains_bounded_per_step": 2 * args.batch,
                    "children_per_step": 4 * args.batch},
         "roofline": {
@@ -623,6 +694,8 @@ def main():
                                   "unit": "subdomain bounds/s", "cores": cores, "kind": "port",
                                   "sample": f"{L.shape[0]} children of one step, FP64 oracle "
                                             f"({tc:.2f} s wall)"}
+    if not args.no_cpu:
+        result["nlp_cpu_config1"] = bench_nlp_config1()
     if not args.no_dag:
         result["uneven_dag"] = bench_uneven_dag(cpu=not args.no_cpu)
     if not args.no_large:

@@ -125,6 +125,7 @@ class BaBConfig:
     # verified domains, the interval doubling (to alm_max_interval) after
     # alm_interval_patience calls without a better incumbent and reset by
     # one; 0: ALM only at the root
+    frontier_slack: int = None     # physical rows past cap (None: the frontier's default)
     alm_interval: int = 5
     alm_max_interval: int = 20
     alm_interval_patience: int = 3
@@ -173,7 +174,8 @@ class QCQPBranchAndBound:
             raise ValueError("clip 'complete' runs in the reordered loop with n <= 64")
         n, S = problem.n, problem.S
         self.frontier = DeviceFrontier(n, S, self.cfg.cap, self.device,
-                                       payload=S * (n + 1) if self.clip else 0)
+                                       payload=S * (n + 1) if self.clip else 0,
+                                       slack=self.cfg.frontier_slack)
         # the cut rows of a domain without any: inactive (0 x - 1 <= 0)
         self._no_cuts = no_cuts_row(n, S, self.device) if self.clip else None
         self.alm = ALMPGDSolver(self.cfg.alm)

@@ -320,19 +320,33 @@ class DeviceFrontier:
         self._sk = None         # sort buffers, allocated by the first sort
         self._pending = None
 
-    def _alloc(self, rows):
+    def _storage(self, rows):
+        """(L, U, lb, payload) arrays of `rows` physical rows plus the dead
+        record (an empty box whose bound is +inf on every row) at `rows`."""
         d = dict(dtype=torch.float64, device=self.device)
-        self._L = torch.empty(rows + 1, self.n, **d)
-        self._U = torch.empty(rows + 1, self.n, **d)
-        self._lb = torch.empty(rows + 1, self.S, **d)
-        self._pay = torch.empty(rows + 1, self.W, **d) if self.W else None
-        # the dead record: an empty box whose bound is +inf on every row
-        self._L[rows] = 1.0
-        self._U[rows] = 0.0
-        self._lb[rows] = float("inf")
-        if self._pay is not None:
-            self._pay[rows] = 0.0
-            self._pay[rows, self.S * self.n:] = -1.0
+        L = torch.empty(rows + 1, self.n, **d)
+        U = torch.empty(rows + 1, self.n, **d)
+        lb = torch.empty(rows + 1, self.S, **d)
+        pay = torch.empty(rows + 1, self.W, **d) if self.W else None
+        L[rows] = 1.0
+        U[rows] = 0.0
+        lb[rows] = float("inf")
+        if pay is not None:
+            pay[rows] = 0.0
+            pay[rows, self.S * self.n:] = -1.0
+        return L, U, lb, pay
+
+    def _alloc(self, rows):
+        self._L, self._U, self._lb, self._pay = self._storage(rows)
+        # the compaction's target, made up front so that no step allocates
+        self._spare = self._storage(rows)
+        # run the compaction's library kernels once (a masked select and row
+        # gathers), so their first use inside a step does not load code
+        one = torch.zeros(1, dtype=torch.int64, device=self.device)
+        one[one != 1]
+        for src, dst in zip((self._L, self._U, self._lb, self._pay), self._spare):
+            if src is not None:
+                torch.index_select(src, 0, one, out=dst[:1])
         self.unsynced = 0         # appends since the last exact count
 
     def __len__(self):
@@ -387,12 +401,14 @@ class DeviceFrontier:
         # the copy below would turn the dead record into ordinary rows
         self.compact_order()
         live = self.order[:self.count]
+        # the live rows move, in logical order, into the second storage (kept
+        # between compactions, so none allocates after the first)
         old = (self._L, self._U, self._lb, self._pay)
-        self._alloc(self.phys)
-        for src, dst in zip(old, (self._L, self._U, self._lb, self._pay)):
+        for src, dst in zip(old, self._spare):
             if src is not None:
                 torch.index_select(src, 0, live, out=dst[:self.count])
-        del old
+        self._L, self._U, self._lb, self._pay = self._spare
+        self._spare = old
         self.order[:self.count].copy_(self._iota[:self.count])
         self.hw = self.count
         if self.hw + B > self.phys:

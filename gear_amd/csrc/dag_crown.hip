@@ -296,7 +296,8 @@ __device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlan
 
 // One backward pass; writes its lower (and upper) bound.
 __device__ void crown_pass(const DagPass *passes, const DagStep *steps, const int32_t *cin,
-                           int pidx, Lane W, int V, int n, double &out_l, double &out_u)
+                           int pidx, Lane W, int V, int n, double &out_l, double &out_u,
+                           double &bias_l, int &undef_l)
 {
     const DagPass P = passes[pidx];
     const bool want_u = P.want_u != 0;
@@ -456,6 +457,8 @@ __device__ void crown_pass(const DagPass *passes, const DagStep *steps, const in
     if (isnan(lo) || (und & 1)) lo = -INFINITY;
     out_l = lo;
     out_u = INFINITY;
+    bias_l = lacc;
+    undef_l = und & 1;
     if (want_u) {
         double bu = widen(uacc, finite_abs(uacc) * kn, 1.0);
         double ru = widen(base_u + et_u, ma_u * kn, 1.0);
@@ -505,22 +508,43 @@ __device__ void run_job(const DagJob &J, const DagFwd *fwd, const DagPass *passe
         }
         if (pass >= 0) {
             // CROWN bound intersected with the interval (bound_general.py:1160-1185)
-            double cl, cu;
-            crown_pass(passes, steps, cin, pass, W, V, n, cl, cu);
+            double cl, cu, bias;
+            int und;
+            crown_pass(passes, steps, cin, pass, W, V, n, cl, cu, bias, und);
             W[k] = fmax(cl, il);
             W[V + k] = fmin(cu, iu);
         }
     }
-    double out;
+    double out, bias = 0.0;
+    int und = 0;
     if (J.final_pass < 0) {
         out = W[J.out];
     } else {
         double cl, cu;
-        crown_pass(passes, steps, cin, J.final_pass, W, V, n, cl, cu);
+        crown_pass(passes, steps, cin, J.final_pass, W, V, n, cl, cu, bias, und);
         // a row with no relaxation falls back to its interval (backward_bound.py:826-877)
         out = (cl == -INFINITY) ? W[J.out] : fmax(cl, -INFINITY);
     }
-    if (valid) bt.lb[box * J.R + J.row] = out;
+    if (!valid) return;
+    const size_t br = (size_t)box * J.R + J.row;
+    bt.lb[br] = out;
+    if (bt.lA) {
+        // the row's plane over the program's inputs: zeros, then the A of
+        // the inputs the row reads (a bare input: its unit vector)
+        double *a = bt.lA + br * n;
+        for (int i = 0; i < n; ++i) a[i] = 0.0;
+        if (J.final_pass < 0) {
+            a[uni(gin[J.gin_off + J.out])] = 1.0;
+        } else if (!und) {
+            const DagPass P = passes[J.final_pass];
+            for (int k = 0; k < P.ncin; ++k) {
+                const int e = uni(cin[P.cin_off + k]);
+                a[uni(gin[J.gin_off + (e >> 16)])] = W[e & 0xFFFF];
+            }
+        }
+        bt.lbias[br] = und ? -INFINITY : bias;
+        if (bt.undef) bt.undef[br] = (uint8_t)und;
+    }
 }
 
 // Chunk counts per job and their exclusive prefix sum; resets the queue.

@@ -378,7 +378,7 @@ __global__ __launch_bounds__(W * 64) __attribute__((amdgpu_waves_per_eu(2))) voi
     int B, int n, int S, const double* __restrict__ QB, const double* __restrict__ QA,
     const double* __restrict__ Cm, const double* __restrict__ d, const double* __restrict__ xL,
     const double* __restrict__ xU, const int64_t* __restrict__ rows, double* __restrict__ lb,
-    double* __restrict__ lA, double* __restrict__ lbias, int PR, QcqpConsts k)
+    double* __restrict__ lA, double* __restrict__ lbias, int PR, QcqpConsts k, int rpb)
 {
     constexpr int NCH = (KT + 3) / 4;
     constexpr int NM = KT;
@@ -402,17 +402,20 @@ __global__ __launch_bounds__(W * 64) __attribute__((amdgpu_waves_per_eu(2))) voi
     }
 
     const size_t rowstride = (size_t)packed_tiles(KT) * KT * 64;
+    // this workgroup's rows: the spec rows are independent, so a batch too
+    // small to fill the GPU is also split over rows (blockIdx.y)
+    const int s0 = blockIdx.y * rpb, s1 = min(S, s0 + rpb);
     RowState<KT> st;
     if constexpr (W > 1) {
-        gh::stage_row<double, W>(QB, QA, rowstride, 0, qsm, wv, l);
+        gh::stage_row<double, W>(QB, QA, rowstride, s0, qsm + (s0 & 1) * 2 * rowstride, wv, l);
         __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
     }
-    for (int s = 0; s < S; ++s) {
+    for (int s = s0; s < s1; ++s) {
         const double* qb;
         const double* qa;
         if constexpr (W > 1) {
-            if (s + 1 < S)
+            if (s + 1 < s1)
                 gh::stage_row<double, W>(QB, QA, rowstride, s + 1,
                                          qsm + ((s + 1) & 1) * 2 * rowstride, wv, l);
             const double* buf = qsm + (s & 1) * 2 * rowstride;
@@ -436,9 +439,34 @@ __global__ __launch_bounds__(W * 64) __attribute__((amdgpu_waves_per_eu(2))) voi
 }
 
 // Batches of at least this many boxes take the LDS-staged form (128 boxes
-// per workgroup; one workgroup per CU at 32,768 boxes).
-constexpr int kStagedMinBoxes = 16384;
+// per workgroup; one workgroup per CU at 32,768 boxes).  Below one
+// workgroup per CU the rows are split over more workgroups.
+constexpr int kStagedMinBoxes = 4096;
 constexpr int kStagedWaves = 8;
+
+int compute_units()
+{
+    static int cus = 0;
+    if (!cus) {
+        int dev = 0, v = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+            v > 0)
+            cus = v;
+        else
+            cus = 256;
+    }
+    return cus;
+}
+
+// rows per workgroup so that about `target` workgroups run
+inline int rows_per_block(int S, long nwg, long target)
+{
+    long chunks = (target + nwg - 1) / nwg;
+    if (chunks < 1) chunks = 1;
+    if (chunks > S) chunks = S;
+    return (int)((S + chunks - 1) / chunks);
+}
 
 template <int KT>
 int launch_f64(int B, int n, int S, const double* QB, const double* QA, const double* Cm,
@@ -458,13 +486,17 @@ int launch_f64(int B, int n, int S, const double* QB, const double* QA, const do
                 return GH_ELAUNCH;
             attr = true;
         }
-        dim3 grid((B + 16 * W - 1) / (16 * W));
+        const long nwg = (B + 16 * W - 1) / (16 * W);
+        const int rpb = rows_per_block(S, nwg, compute_units());
+        dim3 grid((unsigned)nwg, (unsigned)((S + rpb - 1) / rpb));
         hipLaunchKernelGGL((qcqp_crown_f64_kernel<KT, W>), grid, dim3(64 * W), lds, stream, B, n,
-                           S, QB, QA, Cm, d, xL, xU, rows, lb, lA, lbias, PR, k);
+                           S, QB, QA, Cm, d, xL, xU, rows, lb, lA, lbias, PR, k, rpb);
     } else {
-        dim3 grid((B + 15) / 16);
+        const long nwg = (B + 15) / 16;
+        const int rpb = rows_per_block(S, nwg, 4L * compute_units());
+        dim3 grid((unsigned)nwg, (unsigned)((S + rpb - 1) / rpb));
         hipLaunchKernelGGL((qcqp_crown_f64_kernel<KT, 1>), grid, dim3(64), 0, stream, B, n, S,
-                           QB, QA, Cm, d, xL, xU, rows, lb, lA, lbias, PR, k);
+                           QB, QA, Cm, d, xL, xU, rows, lb, lA, lbias, PR, k, rpb);
     }
     return hipGetLastError() == hipSuccess ? GH_OK : GH_ELAUNCH;
 }

@@ -116,7 +116,7 @@ __global__ __launch_bounds__(STAGED ? 512 : 64) __attribute__((amdgpu_waves_per_
     const double* __restrict__ Q64, const double* __restrict__ Cm, const double* __restrict__ d,
     const double* __restrict__ xL, const double* __restrict__ xU,
     const int64_t* __restrict__ rows, double* __restrict__ lb, int* __restrict__ flag,
-    double* __restrict__ lA, double* __restrict__ lbias, int PR, F32Consts k)
+    double* __restrict__ lA, double* __restrict__ lbias, int PR, F32Consts k, int rpb)
 {
     constexpr int NCH = (KT + 3) / 4;
     constexpr int NM = KT;
@@ -142,16 +142,18 @@ __global__ __launch_bounds__(STAGED ? 512 : 64) __attribute__((amdgpu_waves_per_
     int ambiguous = 0;
 
     const size_t rowstride = (size_t)NCH * KT * 64;
+    // this workgroup's rows (blockIdx.y): the spec rows are independent
+    const int s0 = blockIdx.y * rpb, s1 = min(S, s0 + rpb);
     if constexpr (STAGED) {
-        gh::stage_row<float, W>(QB, QA, rowstride, 0, qsf, wv, l);
+        gh::stage_row<float, W>(QB, QA, rowstride, s0, qsf + (s0 & 1) * 2 * rowstride, wv, l);
         __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
     }
-    for (int s = 0; s < S; ++s) {
+    for (int s = s0; s < s1; ++s) {
         const float* qb;
         const float* qa;
         if constexpr (STAGED) {
-            if (s + 1 < S)
+            if (s + 1 < s1)
                 gh::stage_row<float, W>(QB, QA, rowstride, s + 1,
                                         qsf + ((s + 1) & 1) * 2 * rowstride, wv, l);
             const float* buf = qsf + (s & 1) * 2 * rowstride;
@@ -326,7 +328,37 @@ __global__ __launch_bounds__(STAGED ? 512 : 64) __attribute__((amdgpu_waves_per_
     // one flag per box: some plane choice was near a tie and was taken
     // from FP64 (diagnostic); box r's lanes are r, r + 16, r + 32, r + 48
     unsigned long long any = __ballot(ambiguous);
-    if (live && lq == 0) flag[box] = (any & (0x0001000100010001ull << lr)) ? 1 : 0;
+    if (live && lq == 0 && (any & (0x0001000100010001ull << lr))) atomicOr(&flag[box], 1);
+}
+
+__global__ void __launch_bounds__(256) zero_i32_kernel(int n, int* __restrict__ p)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+int compute_units_f32()
+{
+    static int cus = 0;
+    if (!cus) {
+        int dev = 0, v = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+            v > 0)
+            cus = v;
+        else
+            cus = 256;
+    }
+    return cus;
+}
+
+// rows per workgroup so that about `target` workgroups run
+inline int rows_per_block_f32(int S, long nwg, long target)
+{
+    long chunks = (target + nwg - 1) / nwg;
+    if (chunks < 1) chunks = 1;
+    if (chunks > S) chunks = S;
+    return (int)((S + chunks - 1) / chunks);
 }
 
 template <int KT>
@@ -335,7 +367,9 @@ int launch_f32(int B, int n, int S, const float* QB, const float* QA, const doub
                const int64_t* rows, double* lb, int* flag, double* lA, double* lbias, int PR,
                const F32Consts& k, hipStream_t stream)
 {
-    if (B >= 16384) {   // the LDS-staged form, 128 boxes per workgroup
+    // the flags are OR-ed over the row blocks of a box
+    hipLaunchKernelGGL(zero_i32_kernel, dim3((B + 255) / 256), dim3(256), 0, stream, B, flag);
+    if (B >= 4096) {   // the LDS-staged form, 128 boxes per workgroup
         const size_t rowstride = (size_t)((KT + 3) / 4) * KT * 64;
         const size_t lds = 4 * rowstride * sizeof(float);
         static bool attr = false;   // once per instantiation (same value every call)
@@ -346,13 +380,19 @@ int launch_f32(int B, int n, int S, const float* QB, const float* QA, const doub
                 return GH_ELAUNCH;
             attr = true;
         }
-        hipLaunchKernelGGL((qcqp_crown_f32s_kernel<KT, true>), dim3((B + 127) / 128), dim3(512),
+        const long nwg = (B + 127) / 128;
+        const int rpb = rows_per_block_f32(S, nwg, compute_units_f32());
+        hipLaunchKernelGGL((qcqp_crown_f32s_kernel<KT, true>),
+                           dim3((unsigned)nwg, (unsigned)((S + rpb - 1) / rpb)), dim3(512),
                            lds, stream, B, n, S, QB, QA, Q64, Cm, d, xL, xU, rows, lb, flag, lA,
-                           lbias, PR, k);
+                           lbias, PR, k, rpb);
     } else {
-        hipLaunchKernelGGL((qcqp_crown_f32s_kernel<KT, false>), dim3((B + 15) / 16), dim3(64), 0,
+        const long nwg = (B + 15) / 16;
+        const int rpb = rows_per_block_f32(S, nwg, 4L * compute_units_f32());
+        hipLaunchKernelGGL((qcqp_crown_f32s_kernel<KT, false>),
+                           dim3((unsigned)nwg, (unsigned)((S + rpb - 1) / rpb)), dim3(64), 0,
                            stream, B, n, S, QB, QA, Q64, Cm, d, xL, xU, rows, lb, flag, lA, lbias,
-                           PR, k);
+                           PR, k, rpb);
     }
     return hipGetLastError() == hipSuccess ? GH_OK : GH_ELAUNCH;
 }

@@ -79,8 +79,10 @@ class DagPlan:
             self._ws_waves = waves
         return self._ws
 
-    def make_batches(self, xLs, xUs, lbs, counts):
-        """Device array of gh_dag_batch: per program (xL, xU, lb, count*)."""
+    def make_batches(self, xLs, xUs, lbs, counts, planes=None):
+        """Device array of gh_dag_batch: per program (xL, xU, lb, count*,
+        lA, lbias, undef); planes: per program (lA (cap, R, n), lbias
+        (cap, R), undef (cap, R) uint8) or None."""
         rows = []
         for k, (xL, xU, lb, cnt) in enumerate(zip(xLs, xUs, lbs, counts)):
             for t in (xL, xU, lb):
@@ -89,8 +91,18 @@ class DagPlan:
             if xL.shape[-1] != self.n[k] or lb.shape[-1] != self.R[k]:
                 raise _lib.GearHipError(f"program {k}: box width {xL.shape[-1]} / rows {lb.shape[-1]} "
                                         f"do not match n={self.n[k]}, R={self.R[k]}")
-            rows.append([xL.data_ptr(), xU.data_ptr(), lb.data_ptr(), cnt.data_ptr()])
-        return torch.tensor(np.asarray(rows, dtype=np.uint64).view(np.int64).reshape(-1, 4),
+            pl = [0, 0, 0]
+            if planes is not None and planes[k] is not None:
+                lA, lbias, und = planes[k]
+                cap = xL.shape[0]
+                if lA.shape != (cap, self.R[k], self.n[k]) or lbias.shape != (cap, self.R[k]) \
+                        or und.shape != (cap, self.R[k]) or und.dtype != torch.uint8 \
+                        or not (lA.is_contiguous() and lbias.is_contiguous() and und.is_contiguous()):
+                    raise _lib.GearHipError(f"program {k}: plane buffers must be contiguous "
+                                            f"(B, R, n), (B, R) float64 and (B, R) uint8")
+                pl = [lA.data_ptr(), lbias.data_ptr(), und.data_ptr()]
+            rows.append([xL.data_ptr(), xU.data_ptr(), lb.data_ptr(), cnt.data_ptr(), *pl])
+        return torch.tensor(np.asarray(rows, dtype=np.uint64).view(np.int64).reshape(-1, 7),
                             dtype=torch.int64).to(self.device)
 
     def launch(self, batches, waves, stream=None):
@@ -128,16 +140,26 @@ class DagPlan:
             self.launch(batches, waves)
         return g
 
-    def bound(self, xLs, xUs, waves=None):
-        """lb per program (B_k, R_k), FP64."""
+    def bound(self, xLs, xUs, waves=None, return_A=False, stream=None):
+        """lb per program (B_k, R_k), FP64; with return_A also each row's
+        lower plane and undefined flag: (lbs, lAs (B_k, R_k, n_k), lbiases
+        (B_k, R_k), undefs (B_k, R_k) bool)."""
         waves = waves or default_waves(self.device)
         xLs = [x.contiguous() for x in xLs]
         xUs = [x.contiguous() for x in xUs]
-        lbs = [torch.empty(x.shape[0], r, dtype=torch.float64, device=self.device)
-               for x, r in zip(xLs, self.R)]
+        d = dict(dtype=torch.float64, device=self.device)
+        lbs = [torch.empty(x.shape[0], r, **d) for x, r in zip(xLs, self.R)]
+        planes = None
+        if return_A:
+            planes = [(torch.empty(x.shape[0], r, n, **d), torch.empty(x.shape[0], r, **d),
+                       torch.empty(x.shape[0], r, dtype=torch.uint8, device=self.device))
+                      for x, r, n in zip(xLs, self.R, self.n)]
         counts = torch.tensor([x.shape[0] for x in xLs], dtype=torch.int64, device=self.device)
         cnt_views = [counts[k:k + 1] for k in range(len(xLs))]
-        batches = self.make_batches(xLs, xUs, lbs, cnt_views)
-        self.launch(batches, waves)
-        self._keep = (batches, counts)
-        return lbs
+        batches = self.make_batches(xLs, xUs, lbs, cnt_views, planes)
+        self.launch(batches, waves, stream)
+        self._keep = (batches, counts, xLs, xUs)
+        if not return_A:
+            return lbs
+        return (lbs, [p[0] for p in planes], [p[1] for p in planes],
+                [p[2].bool() for p in planes])

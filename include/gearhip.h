@@ -488,11 +488,19 @@ int gh_dag_plan_info(const gh_dag_plan *plan, int64_t *jobs, int64_t *max_slots,
 int gh_dag_plan_stats(const gh_dag_plan *plan, int64_t *state_bytes, int64_t *io_bytes,
                       int64_t *steps);
 
-/* One program's batch of boxes (device pointers). */
+/* One program's batch of boxes (device pointers).  lA, lbias and undef may
+ * be NULL: each row's lower plane lA x + lbias over the program's inputs
+ * (before concretization, as BoundedModule.compute_bounds returns it in
+ * A_dict with return_A, auto_LiRPA/backward_bound.py:1093-1111) and the
+ * rows whose relaxation was undefined (their plane is A = 0, bias -inf,
+ * :445-463; their lb is the interval fallback). */
 typedef struct {
     const double *xL, *xU;   /* (cap, n) */
     double *lb;              /* (cap, R) out */
     const int64_t *count;    /* device scalar: boxes to bound, <= cap */
+    double *lA;              /* (cap, R, n) out, or NULL */
+    double *lbias;           /* (cap, R) out, or NULL */
+    uint8_t *undef;          /* (cap, R) out, or NULL */
 } gh_dag_batch;
 
 /* Workspace bytes for gh_dag_crown_f64 with `waves` resident wavefronts. */
@@ -502,8 +510,8 @@ size_t gh_dag_workspace_bytes(const gh_dag_plan *plan, int waves);
  * CROWN lower bounds of every row of every program over its batch, FP64,
  * in one persistent launch: `waves` wavefronts (64 boxes each) pull
  * (row, 64-box chunk) items from a device queue, largest rows first, until
- * the queue drains.  Same contract as gh_qcqp_crown_f64 without A:
- *   compute_bounds(..., method='CROWN', bound_upper=False)
+ * the queue drains.  Same contract as gh_qcqp_crown_f64:
+ *   compute_bounds(..., method='CROWN', bound_upper=False, return_A)
  * (auto_LiRPA/bound_general.py:1275-1593): intermediate bounds of every
  * input of a relaxed node by CROWN intersected with intervals
  * (bound_general.py:1032-1222), the operators' relaxations

@@ -335,7 +335,8 @@ static double push_A(dag_ws *w, int v, double a, int side, int later, int is_inp
 /* CROWN pass from node s: lower (and upper) bound of node s over the box.
  * Returns bits: 1 lower vacuous, 2 upper vacuous. */
 static void crown_pass(const gh_prog *P, dag_ws *w, const double *xL, const double *xU, int s,
-                       int want_u, double *out_l, double *out_u)
+                       int want_u, double *out_l, double *out_u, double *plane_A,
+                       double *plane_b, int *undef)
 {
     const gh_node *N = P->nodes;
     int n = P->n;
@@ -498,10 +499,19 @@ static void crown_pass(const gh_prog *P, dag_ws *w, const double *xL, const doub
     if (isnan(hi) || (und & 2)) hi = INFINITY;
     *out_l = lo;
     *out_u = want_u ? hi : INFINITY;
+    /* the lower plane A x + bias over the inputs, before concretization
+     * (backward_bound.py:1093-1111); a discarded row's plane is vacuous:
+     * A = 0, bias -inf (_zero_undefined_root_A, :445-463) */
+    if (plane_A) {
+        for (int i = 0; i < n; ++i) plane_A[i] = (und & 1) ? 0.0 : w->Axl[i];
+        *plane_b = (und & 1) ? -INFINITY : lacc;
+        *undef = und & 1;
+    }
 }
 
 static void bound_box(const gh_prog *P, dag_ws *w, const double *xL, const double *xU,
-                      const uint8_t *needs, double *lb)
+                      const uint8_t *needs, double *lb, double *lA, double *lbias,
+                      uint8_t *undef)
 {
     const gh_node *N = P->nodes;
     int n = P->n, V = P->V;
@@ -516,7 +526,7 @@ static void bound_box(const gh_prog *P, dag_ws *w, const double *xL, const doubl
             w->udef[v] = relax_node(&N[v], w->L, w->U, w->R + 6 * (size_t)v);
         if (needs[v] && N[v].op != GH_OP_CONST) {
             double cl, cu;
-            crown_pass(P, w, xL, xU, v, 1, &cl, &cu);
+            crown_pass(P, w, xL, xU, v, 1, &cl, &cu, NULL, NULL, NULL);
             /* intersect with the interval (bound_general.py:1160-1185) */
             w->L[v] = fmax(cl, il);
             w->U[v] = fmin(cu, iu);
@@ -525,8 +535,21 @@ static void bound_box(const gh_prog *P, dag_ws *w, const double *xL, const doubl
     for (int r = 0; r < P->R; ++r) {
         int o = P->out[r];
         double cl, cu;
-        if (o < n) { lb[r] = xL[o]; continue; }
-        crown_pass(P, w, xL, xU, o, 0, &cl, &cu);
+        double *pA = lA ? lA + (size_t)r * n : NULL;
+        if (o < n) {
+            /* a bare input: its plane is the unit row */
+            lb[r] = xL[o];
+            if (pA) {
+                for (int i = 0; i < n; ++i) pA[i] = (i == o) ? 1.0 : 0.0;
+                lbias[r] = 0.0;
+                undef[r] = 0;
+            }
+            continue;
+        }
+        int und = 0;
+        double b = 0.0;
+        crown_pass(P, w, xL, xU, o, 0, &cl, &cu, pA, &b, &und);
+        if (pA) { lbias[r] = b; undef[r] = (uint8_t)und; }
         /* undefined rows fall back to the interval (backward_bound.py:826-877) */
         lb[r] = (cl == -INFINITY) ? w->L[o] : fmax(cl, -INFINITY);
     }
@@ -545,6 +568,14 @@ void gho_dag_needs(const gh_prog *P, uint8_t *needs)
 
 int gho_dag_crown(const gh_prog *P, int B, const double *xL, const double *xU, double *lb,
                   int threads)
+{
+    return gho_dag_crown_planes(P, B, xL, xU, lb, NULL, NULL, NULL, threads);
+}
+
+/* As gho_dag_crown, with each row's lower plane: lA (B, R, n), lbias (B, R)
+ * and the rows whose relaxation was undefined, undef (B, R). */
+int gho_dag_crown_planes(const gh_prog *P, int B, const double *xL, const double *xU,
+                         double *lb, double *lA, double *lbias, uint8_t *undef, int threads)
 {
 #ifdef _OPENMP
     if (threads > 0) omp_set_num_threads(threads);
@@ -573,7 +604,9 @@ int gho_dag_crown(const gh_prog *P, int B, const double *xL, const double *xU, d
         w.dfl = ib + 5 * V; w.dfu = ib + 6 * V; w.hasx = ib + 7 * V;
 #pragma omp for schedule(dynamic, 16)
         for (int b = 0; b < B; ++b)
-            bound_box(P, &w, xL + (size_t)b * n, xU + (size_t)b * n, needs, lb + (size_t)b * P->R);
+            bound_box(P, &w, xL + (size_t)b * n, xU + (size_t)b * n, needs, lb + (size_t)b * P->R,
+                      lA ? lA + (size_t)b * P->R * n : NULL, lbias ? lbias + (size_t)b * P->R : NULL,
+                      undef ? undef + (size_t)b * P->R : NULL);
         free(buf);
         free(ib);
     }

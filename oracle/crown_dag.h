@@ -4,6 +4,8 @@
 #include "../include/gearhip_program.h"
 int gho_dag_crown(const gh_prog *P, int B, const double *xL, const double *xU, double *lb,
                   int threads);
+int gho_dag_crown_planes(const gh_prog *P, int B, const double *xL, const double *xU,
+                         double *lb, double *lA, double *lbias, uint8_t *undef, int threads);
 int gho_dag_ibp(const gh_prog *P, int B, const double *xL, const double *xU, double *lo,
                 double *hi);
 int gho_dag_eval(const gh_prog *P, int N, const double *X, double *F);

@@ -137,6 +137,9 @@ def _dag_lib():
         h.gho_dag_crown.restype = ctypes.c_int
         h.gho_dag_ibp.argtypes = [ctypes.POINTER(_GhProg), ctypes.c_int, _D, _D, _D, _D]
         h.gho_dag_ibp.restype = ctypes.c_int
+        h.gho_dag_crown_planes.argtypes = [ctypes.POINTER(_GhProg), ctypes.c_int, _D, _D, _D, _D,
+                                           _D, _U8, ctypes.c_int]
+        h.gho_dag_crown_planes.restype = ctypes.c_int
         h.gho_dag_set_flags.argtypes = [ctypes.c_int]
         h.gho_dag_set_flags.restype = None
         h.gho_relax_unary.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_int, _D, _D, _D,
@@ -166,6 +169,22 @@ def dag_crown(prog, xL, xU, threads=1, flags=0):
     finally:
         h.gho_dag_set_flags(0)
     return lb
+
+
+def dag_crown_planes(prog, xL, xU, threads=1):
+    """(lb (B, R), lA (B, R, n), lbias (B, R), undefined (B, R) bool): the
+    bounds and each row's lower plane lA x + lbias over the inputs, before
+    concretization (backward_bound.py:1093-1111); a row whose relaxation
+    was undefined has A = 0 and bias -inf (:445-463)."""
+    xL, xU = _c(np.atleast_2d(xL)), _c(np.atleast_2d(xU))
+    st, keep = _prog_struct(prog)
+    B = xL.shape[0]
+    lb = np.empty((B, prog.R))
+    lA = np.empty((B, prog.R, prog.n))
+    lbias = np.empty((B, prog.R))
+    und = np.empty((B, prog.R), dtype=np.uint8)
+    _dag_lib().gho_dag_crown_planes(ctypes.byref(st), B, xL, xU, lb, lA, lbias, und, threads)
+    return lb, lA, lbias, und.astype(bool)
 
 
 def relax_unary(op, c, l, u, flags=0):

@@ -217,3 +217,37 @@ def test_autotuned_waves_give_identical_bounds(cuda):
     plan.launch(batches, w)
     torch.cuda.synchronize()
     _check(lb.cpu().numpy(), O.dag_crown(P, L, U), "tuned")
+
+
+@pytest.mark.parametrize("which", ["nlp", "edge", "sshape", "random"])
+def test_planes_match_oracle(cuda, which):
+    """With return_A the kernel writes each row's lower plane (lA over the
+    program's inputs, lbias before concretization) and the undefined-row
+    flags, equal to the oracle's bit for bit; the bounds are unchanged."""
+    from gear_amd.dag import DagPlan
+    print(f"case planes {which}", flush=True)
+    if which == "nlp":
+        P = nlp_n8_m3_problem()
+        L, U = random_bisection_boxes(P.xl, P.xu, 300, seed=4)
+    elif which == "edge":
+        P = _edge_program()
+        L, U = random_bisection_boxes(np.array([-2.0, -1.5, -3.0]), np.array([2.0, 1.0, 3.0]),
+                                      300, seed=5)
+    elif which == "sshape":
+        P = _sshape_program()
+        rng = np.random.default_rng(7)
+        c = rng.uniform(-4, 4, (300, 3))
+        w = rng.uniform(0.1, 2.0, (300, 3))
+        L, U = c - w / 2, c + w / 2
+    else:
+        P = random_nlp(14, 4, seed=9)
+        L, U = random_bisection_boxes(P.xl, P.xu, 200, seed=9)
+    plan = DagPlan([P], device=cuda)
+    lbs, lAs, lbiases, unds = plan.bound([_dev(L, cuda)], [_dev(U, cuda)], return_A=True)
+    torch.cuda.synchronize()
+    lb_o, lA_o, lbias_o, und_o = O.dag_crown_planes(P, L, U, threads=8)
+    _check(lbs[0].cpu().numpy(), lb_o, f"planes {which} lb")
+    assert np.array_equal(unds[0].cpu().numpy(), und_o)
+    assert np.array_equal(lAs[0].cpu().numpy(), lA_o)
+    assert np.array_equal(lbiases[0].cpu().numpy(), lbias_o)
+    print(f"  undefined rows {int(und_o.sum())} of {und_o.size}", flush=True)

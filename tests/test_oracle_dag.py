@@ -450,3 +450,61 @@ def test_nlp_n8_m3_sound():
     for b in range(len(L)):
         F = P.eval(L[b] + rng.random((300, 8)) * (U[b] - L[b]))
         assert np.all(F >= lb[b] - 1e-12 * (1 + np.abs(F)))
+
+
+def test_planes_of_linear_rows_are_their_coefficients():
+    """A linear row's plane is the row itself: lA the exact coefficients,
+    lbias the constant (within rounding), for inputs the row reads and
+    zeros for the others; a bare-input row's plane is its unit vector."""
+    P = Program(4)
+    P.output(P.addc(P.sub(P.mulc(0, 2.5), P.mulc(2, 0.75)), -1.25))
+    P.output(P.x(3))
+    L = np.array([[-1.0, 0.0, 2.0, -3.0], [0.5, 1.0, -1.0, 0.0]])
+    U = L + 1.0
+    lb, lA, lbias, und = O.dag_crown_planes(P, L, U)
+    assert not und.any()
+    assert np.allclose(lA[:, 0], [2.5, 0.0, -0.75, 0.0], atol=0, rtol=0)
+    assert np.allclose(lbias[:, 0], -1.25, atol=1e-14)
+    assert np.array_equal(lA[:, 1], np.tile([0.0, 0.0, 0.0, 1.0], (2, 1)))
+    assert np.array_equal(lbias[:, 1], [0.0, 0.0])
+    assert np.array_equal(lb[:, 1], L[:, 3])
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_planes_are_sound_and_concretize_to_the_bound(seed):
+    """Each row's plane lies below the row at sampled points of its box, and
+    its exact box minimum (rational arithmetic) is at or above the bound and
+    within the concretization margin of it."""
+    from fractions import Fraction as F
+    Q = random_nlp(8, 2, seed=seed)
+    L, U = random_bisection_boxes(Q.xl, Q.xu, 12, seed=seed)
+    lb, lA, lbias, und = O.dag_crown_planes(Q, L, U)
+    rng = np.random.default_rng(seed)
+    for b in range(len(L)):
+        X = L[b] + rng.random((300, Q.n)) * (U[b] - L[b])
+        Fx = Q.eval(X)
+        for r in range(Q.R):
+            if und[b, r] or not np.isfinite(lbias[b, r]):
+                continue
+            plane = X @ lA[b, r] + lbias[b, r]
+            ok = np.isfinite(Fx[:, r])
+            assert np.all(Fx[ok, r] >= plane[ok] - 1e-9 * (1 + np.abs(Fx[ok, r])))
+            mn = F(float(lbias[b, r])) + sum(min(F(float(a)) * F(float(l)), F(float(a)) * F(float(u)))
+                                            for a, l, u in zip(lA[b, r], L[b], U[b]))
+            if np.isfinite(lb[b, r]):
+                assert F(float(lb[b, r])) <= mn
+                scale = 1 + abs(float(lbias[b, r])) + float(np.abs(lA[b, r]) @ np.maximum(
+                    np.abs(L[b]), np.abs(U[b])))
+                assert float(mn) - lb[b, r] <= 1e-12 * scale
+
+
+def test_undefined_row_plane_is_vacuous():
+    """1 / x over a box through 0: the row is flagged, its plane is A = 0
+    with bias -inf, and the bound falls back to the interval."""
+    P = Program(2)
+    P.output(P.recip(0))
+    P.output(P.mul(0, P.exp(1)))
+    lb, lA, lbias, und = O.dag_crown_planes(P, np.array([[-1.0, -1.0]]), np.array([[1.0, 1.0]]))
+    assert und[0, 0] and not und[0, 1]
+    assert np.array_equal(lA[0, 0], [0.0, 0.0]) and lbias[0, 0] == -math.inf
+    assert lb[0, 0] == -math.inf and np.isfinite(lb[0, 1])
