@@ -74,7 +74,11 @@ class ALMResult:
 
 
 def qcqp_values(dev, x):
-    """f_s(x) for x (N, n) on the device problem dev -> (N, S)."""
+    """f_s(x) for x (N, n) on the device problem dev -> (N, S) (a device
+    problem with its own values(x), such as an expression program's, is
+    evaluated by it)."""
+    if hasattr(dev, "values"):
+        return dev.values(x)
     S, n = dev.S, dev.n
     y = x @ dev.Q.reshape(S * n, n).t()
     return (x.unsqueeze(1) * y.view(-1, S, n)).sum(-1) + x @ dev.c.t() + dev.d
@@ -173,9 +177,12 @@ class ALMPGDSolver:
     def solve_with_restarts(self, dev, x_L, x_U, num_restarts=None, x_init=None, trace=False):
         """Search every box x_L/x_U (B, n) with R restarts each; returns the
         best feasible point over all of them."""
-        R = restart_count(self.cfg, x_L.shape[0], num_restarts, dev.S * dev.n * dev.n)
+        R = restart_count(self.cfg, x_L.shape[0], num_restarts,
+                          0 if hasattr(dev, "values") else dev.S * dev.n * dev.n)
         lo, hi, x = self.starts(x_L, x_U, R, x_init)
         if self.backend == "hip":
+            if hasattr(dev, "values"):
+                raise ValueError("the fused ALM kernel covers QCQPs; use backend='torch'")
             best_x, best_obj, tr = self._run_hip(dev, lo, hi, x, trace)
         else:
             best_x, best_obj, _ = self._run_torch(dev, lo, hi, x)

@@ -163,9 +163,16 @@ class QCQPBranchAndBound:
     """Device-resident B&B over one QCQP instance."""
 
     def __init__(self, problem, cfg: BaBConfig = None, device="cuda", precision="fp64"):
+        self._init_common(problem, cfg or BaBConfig(),
+                          BoundedQCQP(problem, device, precision=precision))
+
+    def _init_common(self, problem, cfg, net):
+        """State shared by the loops: the bound `net` (bound(L, U, return_A,
+        plane_rows, rows) and the device problem net.dev), the frontier, the
+        thresholds and the ALM."""
         self.p = problem
-        self.cfg = cfg or BaBConfig()
-        self.net = BoundedQCQP(problem, device, precision=precision)
+        self.cfg = cfg
+        self.net = net
         self.device = self.net.device
         if self.cfg.clip not in ("none", "complete"):
             raise ValueError(f"clip {self.cfg.clip!r}: 'none' or 'complete'")
