@@ -55,6 +55,10 @@ SIGNATURES = {
     "gh_fill_order_i64": [_I, _P, ctypes.c_int64, ctypes.c_int64, _P, _P],
     "gh_merge_desc_f64": [ctypes.c_int64, _P, _P, ctypes.c_int64, _P, _P, _P, _P, _P],
     "gh_split_append_workspace_bytes": [_I],
+    "gh_branch_sb_f64": [_I, _I, _I, _P, _P, _P, _P, _P, _P, ctypes.c_double, _P,
+                         ctypes.c_double, _P, _P, _P],
+    "gh_split_append_idx_f64": [_I, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _L,
+                                _L, _P, _L, _P, _P, _P, _P],
     "gh_split_append_f64_noalloc": [_I, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _L, _L,
                                     _P, _L, _P, _P, _P],
     "gh_split_append_noalloc_max_parents": [],

@@ -20,7 +20,7 @@ from dataclasses import dataclass, field
 import torch
 
 from .alm import ALMConfig, ALMPGDSolver, replay_float64
-from .crown import (BoundedQCQP, DeviceFrontier, child_keys, input_split_branching,
+from .crown import (BoundedQCQP, DeviceFrontier, branch_sb, child_keys, input_split_branching,
                     input_split_parallel)
 
 
@@ -58,7 +58,8 @@ def no_cuts_row(n, S, device):
 
 
 def reordered_step(f, bound, thr, nb, n, S, planes=True, lp=None, clip=False,
-                   clip_max_rows=16, sync=True, in_place=False):
+                   clip_max_rows=16, sync=True, in_place=False, branching="naive",
+                   root_width=None):
     """One reordered B&B step on the last nb domains of frontier f
     (batch_branch_and_bound.py:395-560 with reorder_bab): bound them
     (CROWN; with clip 'complete' concretized over the box and the cuts they
@@ -71,21 +72,28 @@ def reordered_step(f, bound, thr, nb, n, S, planes=True, lp=None, clip=False,
     prune drops them).  bound(L, U, return_A=, plane_rows=) is the bound
     of the served form (FP64 or FP32-sound).  in_place: bound(..., rows=)
     accepts a row index, so without clip and LP the picked domains are
-    bound and split where the frontier stores them (no gather)."""
+    bound and split where the frontier stores them (no gather).
+    branching 'sb': the split input comes from the smart-branching score of
+    the domains' planes (branch_sb, root_width for its exhaustive guard)
+    instead of the widest input; it reads every row's plane."""
+    sb = branching == "sb" and planes
+    if branching not in ("naive", "sb"):
+        raise ValueError(f"branching {branching!r}")
     if in_place and not clip and lp is None and f.in_place_room(nb):
         idx, pkey = f.pick_rows(nb)
         L, U = f.storage()
         if planes:
-            lb, lA, lbias = bound(L, U, return_A=True, plane_rows=1, rows=idx)
+            lb, lA, lbias = bound(L, U, return_A=True, plane_rows=S if sb else 1, rows=idx)
         else:
             lb, lA, lbias = bound(L, U, rows=idx), None, None
         lb = torch.maximum(lb, pkey)
-        return f.split_append(lb, L, U, thr, lA, lbias, sync=sync, rows=idx)
+        choice = branch_sb(L, U, lb, thr, lA, root_width=root_width, rows=idx) if sb else None
+        return f.split_append(lb, L, U, thr, lA, lbias, sync=sync, rows=idx, idx=choice)
     pL, pU, pkey = f.pick_out(nb)
     if planes:
-        # the LP and the cuts read every row's plane, the child keys the
-        # objective's
-        lb, lA, lbias = bound(pL, pU, return_A=True, plane_rows=S if (lp or clip) else 1)
+        # the LP, the cuts and smart branching read every row's plane, the
+        # child keys the objective's
+        lb, lA, lbias = bound(pL, pU, return_A=True, plane_rows=S if (lp or clip or sb) else 1)
     else:
         lb, lA, lbias = bound(pL, pU), None, None
     cuts = None
@@ -99,12 +107,14 @@ def reordered_step(f, bound, thr, nb, n, S, planes=True, lp=None, clip=False,
     lb = torch.maximum(lb, pkey)
     if lp is not None and lA is not None:
         lp(lb, lA, lbias, pL, pU)
+    choice = branch_sb(pL, pU, lb, thr, lA, root_width=root_width) if sb else None
     if not clip:
         # branch, bisect, key, prune and append without materializing the
         # children (the same frontier as branch_split_keys + filter_append)
-        return f.split_append(lb, pL, pU, thr, lA, lbias, sync=sync)
+        return f.split_append(lb, pL, pU, thr, lA, lbias, sync=sync, idx=choice)
     from .concretize import clip_domains
-    idx = input_split_branching(None, None, pL, pU, None, None, "naive")
+    idx = (choice.long().unsqueeze(1) if sb else
+           input_split_branching(None, None, pL, pU, None, None, "naive"))
     cL, cU, _ = input_split_parallel(pL, pU, i_idx=idx)
     clip_domains(lA, lbias, thr, cL, cU, out=(cL, cU))
     keys = child_keys(lb, lA, lbias, cL, cU)
@@ -126,6 +136,10 @@ class BaBConfig:
     # alm_interval_patience calls without a better incumbent and reset by
     # one; 0: ALM only at the root
     frontier_slack: int = None     # physical rows past cap (None: the frontier's default)
+    # split input: 'naive' (widest) or 'sb' (the GEAR presets' smart
+    # branching, branching_heuristics.py:25-125, with its exhaustive guard
+    # against the root widths)
+    branching: str = "naive"
     alm_interval: int = 5
     alm_max_interval: int = 20
     alm_interval_patience: int = 3
@@ -195,6 +209,8 @@ class QCQPBranchAndBound:
         thr[1:] += self.cfg.constr_viol_tol
         self.thr = thr.to(self.device)
         self._glb = torch.empty(1, dtype=torch.float64, device=self.device)
+        self.root_width = torch.as_tensor(problem.xu, dtype=torch.float64, device=self.device) - \
+            torch.as_tensor(problem.xl, dtype=torch.float64, device=self.device)
 
     def set_incumbent(self, obj, x):
         if obj < self.P:
@@ -253,6 +269,9 @@ class QCQPBranchAndBound:
     def step_regular(self, batch=None, sync=True):
         """Sort, pick `batch` parents, split them, bound the 2 batch
         children, prune and append (batch_branch_and_bound.py:275-652)."""
+        if self.cfg.branching != "naive":
+            raise ValueError("smart branching runs in the reordered loop (it reads the "
+                             "picked domains' planes)")
         f = self.frontier
         si = self.cfg.sort_interval
         if si > 0 and self.stats.iterations % si == 0:
@@ -298,7 +317,8 @@ class QCQPBranchAndBound:
         large = getattr(self.net, "large", False)
         reordered_step(f, self.net.bound, self.thr, nb, self.p.n, self.p.S, planes=not large,
                        lp=lp, clip=self.clip, clip_max_rows=self.cfg.clip_max_rows, sync=sync,
-                       in_place=not large)
+                       in_place=not large, branching=self.cfg.branching,
+                       root_width=self.root_width)
         self.stats.iterations += 1
         self.stats.boxes_bounded += nb
         return nb

@@ -201,11 +201,40 @@ class BoundedQCQP:
         return lb, None, A_dict
 
 
+def branch_sb(x_L, x_U, dom_lb, thresholds, lA, coeff_thresh=1e-3, root_width=None,
+              min_width_ratio=1e-3, rows=None, stream=None):
+    """(B,) int32 smart-branching choices (gh_branch_sb_f64): the GEAR
+    presets' sb heuristic (branching_heuristics.py:25-125, 176-200).
+    thresholds (S,); lA (B, S, n); rows: boxes rows[b] of x_L, x_U."""
+    B, S, n = lA.shape
+    x_L, x_U = x_L.contiguous(), x_U.contiguous()
+    dom_lb, thresholds, lA = dom_lb.contiguous(), thresholds.contiguous(), lA.contiguous()
+    if dom_lb.shape != (B, S) or thresholds.shape != (S,) or x_L.shape[1] != n:
+        raise ValueError("sb branching takes lb (B, S), thresholds (S,), lA (B, S, n)")
+    if rows is None and x_L.shape[0] != B:
+        raise ValueError("boxes (B, n) or a row index of B")
+    rw = None if root_width is None else root_width.contiguous()
+    idx = torch.empty(B, dtype=torch.int32, device=lA.device)
+    flag = torch.empty(1, dtype=torch.int32, device=lA.device)
+    check(lib().gh_branch_sb_f64(B, n, S, ptr(x_L), ptr(x_U), ptr(rows), ptr(dom_lb),
+                                 ptr(thresholds), ptr(lA), float(coeff_thresh), ptr(rw),
+                                 float(min_width_ratio), ptr(idx), ptr(flag), stream_ptr(stream)),
+          "gh_branch_sb_f64")
+    return idx
+
+
 def input_split_branching(net, dom_lb, x_L, x_U, lA, thresholds, branching_method="naive",
                           split_depth=1, **kwargs):
-    """(B, 1) int64 indices of the widest input per box (ties: lowest index)."""
-    if branching_method != "naive" or split_depth != 1:
-        raise NotImplementedError("only naive (widest) branching at depth 1 is on this path")
+    """(B, 1) int64 indices of the input to split per box: 'naive' the
+    widest (ties: lowest index), 'sb' the smart-branching score (branch_sb;
+    thresholds (S,), lA (B, S, n), root_width= optional)."""
+    if split_depth != 1:
+        raise NotImplementedError("depth-1 splits only")
+    if branching_method == "sb":
+        return branch_sb(x_L, x_U, dom_lb, thresholds, lA,
+                         root_width=kwargs.get("root_width")).long().unsqueeze(1)
+    if branching_method != "naive":
+        raise NotImplementedError(f"branching {branching_method!r}")
     B, n = x_L.shape
     x_L, x_U = x_L.contiguous(), x_U.contiguous()
     idx = torch.empty(B, dtype=torch.int32, device=x_L.device)
@@ -513,7 +542,8 @@ class DeviceFrontier:
             self._ws = torch.empty(need, dtype=torch.uint8, device=self.device)
         return self._ws
 
-    def split_append(self, plb, pL, pU, thr, lA=None, lbias=None, sync=True, rows=None):
+    def split_append(self, plb, pL, pU, thr, lA=None, lbias=None, sync=True, rows=None,
+                     idx=None):
         """Widest branching, bisection, child keys from the parents'
         objective planes, prune and append for K parents in one call
         (gh_split_append_f64): the frontier branch_split_keys followed by
@@ -548,14 +578,21 @@ class DeviceFrontier:
         self._last_B = B
         ws = self.workspace(B, lib().gh_split_append_workspace_bytes(K))
         pL, pU, plb, thr = pL.contiguous(), pU.contiguous(), plb.contiguous(), thr.contiguous()
-        check(lib().gh_split_append_f64(K, self.n, self.S, R, ptr(pL),
-                                        ptr(pU), ptr(rows), ptr(plb),
-                                        ptr(lA),
-                                        ptr(lbias), ptr(thr), ptr(self._L),
-                                        ptr(self._U), ptr(self._lb), self.hw, self.phys,
-                                        ptr(self.order[self.count:self.count + B]), self.dead,
-                                        ptr(self.total), ptr(self.disc_min), ptr(ws),
-                                        stream_ptr()), "gh_split_append_f64")
+        tail = (ptr(self._L), ptr(self._U), ptr(self._lb), self.hw, self.phys,
+                ptr(self.order[self.count:self.count + B]), self.dead, ptr(self.total),
+                ptr(self.disc_min), ptr(ws), stream_ptr())
+        if idx is None:
+            check(lib().gh_split_append_f64(K, self.n, self.S, R, ptr(pL), ptr(pU), ptr(rows),
+                                            ptr(plb), ptr(lA), ptr(lbias), ptr(thr), *tail),
+                  "gh_split_append_f64")
+        else:
+            if idx.shape != (K,) or idx.dtype != torch.int32:
+                raise ValueError("idx must be (K,) int32")
+            idx = idx.contiguous()
+            check(lib().gh_split_append_idx_f64(K, self.n, self.S, R, ptr(pL), ptr(pU),
+                                                ptr(rows), ptr(plb), ptr(lA), ptr(lbias),
+                                                ptr(thr), ptr(idx), *tail),
+                  "gh_split_append_idx_f64")
         self._pending = B
         if sync:
             return self.finish_append()

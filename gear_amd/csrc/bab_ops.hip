@@ -1059,7 +1059,8 @@ split_keep_wave_kernel(int K, int n, int S, int R, const double *__restrict__ pL
                        const double *__restrict__ pU, const int64_t *__restrict__ prows,
                   const double *__restrict__ plb,
                        const double *__restrict__ lA, const double *__restrict__ lbias,
-                       const double *__restrict__ thr, double *__restrict__ ckey,
+                       const double *__restrict__ thr, const int32_t *__restrict__ cidx,
+                       double *__restrict__ ckey,
                        int32_t *__restrict__ sidx, int32_t *__restrict__ keep,
                        int32_t *__restrict__ cnt, double *__restrict__ disc_min)
 {
@@ -1076,7 +1077,7 @@ split_keep_wave_kernel(int K, int n, int S, int R, const double *__restrict__ pL
         const double *a = lA ? lA + (size_t)p * R * n : nullptr;
         const double b = lA ? lbias[(size_t)p * R] : 0.0;
         const double p0 = prow[0];
-        const int idx = first_widest(n, lane, l, u);
+        const int idx = cidx ? cidx[p] : first_widest(n, lane, l, u);
         for (int c = 0; c < 2; ++c) {
             int fl;
             double v;
@@ -1118,7 +1119,8 @@ split_keep16_kernel(int K, int n, int S, int R, const double *__restrict__ pL,
                     const double *__restrict__ pU, const int64_t *__restrict__ prows,
                   const double *__restrict__ plb,
                     const double *__restrict__ lA, const double *__restrict__ lbias,
-                    const double *__restrict__ thr, double *__restrict__ ckey,
+                    const double *__restrict__ thr, const int32_t *__restrict__ cidx,
+                    double *__restrict__ ckey,
                     int32_t *__restrict__ sidx, int32_t *__restrict__ keep,
                     int32_t *__restrict__ cnt, double *__restrict__ disc_min)
 {
@@ -1178,7 +1180,8 @@ split_keep16_kernel(int K, int n, int S, int R, const double *__restrict__ pL,
         }
     }
     const double w0 = __shfl(ui[0] - li[0], lead);
-    const int idx = (isnan(w0) || bi < 0) ? 0 : bi;
+    // a given choice (smart branching) replaces the widest input
+    const int idx = (cidx && valid) ? cidx[p] : ((isnan(w0) || bi < 0) ? 0 : bi);
     int kept[2];
     double vt = INFINITY;
 #pragma unroll
@@ -1406,9 +1409,10 @@ namespace {
 
 int split_append_impl(int K, int n, int S, int R, const double *pL, const double *pU,
                       const int64_t *prows, const double *plb, const double *lA,
-                      const double *lbias, const double *thr, double *FL, double *FU, double *Flb,
-                      int64_t base, int64_t cap, int64_t *order_out, int64_t dead, int64_t *total,
-                      double *disc_min, void *ws, hipStream_t stream)
+                      const double *lbias, const double *thr, const int32_t *cidx, double *FL,
+                      double *FU, double *Flb, int64_t base, int64_t cap, int64_t *order_out,
+                      int64_t dead, int64_t *total, double *disc_min, void *ws,
+                      hipStream_t stream)
 {
     if (K < 0 || n < 1 || S < 1 || base < 0 || !total || !disc_min) return GH_EINVAL;
     if (lA && (R < 1 || R > S || !lbias)) return GH_EINVAL;
@@ -1425,10 +1429,11 @@ int split_append_impl(int K, int n, int S, int R, const double *pL, const double
     int32_t *cnt = keep + 2 * (size_t)K;
     if (n <= 64 && S <= 64)
         hipLaunchKernelGGL(split_keep16_kernel, dim3(nblk), dim3(256), 0, stream, K, n, S, R, pL,
-                           pU, prows, plb, lA, lbias, thr, ckey, sidx, keep, cnt, disc_min);
+                           pU, prows, plb, lA, lbias, thr, cidx, ckey, sidx, keep, cnt, disc_min);
     else
         hipLaunchKernelGGL(split_keep_wave_kernel, dim3(nblk), dim3(1024), 0, stream, K, n, S, R,
-                           pL, pU, prows, plb, lA, lbias, thr, ckey, sidx, keep, cnt, disc_min);
+                           pL, pU, prows, plb, lA, lbias, thr, cidx, ckey, sidx, keep, cnt,
+                           disc_min);
     hipLaunchKernelGGL(filter_scan_kernel, dim3(1), dim3(1024), 0, stream, 2 * nblk, cnt, total);
     if (n <= 64 && S <= 64)
         hipLaunchKernelGGL(split_copy16_kernel, dim3(nblk), dim3(256), 0, stream, K, n, S, pL, pU,
@@ -1487,8 +1492,21 @@ extern "C" int gh_split_append_f64(int K, int n, int S, int R, const double *pL,
                                    int64_t dead, int64_t *total, double *disc_min, void *ws,
                                    hipStream_t stream)
 {
-    return split_append_impl(K, n, S, R, pL, pU, prows, plb, lA, lbias, thr, FL, FU, Flb, base,
-                             cap, order_out, dead, total, disc_min, ws, stream);
+    return split_append_impl(K, n, S, R, pL, pU, prows, plb, lA, lbias, thr, nullptr, FL, FU,
+                             Flb, base, cap, order_out, dead, total, disc_min, ws, stream);
+}
+
+extern "C" int gh_split_append_idx_f64(int K, int n, int S, int R, const double *pL,
+                                       const double *pU, const int64_t *prows, const double *plb,
+                                       const double *lA, const double *lbias, const double *thr,
+                                       const int32_t *idx, double *FL, double *FU, double *Flb,
+                                       int64_t base, int64_t cap, int64_t *order_out,
+                                       int64_t dead, int64_t *total, double *disc_min, void *ws,
+                                       hipStream_t stream)
+{
+    if (K > 0 && !idx) return GH_EINVAL;
+    return split_append_impl(K, n, S, R, pL, pU, prows, plb, lA, lbias, thr, idx, FL, FU, Flb,
+                             base, cap, order_out, dead, total, disc_min, ws, stream);
 }
 
 extern "C" int gh_split_append_f64_noalloc(int K, int n, int S, int R, const double *pL,
@@ -1504,8 +1522,119 @@ extern "C" int gh_split_append_f64_noalloc(int K, int n, int S, int R, const dou
     void *ws = nullptr;
     int rc = scratch_address(&ws);
     if (rc != GH_OK) return rc;
-    return split_append_impl(K, n, S, R, pL, pU, prows, plb, lA, lbias, thr, FL, FU, Flb, base,
-                             cap, order_out, dead, total, disc_min, ws, stream);
+    return split_append_impl(K, n, S, R, pL, pU, prows, plb, lA, lbias, thr, nullptr, FL, FU,
+                             Flb, base, cap, order_out, dead, total, disc_min, ws, stream);
 }
 
 extern "C" int gh_split_append_noalloc_max_parents(void) { return kNoAllocMaxParents; }
+
+namespace {
+
+// Any non-finite plane coefficient, bound or threshold (+inf thresholds
+// count as the box's bound) sets *flag: the batch then takes the widest
+// input (branching_heuristics.py:68-87).
+__global__ void __launch_bounds__(256)
+sb_finite_kernel(int B, int n, int S, const double *__restrict__ lb,
+                 const double *__restrict__ thr, const double *__restrict__ lA,
+                 int32_t *__restrict__ flag)
+{
+    const size_t t = (size_t)blockIdx.x * 256 + threadIdx.x;
+    const size_t na = (size_t)B * S * n, nb = (size_t)B * S;
+    bool bad = false;
+    if (t < na) bad = !isfinite(lA[t]);
+    if (t < nb) {
+        const double l = lb[t];
+        const double tv = thr[t % S];
+        const double te = (isinf(tv) && tv > 0) ? l : tv;
+        bad = bad || !isfinite(l) || !isfinite(te);
+    }
+    if (__ballot(bad) != 0 && (threadIdx.x & 63) == 0) atomicOr(flag, 1);
+}
+
+__global__ void set_i32_kernel(int32_t *p, int32_t v) { *p = v; }
+
+// One wave per box: lane i's score sum_s max(|a_si|, t) w_i / 2 summed over
+// the rows in order, the largest score with the lowest index, then the
+// exhaustive guard against the widest input relative to the root.
+__global__ void __launch_bounds__(256)
+sb_choice_kernel(int B, int n, int S, const double *__restrict__ xL,
+                 const double *__restrict__ xU, const int64_t *__restrict__ rows,
+                 const double *__restrict__ lA, double coeff_thresh,
+                 const double *__restrict__ root_w, double ratio,
+                 const int32_t *__restrict__ flag, int32_t *__restrict__ idx)
+{
+    const int lane = threadIdx.x & 63;
+    const int b = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (b >= B) return;
+    const size_t br = rows ? (size_t)rows[b] : (size_t)b;
+    const double *l = xL + br * n, *u = xU + br * n;
+    if (*flag) {
+        const int w = first_widest(n, lane, l, u);
+        if (lane == 0) idx[b] = w;
+        return;
+    }
+    const double *a = lA + (size_t)b * S * n;
+    double bs = -INFINITY;
+    int bi = -1;
+    for (int i = lane; i < n; i += 64) {
+        const double w = u[i] - l[i];
+        double sc = 0.0;
+        for (int s = 0; s < S; ++s) sc = sc + (fmax(fabs(a[(size_t)s * n + i]), coeff_thresh) * w) / 2;
+        if (bi < 0 || sc > bs) {
+            bs = sc;
+            bi = i;
+        }
+    }
+    for (int o = 32; o >= 1; o >>= 1) {
+        const double s2 = __shfl_xor(bs, o);
+        const int i2 = __shfl_xor(bi, o);
+        if (i2 >= 0 && (bi < 0 || s2 > bs || (s2 == bs && i2 < bi))) {
+            bs = s2;
+            bi = i2;
+        }
+    }
+    int best = bi < 0 ? 0 : bi;
+    if (root_w && ratio > 0) {
+        double rm = -INFINITY, rc = 0.0;
+        int wi = -1;
+        for (int i = lane; i < n; i += 64) {
+            const double r = root_w[i] > 0 ? (u[i] - l[i]) / fmax(root_w[i], 1e-300) : 0.0;
+            if (wi < 0 || r > rm) {
+                rm = r;
+                wi = i;
+            }
+            if (i == best) rc = r;
+        }
+        for (int o = 32; o >= 1; o >>= 1) {
+            const double r2 = __shfl_xor(rm, o);
+            const int i2 = __shfl_xor(wi, o);
+            if (i2 >= 0 && (wi < 0 || r2 > rm || (r2 == rm && i2 < wi))) {
+                rm = r2;
+                wi = i2;
+            }
+        }
+        rc = __shfl(rc, best & 63);
+        if (rc < ratio * rm) best = wi;
+    }
+    if (lane == 0) idx[b] = best;
+}
+
+}  // namespace
+
+extern "C" int gh_branch_sb_f64(int B, int n, int S, const double *xL, const double *xU,
+                                const int64_t *rows, const double *lb, const double *thr,
+                                const double *lA, double coeff_thresh, const double *root_w,
+                                double min_width_ratio, int32_t *idx, int32_t *flag,
+                                hipStream_t stream)
+{
+    if (B < 0 || n < 1 || S < 1) return GH_EINVAL;
+    if (B == 0) return GH_OK;
+    if (!xL || !xU || !lb || !thr || !lA || !idx || !flag) return GH_EINVAL;
+    hipLaunchKernelGGL(set_i32_kernel, dim3(1), dim3(1), 0, stream, flag, 0);
+    const size_t na = (size_t)B * S * n;
+    hipLaunchKernelGGL(sb_finite_kernel, dim3((unsigned)((na + 255) / 256)), dim3(256), 0, stream,
+                       B, n, S, lb, thr, lA, flag);
+    hipLaunchKernelGGL(sb_choice_kernel, dim3((B + 3) / 4), dim3(256), 0, stream, B, n, S, xL,
+                       xU, rows, lA, coeff_thresh, root_w, min_width_ratio, flag, idx);
+    return hipGetLastError() == hipSuccess ? GH_OK : GH_ELAUNCH;
+}

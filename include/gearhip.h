@@ -320,6 +320,26 @@ int gh_branch_split_keys_f64(int K, int n, int S, int R, const double *pL, const
                              const double *plb, const double *lA, const double *lbias,
                              double *cL, double *cU, double *keys, hipStream_t stream);
 
+/*
+ * Smart branching of B boxes (input_split_branching with method 'sb',
+ * complete_verifier/input_split/branching_heuristics.py:25-125, 176-200,
+ * with the GEAR presets' options sb_sum, touch_zero_score 0): idx[b] =
+ * argmax_i sum_s max(|lA[b,s,i]|, coeff_thresh) (xU - xL)[i] / 2 (rows
+ * summed in order, lowest index on ties); an objective threshold of +inf
+ * counts as the box's own bound, and any non-finite plane coefficient,
+ * bound or threshold in the batch sends every box to the widest input;
+ * with root widths root_w (n, or NULL) and min_width_ratio > 0 a choice
+ * narrower, relative to the root, than min_width_ratio times the widest
+ * relative width becomes that widest input (the exhaustive guard).
+ *   xL, xU (B, n), or rows (B) int64 into them (may be NULL); lb (B, S);
+ *   thr (S); lA (B, S, n); idx (B) int32 out; flag: one device int32 of
+ *   scratch.
+ */
+int gh_branch_sb_f64(int B, int n, int S, const double *xL, const double *xU,
+                     const int64_t *rows, const double *lb, const double *thr, const double *lA,
+                     double coeff_thresh, const double *root_w, double min_width_ratio,
+                     int32_t *idx, int32_t *flag, hipStream_t stream);
+
 /* Bytes of device workspace gh_filter_append_f64 needs for B boxes. */
 size_t gh_filter_workspace_bytes(int B);
 
@@ -441,6 +461,17 @@ int gh_fill_order_i64(int B, const int64_t *total, int64_t hw, int64_t dead, int
 int gh_gather_rows_f64(int64_t N, const int64_t *idx, int w0, const double *s0, double *d0,
                        int w1, const double *s1, double *d1, int w2, const double *s2,
                        double *d2, hipStream_t stream);
+
+/*
+ * gh_split_append_f64 with the split input of each parent given (idx (K)
+ * int32, e.g. from gh_branch_sb_f64) instead of its widest input.
+ */
+int gh_split_append_idx_f64(int K, int n, int S, int R, const double *pL, const double *pU,
+                            const int64_t *prows, const double *plb, const double *lA,
+                            const double *lbias, const double *thr, const int32_t *idx,
+                            double *FL, double *FU, double *Flb, int64_t base, int64_t cap,
+                            int64_t *order_out, int64_t dead, int64_t *total, double *disc_min,
+                            void *ws, hipStream_t stream);
 
 /*
  * gh_split_append_f64 without a workspace argument, for callers that cannot

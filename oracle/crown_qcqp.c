@@ -294,6 +294,64 @@ void gho_branch_widest(int B, int n, const double *xL, const double *xU,
     }
 }
 
+/* Smart branching, one split per box (branching_heuristics.py:25-56,
+ * 59-101, 104-125, 176-200) with the GEAR presets' options (sb_sum, no
+ * touch-zero score, the exhaustive guard; gear-f0.yaml:518-541):
+ *   - an objective threshold of +inf (no incumbent yet) counts as the box's
+ *     own bound (:68-73);
+ *   - if any plane coefficient, bound or threshold of the batch is not
+ *     finite, every box takes the widest input (:74-87);
+ *   - else score_i = sum_s max(|lA[s][i]|, coeff_thresh) (u_i - l_i) / 2,
+ *     summed over the rows in order, the largest score wins (lowest index
+ *     on ties: the reference's topk order is unspecified);
+ *   - the exhaustive guard (:176-200): with root widths given and
+ *     min_width_ratio > 0, a choice whose width relative to the root is
+ *     below min_width_ratio times the largest relative width becomes that
+ *     widest (relative) input.
+ * lb (B, S), thr (S), lA (B, S, n), root_w (n) or NULL. */
+void gho_branch_sb(int B, int n, int S, const double *xL, const double *xU, const double *lb,
+                   const double *thr, const double *lA, double coeff_thresh,
+                   const double *root_w, double min_width_ratio, int32_t *idx)
+{
+    int finite = 1;
+    for (size_t k = 0; k < (size_t)B * S * n && finite; ++k) finite = isfinite(lA[k]);
+    for (int b = 0; b < B && finite; ++b)
+        for (int s = 0; s < S; ++s) {
+            double l = lb[(size_t)b * S + s];
+            double t = (isinf(thr[s]) && thr[s] > 0) ? l : thr[s];
+            if (!isfinite(l) || !isfinite(t)) { finite = 0; break; }
+        }
+    if (!finite) {
+        gho_branch_widest(B, n, xL, xU, idx);
+        return;
+    }
+    for (int b = 0; b < B; ++b) {
+        const double *l = xL + (size_t)b * n, *u = xU + (size_t)b * n;
+        const double *a = lA + (size_t)b * S * n;
+        int best = 0;
+        double bs = -INFINITY;
+        for (int i = 0; i < n; ++i) {
+            double w = u[i] - l[i], sc = 0.0;
+            for (int s = 0; s < S; ++s) {
+                double c = fabs(a[(size_t)s * n + i]);
+                sc = sc + (fmax(c, coeff_thresh) * w) / 2;
+            }
+            if (sc > bs) { bs = sc; best = i; }
+        }
+        if (root_w && min_width_ratio > 0) {
+            double rmax = -INFINITY, rc = 0.0;
+            int widest = 0;
+            for (int i = 0; i < n; ++i) {
+                double r = root_w[i] > 0 ? (u[i] - l[i]) / fmax(root_w[i], 1e-300) : 0.0;
+                if (r > rmax) { rmax = r; widest = i; }
+                if (i == best) rc = r;
+            }
+            if (rc < min_width_ratio * rmax) best = widest;
+        }
+        idx[b] = best;
+    }
+}
+
 /* Bisection (split.py:26-122 with split_partitions = 2, depth 1): child 0
  * is [mid, xU] and child 1 is [xL, mid] on the split dimension, with
  * mid = (1*xL + 1*xU) / 2; children are laid out [all child 0; all child 1]

@@ -130,6 +130,23 @@ def _prog_struct(prog):
     return st, (nodes, out)   # keep the arrays alive with the struct
 
 
+def branch_sb(L, U, lb, thr, lA, coeff_thresh=1e-3, root_w=None, min_width_ratio=1e-3):
+    """(B,) int32 smart-branching choices (gho_branch_sb)."""
+    L, U, lb, lA = _c(L), _c(U), _c(lb), _c(lA)
+    thr = _c(np.asarray(thr, dtype=np.float64))
+    B, S, n = lA.shape
+    idx = np.empty(B, dtype=np.int32)
+    h = lib()
+    h.gho_branch_sb.argtypes = [ctypes.c_int] * 3 + [_D] * 5 + [ctypes.c_double, ctypes.c_void_p,
+                                                               ctypes.c_double, _I32]
+    h.gho_branch_sb.restype = None
+    rw = None if root_w is None else _c(np.asarray(root_w, dtype=np.float64))
+    h.gho_branch_sb(B, n, S, L, U, lb, thr, lA, float(coeff_thresh),
+                    None if rw is None else rw.ctypes.data_as(ctypes.c_void_p),
+                    float(min_width_ratio), idx)
+    return idx
+
+
 def _dag_lib():
     h = lib()
     if not hasattr(h, "_dag_ready"):

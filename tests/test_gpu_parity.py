@@ -523,3 +523,87 @@ def test_merge_desc_equals_stable_sort(cuda):
                                        ptr(kB) if nB else None, ptr(vB) if nB else None, ptr(kO),
                                        ptr(vO), stream_ptr()) == 0
         assert torch.equal(vO, refv), (nA, nB)
+
+
+@pytest.mark.parametrize("case", ["plain", "ties", "nonfinite", "guard", "rows"])
+def test_branch_sb_matches_oracle(cuda, case):
+    """gh_branch_sb_f64 equals the oracle's smart branching: the score's
+    argmax with the lowest index on ties, +inf thresholds taken as the box's
+    bound, the batch-wide widest fallback on a non-finite input, the
+    exhaustive guard, boxes read through a row index."""
+    from gear_amd.crown import branch_sb
+    print(f"case sb {case}", flush=True)
+    rng = np.random.default_rng(hash(case) % 1000)
+    B, S, n = 300, 5, 37
+    L = rng.uniform(-1, 0, (B, n))
+    U = L + rng.uniform(1e-6, 1.0, (B, n))
+    lA = rng.standard_normal((B, S, n)) * (rng.random((B, S, n)) < 0.5)
+    lb = rng.standard_normal((B, S))
+    thr = np.array([np.inf, 0.3, 0.1, 1.0, 0.0])
+    root = np.full(n, 2.0)
+    rows = None
+    if case == "ties":
+        U = L + 0.5
+        lA = np.sign(lA)
+    elif case == "nonfinite":
+        lb[7, 2] = -np.inf
+    elif case == "guard":
+        U[:, :5] = L[:, :5] + 1e-9           # narrow inputs with large coefficients
+        lA[:, :, :5] = 1e3
+    elif case == "rows":
+        perm = rng.permutation(B)
+        rows = perm
+    d = lambda a: _dev(a, cuda)
+    if rows is None:
+        got = branch_sb(d(L), d(U), d(lb), d(thr), d(lA), root_width=d(root))
+        want = O.branch_sb(L, U, lb, thr, lA, root_w=root)
+    else:
+        got = branch_sb(d(L), d(U), d(lb), d(thr), d(lA), root_width=d(root),
+                        rows=torch.as_tensor(rows, device=cuda))
+        want = O.branch_sb(L[rows], U[rows], lb, thr, lA, root_w=root)
+    torch.cuda.synchronize()
+    assert np.array_equal(got.cpu().numpy(), want)
+    if case == "guard":
+        assert np.all(want >= 5)
+
+
+def test_reordered_bab_with_sb_matches_oracle_loop(cuda):
+    """The reordered loop with smart branching (the GEAR presets' 'sb')
+    reproduces the oracle-driven loop bit for bit: the picked domains'
+    planes of every row give the split input, then children, keys, prune."""
+    from gear_amd.bab import BaBConfig, QCQPBranchAndBound
+    p = qcqp_dense(10, 3, seed=11)
+    batch, iters = 16, 8
+    bb = QCQPBranchAndBound(p, BaBConfig(batch=batch, cap=4096, branching="sb"), device=cuda)
+    bb.root(alm=False)
+    bb.set_incumbent(-1.0, None)
+    FL, FU = p.xl[None].copy(), p.xu[None].copy()
+    Flb = O.qcqp_crown(p.Q, p.c, p.d, FL, FU)
+    thr = np.concatenate([[-1.0], p.rhs[1:] + 1e-6])
+    root = p.xu - p.xl
+    not_widest = 0
+    for it in range(iters):
+        keys = (Flb - thr).max(axis=1)
+        order = np.argsort(-keys, kind="stable")
+        FL, FU, Flb = FL[order], FU[order], Flb[order]
+        nb = min(2 * batch, len(FL))
+        pL, pU, pk = FL[-nb:], FU[-nb:], Flb[-nb:]
+        FL, FU, Flb = FL[:-nb], FU[:-nb], Flb[:-nb]
+        lb, lA, lbias = O.qcqp_crown(p.Q, p.c, p.d, pL, pU, return_A=True)
+        lb = np.maximum(lb, pk)
+        idx = O.branch_sb(pL, pU, lb, thr, lA, root_w=root)
+        not_widest += int((idx != O.branch_widest(pL, pU)).sum())
+        cL, cU = O.split(pL, pU, idx)
+        ck = O.child_keys(lb, lA, lbias, cL, cU)
+        keep, _, _ = O.prune(ck, thr, cL, cU)
+        FL = np.vstack([FL, cL[keep]])
+        FU = np.vstack([FU, cU[keep]])
+        Flb = np.vstack([Flb, ck[keep]])
+        assert bb.step() == nb
+        c = len(bb.frontier)
+        assert c == len(FL), it
+        assert _bits_equal(bb.frontier.L[:c].cpu().numpy(), FL)
+        assert _bits_equal(bb.frontier.U[:c].cpu().numpy(), FU)
+        assert _bits_equal(bb.frontier.lb[:c].cpu().numpy(), Flb)
+    print(f"sb chose a non-widest input {not_widest} times", flush=True)
+    assert not_widest > 0

@@ -195,3 +195,47 @@ def test_oracle_deterministic_across_threads():
     a = O.qcqp_crown(p.Q, p.c, p.d, L, U, threads=1)
     b = O.qcqp_crown(p.Q, p.c, p.d, L, U, threads=4)
     np.testing.assert_array_equal(a, b)
+
+
+def test_branch_sb_reference_expectations():
+    """The reference's explicit SB cases (complete_verifier/tests/
+    test_input_split_branching_heuristics.py): a dominant lA coefficient
+    picks its input; a non-finite bound sends the batch to the widest input."""
+    L = np.zeros((2, 4))
+    U = np.ones((2, 4))
+    lA = np.zeros((2, 1, 4))
+    lA[:, :, 2] = 10.0
+    idx = O.branch_sb(L, U, np.zeros((2, 1)), np.zeros(1), lA, coeff_thresh=0.0, root_w=None)
+    assert idx.tolist() == [2, 2]
+    idx = O.branch_sb(np.zeros((1, 4)), np.array([[0.1, 0.8, 0.2, 0.4]]),
+                      np.array([[np.nan, 0.0]]), np.zeros(2),
+                      np.random.default_rng(0).standard_normal((1, 2, 4)), root_w=None)
+    assert idx.tolist() == [1]
+
+
+def test_branch_sb_score_closed_form():
+    """score_i = sum_s max(|a_si|, t) w_i / 2 (argmax, lowest index on
+    ties); +inf objective thresholds count as the box's bound; the
+    exhaustive guard replaces a choice narrower than ratio x the widest
+    relative width."""
+    rng = np.random.default_rng(3)
+    B, S, n = 40, 3, 6
+    L = rng.uniform(-1, 0, (B, n))
+    U = L + rng.uniform(0.01, 1.0, (B, n))
+    lA = rng.standard_normal((B, S, n)) * (rng.random((B, S, n)) < 0.7)
+    lb = rng.standard_normal((B, S))
+    thr = np.array([np.inf, 0.5, 0.2])
+    idx = O.branch_sb(L, U, lb, thr, lA, coeff_thresh=1e-3, root_w=None)
+    score = (np.maximum(np.abs(lA), 1e-3) * (U - L)[:, None, :] / 2).sum(1)
+    assert np.array_equal(idx, score.argmax(1))
+    # ties: equal widths and coefficients -> the lowest index
+    idx = O.branch_sb(np.zeros((1, 3)), np.ones((1, 3)), np.zeros((1, 1)), np.zeros(1),
+                      np.ones((1, 1, 3)), root_w=None)
+    assert idx.tolist() == [0]
+    # the guard: input 0 has the only large coefficient but is 1e4 times
+    # narrower (relative to the root) than input 1
+    Lg, Ug = np.zeros((1, 2)), np.array([[1e-4, 1.0]])
+    lAg = np.array([[[1e6, 0.0]]])
+    assert O.branch_sb(Lg, Ug, np.zeros((1, 1)), np.zeros(1), lAg, root_w=None).tolist() == [0]
+    assert O.branch_sb(Lg, Ug, np.zeros((1, 1)), np.zeros(1), lAg, root_w=np.ones(2),
+                       min_width_ratio=1e-3).tolist() == [1]
