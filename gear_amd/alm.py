@@ -58,10 +58,10 @@ class ALMConfig:
     scale_eps_with_bias: bool = True
     num_restarts: int = 1000       # bab_num_restarts, when no shared budget is set
     br_budget: int = 20480         # bab_br_budget: B * R restarts per call
-    # cap on restarts x rows x n^2 (multiply-adds of one inner step over a
-    # call): the reference bounds an ALM call by wall time instead
-    # (bab_timeout), which a fused launch cannot check
-    work_budget: float = 2e10
+    # cap on the multiply-adds of a call at full length, restarts x rows x
+    # n^2 x outer x inner iterations: the reference bounds an ALM call by
+    # wall time instead (bab_timeout), which a fused launch cannot check
+    work_budget: float = 2e12
     seed: int = 0
 
 
@@ -104,8 +104,8 @@ class _GhAlmConfig(ctypes.Structure):
 def restart_count(cfg, batch, explicit=None, step_work=0):
     """resolve_alm_restart_count (alm.py:271-285): an explicit count, else
     the shared budget br_budget // batch, else num_restarts; then at most
-    work_budget / (batch * step_work) restarts per box (step_work: the
-    multiply-adds of one restart's inner step)."""
+    work_budget / (batch * step_work * outer * inner) restarts per box
+    (step_work: the multiply-adds of one restart's inner step)."""
     batch = max(1, int(batch))
     if explicit is not None:
         R = max(1, int(explicit))
@@ -114,18 +114,25 @@ def restart_count(cfg, batch, explicit=None, step_work=0):
     else:
         R = max(1, int(cfg.num_restarts))
     if step_work > 0 and cfg.work_budget > 0:
-        R = max(1, min(R, int(cfg.work_budget // (batch * step_work))))
+        steps = max(1, cfg.outer_max_iter) * max(1, cfg.inner_steps)
+        R = max(1, min(R, int(cfg.work_budget // (batch * step_work * steps))))
     return R
 
 
 class ALMPGDSolver:
-    """backend "hip": the fused kernel gh_alm_pgd_f64; "torch": the same
-    iteration as device tensor operations with autograd (the plain baseline
-    and a second implementation the kernel is tested against)."""
+    """backend "hip": the fused kernel gh_alm_pgd_f64 (a restart per
+    workgroup, its rows held in registers or LDS); "torch": the same
+    iteration as device tensor operations with autograd, every restart's
+    products in one GEMM per step (the plain baseline and a second
+    implementation the kernel is tested against); "auto": the kernel up to
+    n = 64, above it the GEMM form, which reads Q once per step for all
+    restarts where the kernel reads it once per restart."""
 
-    def __init__(self, cfg: ALMConfig = None, backend="hip"):
+    KERNEL_MAX_N = 64
+
+    def __init__(self, cfg: ALMConfig = None, backend="auto"):
         self.cfg = cfg or ALMConfig()
-        if backend not in ("hip", "torch"):
+        if backend not in ("hip", "torch", "auto"):
             raise ValueError(backend)
         self.backend = backend
         self._plans = {}
@@ -180,7 +187,11 @@ class ALMPGDSolver:
         R = restart_count(self.cfg, x_L.shape[0], num_restarts,
                           0 if hasattr(dev, "values") else dev.S * dev.n * dev.n)
         lo, hi, x = self.starts(x_L, x_U, R, x_init)
-        if self.backend == "hip":
+        backend = self.backend
+        if backend == "auto":
+            backend = ("hip" if not hasattr(dev, "values") and dev.n <= self.KERNEL_MAX_N
+                       else "torch")
+        if backend == "hip":
             if hasattr(dev, "values"):
                 raise ValueError("the fused ALM kernel covers QCQPs; use backend='torch'")
             best_x, best_obj, tr = self._run_hip(dev, lo, hi, x, trace)

@@ -30,8 +30,13 @@ def _add(a, b):
     return _widen(r, torch.where(torch.isfinite(r), r.abs(), 0) * _k(1), -1)
 
 
-def plain_qcqp_crown(dev, L, U):
-    """lb (B, S) for boxes L, U (B, n) on the device problem `dev`."""
+def plain_qcqp_crown(dev, L, U, return_A=False, plane_rows=None, rows=None, **kw):
+    """lb (B, S) for boxes L, U (B, n) on the device problem `dev`; with
+    return_A also the planes lA (B, R, n) and lbias (B, R) of the leading
+    R = plane_rows rows (the bias before concretization, as the fused path
+    returns it).  rows: bound the boxes rows[b] of L, U."""
+    if rows is not None:
+        L, U = L.index_select(0, rows), U.index_select(0, rows)
     S, n = dev.S, dev.n
     B = L.shape[0]
     W = dev.Q.reshape(S * n, n)
@@ -80,4 +85,8 @@ def plain_qcqp_crown(dev, L, U):
     ma = (A.abs() * mx[:, None, :]).sum(-1)
     root = _widen(base - et, ma * _k(n), -1)
     root = torch.where(torch.isfinite(root), root, -torch.inf)
-    return _add(bias, root)
+    lb = _add(bias, root)
+    if not return_A:
+        return lb
+    R = S if plane_rows is None else int(plane_rows)
+    return lb, A[:, :R].contiguous(), acc[:, :R].contiguous()

@@ -60,10 +60,25 @@ def poisson_stream(count, rate, seed=0, n_median=10.0, n_sigma=0.8, n_min=5, n_m
     return out
 
 
+def config5_stream(count, rate, seed=6, arrival_seed=11, m=20):
+    """The serving workload: `count` requests with exponential inter-arrival
+    times (mean 1 / rate); instance i is the config-2 generator's dense
+    QCQP (coefficients N(0,1)/sqrt(n), box [-1, 1]^n) with n =
+    clip(round(lognormal(ln 60, 1.0)), 5, 1000), m = 20 constraints and seed
+    seed + i."""
+    from .problems import serving_sizes
+    rng = np.random.default_rng(arrival_seed)
+    t = np.cumsum(rng.exponential(1.0 / rate, size=count))
+    t = t - t[0]
+    ns = serving_sizes(count, seed=seed)
+    return [Request(k, float(t[k]), qcqp_dense(int(ns[k]), m, seed=seed + k))
+            for k in range(count)]
+
+
 @dataclass
 class ServeConfig:
     precision: str = "fp32s"      # bound precision of the served form
-    hbm_budget: int = 32 << 30    # bytes the sessions may hold together
+    hbm_budget: int = 128 << 30   # bytes the sessions may hold together (of 288 GB)
     rel_tol: float = 1e-4
     abs_tol: float = 1e-5
     constr_viol_tol: float = 1e-6
@@ -74,7 +89,8 @@ class ServeConfig:
     max_batch: int = 8192
     min_batch: int = 16
     alm_restarts: int = None        # None: the ALM's shared restart budget
-    alm: ALMConfig = field(default_factory=ALMConfig)
+    # the presets' ALM with a smaller work cap per call, for latency
+    alm: ALMConfig = field(default_factory=lambda: ALMConfig(work_budget=2e11))
     # ALM on the least verified frontier boxes every alm_interval steps, and
     # every alm_interval_noinc steps while there is no incumbent
     # (run_bab_alm_step, complete_verifier/input_split/nlp_in_bab.py:1078-1231)
@@ -140,7 +156,7 @@ def footprint(n, S, cfg):
 class Session:
     """One instance's B&B on its own stream."""
 
-    def __init__(self, req: Request, cfg: ServeConfig, device, bound_fn=None, alm_backend="hip"):
+    def __init__(self, req: Request, cfg: ServeConfig, device, bound_fn=None, alm_backend="auto"):
         self.req = req
         self.cfg = cfg
         self.p = p = req.problem
@@ -172,6 +188,9 @@ class Session:
             else:
                 self.dev = p.to_device(self.device)
                 self.bound = self.bound_fn(self.dev)
+                # the same algorithm as the served form: planes (child keys,
+                # LP certificate) where the served bound has them
+                self.planes = p.n <= 64
             self.clip = cfg.clip == "complete" and self.planes and p.n <= 64
             self.frontier = DeviceFrontier(p.n, p.S, frontier_cap(p.n, p.S, cfg), self.device,
                                            payload=p.S * (p.n + 1) if self.clip else 0)
@@ -258,7 +277,7 @@ class Session:
             reordered_step(f, self.bound, self.thr, nb, self.p.n, self.p.S, planes=self.planes,
                            lp=self._raise_by_lp if self._lp() else None, clip=self.clip,
                            clip_max_rows=self.cfg.clip_max_rows, sync=False,
-                           in_place=self.bound_fn is None and self.planes)
+                           in_place=self.planes)
             self._bounded = nb
         return nb
 
@@ -437,7 +456,7 @@ def serve_plain_sequential(requests, cfg: ServeConfig = None, device="cuda", bud
     now = lambda: time.perf_counter() - t0
 
     def plain_bound(d):
-        return lambda L, U, **kw: plain_qcqp_crown(d, L, U)
+        return lambda L, U, **kw: plain_qcqp_crown(d, L, U, **kw)
 
     for req in queue:
         if budget_s is not None and now() > budget_s:

@@ -34,7 +34,7 @@ def test_schedule_matches_reference_rules(cuda, n, m, seed, root):
     p = qcqp_n50_m20_problem() if (n, m, seed) == (50, 20, 3) else qcqp_dense(n, m, seed=seed)
     dev = DeviceQCQP(p, cuda)
     cfg = ALMConfig(num_restarts=24, br_budget=0, update_multipliers_always=root)
-    res = ALMPGDSolver(cfg).solve_with_restarts(dev, *_box(p, cuda), trace=True)
+    res = ALMPGDSolver(cfg, backend="hip").solve_with_restarts(dev, *_box(p, cuda), trace=True)
     tr = res.trace.cpu().numpy()
     probs = O.alm_schedule_check(cfg, tr, p.S)
     assert probs == [], probs[:5]

@@ -110,3 +110,36 @@ def test_served_bound_is_never_above_a_feasible_point(cuda):
           flush=True)
     assert x["D"] <= fmin + 1e-9
     assert x["steps"] > 0
+
+
+def test_serving_workload_sizes_sound(cuda):
+    """The serving workload's sizes (n lognormal(ln 60, 1) in [5, 1000],
+    m = 20): sessions above n = 64 run the large-n FP32-sound bound beside
+    the small ones in the same ticks; every certificate is sound."""
+    from gear_amd.serve import config5_stream
+    reqs = config5_stream(5, rate=100.0)
+    ns = [q.problem.n for q in reqs]
+    print("case serving sizes", ns, flush=True)
+    assert max(ns) > 64 and min(ns) <= 64
+    cfg = ServeConfig(time_limit=2.0)
+    res, summ = serve(reqs, cfg, device=cuda, realtime=False)
+    print(summ, flush=True)
+    for q, r in zip(reqs, res):
+        print(q.problem.n, r["status"], r["P"], r["D"], r["steps"], r["boxes"], flush=True)
+        assert r["steps"] > 0 and r["boxes"] > 0
+        _check_certificate(q, r, cfg)
+
+
+def test_plain_form_runs_the_same_algorithm(cuda):
+    """The plain FP64 sequential form takes planes (child keys and the LP
+    certificate) where the served form does: on an n <= 64 instance both
+    run the reordered loop with planes and their certificates agree."""
+    reqs = poisson_stream(1, rate=100.0, seed=7, n_median=6, n_sigma=0.1)
+    cfg = ServeConfig(time_limit=20.0)
+    fast, _ = serve(reqs, cfg, device=cuda, realtime=False)
+    plain, _ = serve_plain_sequential(reqs, cfg, device=cuda)
+    a, b = fast[0], plain[0]
+    print("served", a, "plain", b, flush=True)
+    _check_certificate(reqs[0], b, cfg)
+    if a["status"] == b["status"] == "converged":
+        assert abs(a["D"] - b["D"]) <= 3e-4 * max(1.0, abs(a["P"]))
