@@ -1,4 +1,4 @@
-"""Large-n FP32-sound QCQP bound (batched FP32 GEMMs + FP64 relaxation) vs
+"""Large-n FP32-sound QCQP bound (fused FP32 MFMA kernels + FP64 relaxation) vs
 the FP64 oracle: never above it (zero violations), valid at sampled points,
 and looseness reported."""
 import numpy as np
@@ -78,6 +78,8 @@ def test_chunked_workspace_and_degenerate_boxes(cuda):
     lb64 = O.qcqp_crown(p.Q, p.c, p.d, L, U, threads=8)
     _compare(lb_small, lb64, "chunked", 1e-2)
     _compare(lb_full, lb64, "one chunk", 1e-2)
+    # a box's bound does not depend on the chunk it is bound in
+    assert np.array_equal(lb_small, lb_full)
 
 
 def test_empty_batch(cuda):
