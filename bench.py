@@ -59,8 +59,8 @@ def parse():
     ap.add_argument("--no-serve", action="store_true", help="skip the serving section")
     ap.add_argument("--serve-only", action="store_true",
                     help="run only the serving section (Poisson stream of QCQP instances)")
-    ap.add_argument("--serve-requests", type=int, default=24)
-    ap.add_argument("--serve-rate", type=float, default=8.0, help="arrivals per second")
+    ap.add_argument("--serve-requests", type=int, default=12)
+    ap.add_argument("--serve-rate", type=float, default=0.25, help="arrivals per second")
     ap.add_argument("--serve-plain-budget", type=float, default=60.0,
                     help="seconds the plain sequential form may run")
     ap.add_argument("--no-clip", action="store_true", help="skip the clip_type 'complete' section")
@@ -318,21 +318,24 @@ def bench_precision_large(boxes=65536, reps=2, cpu=True, sample=16, chunk=512):
     return out
 
 
-def bench_serving(requests=24, rate=8.0, plain_budget=60.0, plain=True):
-    """Serving: a Poisson stream of QCQP instances from the config-2 generator
-    (n lognormal(ln 8, 0.6) clipped to [5, 1000], m = 0.4 n), admitted by HBM
-    budget and solved to a 1e-4 relative gap with continuous batching,
-    FP32-sound bounds and FP64-replayed incumbents; beside it the plain FP64
-    form (op-by-op bound, autograd ALM) one instance at a time."""
+def bench_serving(requests=12, rate=0.25, plain_budget=60.0, plain=True, time_limit=3.0):
+    """Serving: a Poisson stream of the serving workload (config5_stream:
+    the config-2 generator with n lognormal(ln 60, 1) in [5, 1000], m = 20),
+    admitted by HBM budget and solved to a 1e-4 relative gap with
+    continuous batching, FP32-sound bounds and FP64-replayed incumbents;
+    beside it the plain FP64 form (op-by-op bound, GEMM-form ALM, the same
+    planes, keys and LP certificate) one instance at a time.  The arrival
+    rate loads the plain form at about 80 % (its service time is at most
+    the per-instance time limit)."""
     import torch
 
-    from gear_amd.serve import ServeConfig, poisson_stream, serve, serve_plain_sequential
+    from gear_amd.serve import ServeConfig, config5_stream, serve, serve_plain_sequential
 
     dev = torch.device("cuda:0")
-    reqs = poisson_stream(requests, rate, seed=11, n_median=8.0, n_sigma=0.6)
-    cfg = ServeConfig()
-    # warm the kernels once (not timed)
-    serve(poisson_stream(2, 100.0, seed=99, n_median=6, n_sigma=0.2), cfg, device=dev,
+    reqs = config5_stream(requests, rate)
+    cfg = ServeConfig(time_limit=time_limit)
+    # warm the kernels once (not timed): one small and one large instance
+    serve(config5_stream(2, 100.0, seed=99), ServeConfig(time_limit=0.5), device=dev,
           realtime=False)
     res, summ = serve(reqs, cfg, device=dev, realtime=True)
     out = {"value": round(summ["subdomain_bounds_per_s"], 1),
@@ -342,13 +345,15 @@ def bench_serving(requests=24, rate=8.0, plain_budget=60.0, plain=True):
            "converged": summ["converged"], "timeout": summ["timeout"],
            "infeasible": summ["infeasible"], "censored": summ["censored"],
            "wall_s": round(summ["wall_s"], 3),
-           "config": {"workload": f"{requests} QCQP instances, Poisson arrivals at {rate}/s, "
-                                  "n lognormal(ln 8, 0.6) in [5, 1000], m = 0.4 n, "
-                                  "rel gap 1e-4, time limit 10 s per instance",
+           "config": {"workload": f"{requests} QCQP instances (config-2 generator, n lognormal("
+                                  f"ln 60, 1.0) in [5, 1000], m = 20, seeds 6 + i), Poisson "
+                                  f"arrivals at {rate}/s, rel gap 1e-4, time limit "
+                                  f"{time_limit} s per instance",
                       "n": [r["n"] for r in res], "precision": cfg.precision,
                       "hbm_budget_bytes": cfg.hbm_budget},
            "time_to_gap_s": [round(r["time_to_gap"], 4) if r["time_to_gap"] < float("inf")
-                             else None for r in res]}
+                             else None for r in res],
+           "D_at_end": [r["D"] for r in res], "P_at_end": [r["P"] for r in res]}
     if plain:
         pres, psumm = serve_plain_sequential(reqs, cfg, device=dev, budget_s=plain_budget)
         out["plain_fp64_sequential"] = {
@@ -358,7 +363,8 @@ def bench_serving(requests=24, rate=8.0, plain_budget=60.0, plain=True):
             "converged": psumm["converged"], "timeout": psumm["timeout"],
             "censored": psumm["censored"],
             "not_started": sum(r["status"] == "not_started" for r in pres),
-            "wall_s": round(psumm["wall_s"], 3), "budget_s": plain_budget}
+            "wall_s": round(psumm["wall_s"], 3), "budget_s": plain_budget,
+            "D_at_end": [r["D"] for r in pres]}
         same = [abs(a["D"] - b["D"]) <= 3e-4 * max(1.0, abs(a["P"]))
                 for a, b in zip(res, pres) if a["status"] == b["status"] == "converged"]
         out["plain_fp64_sequential"]["certificates_agree"] = f"{sum(same)}/{len(same)}"

@@ -85,7 +85,7 @@ class ServeConfig:
     time_limit: float = 10.0      # per instance, from admission
     cap: int = 1 << 21            # frontier capacity per instance (rows)
     session_bytes: int = 2 << 30  # and at most this many bytes of frontier
-    step_flops: float = 4e9       # bound work per step that sets the batch size
+    step_flops: float = 2e10      # bound work per step that sets the batch size
     max_batch: int = 8192
     min_batch: int = 16
     alm_restarts: int = None        # None: the ALM's shared restart budget
