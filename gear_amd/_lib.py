@@ -40,6 +40,7 @@ SIGNATURES = {
     "gh_alm_plan_bytes": [_I, _I],
     "gh_alm_prepare": [_I, _I, _P, _P, _P],
     "gh_alm_trace_width": [_I],
+    "gh_dag_plan_work": [_P, _P, _P],
     "gh_alm_pgd_f64": [_I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P],
     "gh_lp_dual_bound_f64": [_I, _I, _I, _P, _P, _P, _P, _P, _I, _P, _P, _P],
     "gh_child_keys_f64": [_I, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P],

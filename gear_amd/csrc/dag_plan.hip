@@ -87,6 +87,30 @@ struct Builder {
     }
     // per program: state words, I/O words and backward steps per box
     std::vector<int64_t> prog_words, prog_io, prog_steps;
+    // per program, per box: algorithmic FP64 operations and transcendental
+    // evaluations by the measurement's counting rule (add, sub, mul 1, fma 2;
+    // min / max / compare / select / abs 0; a relaxation ~10 per element,
+    // the A scaling 4 and the A67 enclosure 2 per element per side,
+    // concretization 4 per input per side, interval propagation 2 per node)
+    std::vector<int64_t> prog_flops, prog_trans;
+
+    // the counting rule applied to one pass
+    int64_t pass_flops(int pidx) const
+    {
+        const DagPass &P = passes[pidx];
+        const int sides = P.want_u ? 2 : 1;
+        int64_t f = 0;
+        for (int k = 0; k < P.nsteps; ++k) {
+            const DagStep &st = steps[P.step_off + k];
+            if (st.op == GH_OP_INPUT || st.op == GH_OP_CONST) {
+                f += 2 * sides;
+                continue;
+            }
+            const int pushes = is_binary(st.op) ? 2 : 1;
+            f += sides * pushes * (4 + 2);   // A scaling and enclosure per receiving input
+        }
+        return f + (int64_t)4 * P.ncin * sides;
+    }
 
     // 8-byte slot accesses per lane of one pass, as the kernel performs them
     int64_t pass_words(int pidx) const
@@ -244,6 +268,8 @@ struct Builder {
         prog_words.push_back(0);
         prog_io.push_back(0);
         prog_steps.push_back(0);
+        prog_flops.push_back(0);
+        prog_trans.push_back(0);
         const int n = G.n, Vg = G.V;
         if (n < 0 || Vg < n || G.R < 0 || (Vg > 0 && !G.nodes) || (G.R > 0 && !G.out))
             return GH_EINVAL;
@@ -323,6 +349,7 @@ struct Builder {
             J.pass_off = (int32_t)passes.size();
             int64_t cost = V;
             int64_t words = 2 * (int64_t)ni;  // the row's inputs into L / U
+            int64_t flops = 0, trans = 0;
             for (int k = ni; k < V; ++k) {
                 DagFwd f{};
                 f.op = N[k].op;
@@ -334,10 +361,16 @@ struct Builder {
                 if (N[k].op == GH_OP_SIN || N[k].op == GH_OP_COS) f.tab = table(0);
                 else if (odd_power(G.nodes[N[k].gid])) f.tab = table((int)N[k].c);
                 words += 4 + (is_binary(f.op) ? 2 : 0) + (rs[k] < 0 ? 0 : (f.op == GH_OP_MUL ? 7 : 5));
+                flops += 2 + (rs[k] < 0 ? 0 : 10);
+                if (f.op == GH_OP_EXP) trans += 5;        // interval 2, planes 3
+                else if (f.op == GH_OP_LOG) trans += 5;
+                else if (f.op == GH_OP_SIN || f.op == GH_OP_COS) trans += 8;
+                else if (f.op == GH_OP_POW && f.c != std::floor(f.c)) trans += 6;
                 if (needs[N[k].gid] && N[k].op != GH_OP_CONST) {
                     f.pass = add_pass(N, ni, k, 1, abase, &apeak, rs);
                     cost += passes.back().nsteps;
                     words += pass_words(f.pass) + 2;
+                    flops += pass_flops(f.pass);
                 }
                 fwd.push_back(f);
             }
@@ -347,8 +380,11 @@ struct Builder {
                 J.final_pass = add_pass(N, ni, J.out, 0, abase, &apeak, rs);
                 cost += passes.back().nsteps;
                 words += pass_words(J.final_pass);
+                flops += pass_flops(J.final_pass);
             }
             prog_words[p] += words;
+            prog_flops[p] += flops;
+            prog_trans[p] += trans;
             prog_io[p] += 2 * (int64_t)ni + 1;  // box endpoints in, bound out
             prog_steps[p] += cost - V;
             J.slots = abase + 2 * apeak;
@@ -420,6 +456,8 @@ extern "C" int gh_dag_plan_create(int nprog, const gh_prog *progs, gh_dag_plan *
         P->state_bytes.push_back(8 * B.prog_words[p]);
         P->io_bytes.push_back(8 * B.prog_io[p]);
         P->visits.push_back(B.prog_steps[p]);
+        P->flops.push_back(B.prog_flops[p]);
+        P->trans.push_back(B.prog_trans[p]);
     }
     *plan = P;
     return GH_OK;
@@ -451,6 +489,16 @@ extern "C" int gh_dag_plan_stats(const gh_dag_plan *plan, int64_t *state_bytes, 
         if (state_bytes) state_bytes[p] = plan->state_bytes[p];
         if (io_bytes) io_bytes[p] = plan->io_bytes[p];
         if (steps) steps[p] = plan->visits[p];
+    }
+    return GH_OK;
+}
+
+extern "C" int gh_dag_plan_work(const gh_dag_plan *plan, int64_t *flops, int64_t *transcendentals)
+{
+    if (!plan) return GH_EINVAL;
+    for (int64_t p = 0; p < plan->nprog; ++p) {
+        if (flops) flops[p] = plan->flops[p];
+        if (transcendentals) transcendentals[p] = plan->trans[p];
     }
     return GH_OK;
 }

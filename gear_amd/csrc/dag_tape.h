@@ -95,6 +95,8 @@ struct gh_dag_plan {
     // per program, per box: bytes of per-lane state the tape reads and
     // writes (the A propagation and relaxation data), and box / bound I/O
     std::vector<int64_t> state_bytes, io_bytes, visits;
+    // per program, per box: algorithmic FP64 operations and transcendentals
+    std::vector<int64_t> flops, trans;
 };
 
 #endif

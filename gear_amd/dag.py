@@ -57,6 +57,11 @@ class DagPlan:
         self.state_bytes = list(st)[:len(arrs)]
         self.io_bytes = list(io)[:len(arrs)]
         self.steps = list(steps)[:len(arrs)]
+        fl, tr = (ctypes.c_int64 * k)(), (ctypes.c_int64 * k)()
+        _lib.check(_lib.lib().gh_dag_plan_work(handle, fl, tr), "dag plan work")
+        # per program, per box: algorithmic FP64 operations, transcendentals
+        self.flops = list(fl)[:len(arrs)]
+        self.transcendentals = list(tr)[:len(arrs)]
         self._ws = None
         self._ws_waves = None
 

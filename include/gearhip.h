@@ -518,6 +518,13 @@ int gh_dag_plan_info(const gh_dag_plan *plan, int64_t *jobs, int64_t *max_slots,
  * bytes of box and bound I/O, and backward-pass node visits. */
 int gh_dag_plan_stats(const gh_dag_plan *plan, int64_t *state_bytes, int64_t *io_bytes,
                       int64_t *steps);
+/* Per program (host arrays of nprog, each may be NULL), per box bound: the
+ * algorithmic FP64 operations of every row's bound (add, sub, mul 1, fma 2;
+ * a relaxation ~10 per node, the A scaling 4 and the rounding enclosure 2
+ * per pushed A entry and side, concretization 4 per input and side,
+ * interval propagation 2 per node; intermediate-bound passes included) and
+ * the transcendental evaluations, counted from the tape. */
+int gh_dag_plan_work(const gh_dag_plan *plan, int64_t *flops, int64_t *transcendentals);
 
 /* One program's batch of boxes (device pointers).  lA, lbias and undef may
  * be NULL: each row's lower plane lA x + lbias over the program's inputs
