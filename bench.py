@@ -27,10 +27,11 @@ sys.path.insert(0, HERE)
 
 FP64_PEAK_TFLOPS = 78.6
 FP32_PEAK_TFLOPS = 157.3
-# HBM bytes per bound launch of 32,768 boxes read in place, measured with rocprofv3
-# FETCH_SIZE (x2.00) + WRITE_SIZE (x1.00), scale factors from a 256 MiB copy
-# (profiles/qcqp_crown_f64_n50_m20.md); scaled to the launch size below.
-MEASURED_TRAFFIC_BYTES_32K = 80.9e6
+# HBM bytes per bound launch of 8,192 boxes read in place (the loop at the 2^18
+# frontier cap), measured with rocprofv3 FETCH_SIZE (x2.00) + WRITE_SIZE (x1.00),
+# scale factors from a 256 MiB copy (profiles/qcqp_crown_f64_n50_m20.md); scaled
+# to the launch size below.
+MEASURED_TRAFFIC_BYTES_8K = 35.6e6
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the expression-program bound on the 256-instance
 # workload (rocprofv3 FETCH_SIZE x2.00 + WRITE_SIZE x1.00,
@@ -639,7 +640,7 @@ def main():
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": round(achieved_tf / FP64_PEAK_TFLOPS, 4),
-            "traffic": round(MEASURED_TRAFFIC_BYTES_32K * avg_boxes / 32768),
+            "traffic": round(MEASURED_TRAFFIC_BYTES_8K * avg_boxes / 8192),
             "traffic_source": "rocprofv3 FETCH_SIZE/WRITE_SIZE, profiles/qcqp_crown_f64_n50_m20.md",
             "flops_per_box": fpb,
             "bytes_per_box": bytes_per_box(n, S),
