@@ -117,11 +117,11 @@ int gh_qcqp_crown_rows_f32s(int B, int n, int S, const void *packed, const doubl
  * FP32-sound CROWN lower bound of a QCQP of any size.  Same contract and
  * the same guarantee as gh_qcqp_crown_f32s (a valid lower bound, not above
  * the FP64 bound; near-tie McCormick choices settled with the FP64 path's
- * arithmetic, flag[b] = 1 when that happened), arranged for large n: the
- * dense products y = Q_s x (centre and radius) and Q_s^T beta run as
- * strided-batched FP32 GEMMs over all rows and boxes, with error terms that
- * hold for any summation order; the relaxation, error terms and
- * concretization run in FP64 from the FP32 products.
+ * arithmetic, flag[b] = 1 when that happened), arranged for large n: per
+ * row and tile of boxes, one fused kernel forms the products Q_s x0,
+ * |Q_s| ep and |Q_s| |x0| on FP32 MFMA and relaxes each element in FP64 in
+ * its epilogue, a second forms Q_s^T beta + Ainit and the concretization
+ * sums, with error terms that hold for any summation order.
  *   plan: gh_qcqp_large_plan_bytes(n, S) bytes, filled once by
  *   gh_qcqp_large_prepare from Q (S, n, n) FP64;
  *   Q64: the same Q (FP64, for the near-tie cases); Cm, d, xL, xU, lb, flag
