@@ -73,7 +73,8 @@ SIGNATURES = {
     "gh_dag_plan_info": [_P, _P, _P, _P],
     "gh_dag_plan_stats": [_P, _P, _P, _P],
     "gh_dag_workspace_bytes": [_P, _I],
-    "gh_dag_crown_f64": [_P, _P, _P, ctypes.c_size_t, _I, _P],
+    "gh_dag_crown_f64": [_P, _P, _P, ctypes.c_size_t, _I, _I, _P],
+    "gh_dag_plan_aslots": [_P, _P],
 }
 _RESTYPES = {"gh_filter_workspace_bytes": ctypes.c_size_t,
              "gh_split_append_workspace_bytes": ctypes.c_size_t, "gh_qcqp_pack_bytes": ctypes.c_size_t,

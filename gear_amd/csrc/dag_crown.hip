@@ -38,6 +38,8 @@ __host__ __device__ constexpr double kround(int n) { return (double)(n + 16) * E
 constexpr double K1 = kround(1);
 constexpr double K2 = kround(2);
 constexpr int kLanes = 64;
+// LDS slots per wavefront at most (64 slots = 32 KB)
+constexpr int kMaxLdsSlots = 64;
 
 __device__ __forceinline__ double add_enc(double a, double b, double s)
 {
@@ -50,9 +52,19 @@ __device__ __forceinline__ double mulz(double a, double d) { return a == 0 ? 0.0
 __device__ __forceinline__ double re1(double m) { return m * K1; }
 
 // Per-lane slot view of the wavefront workspace.
+// Slot a of the A area [ab, ab + ka) lives in LDS (the most accessed slots:
+// every pass reads and writes them), every other slot in the wavefront's
+// global workspace.  Slots are wave-uniform, so the choice is a uniform branch.
 struct Lane {
-    double *w;  // workspace base + lane
-    __device__ __forceinline__ double &operator[](int slot) const { return w[(size_t)slot * kLanes]; }
+    double *w;    // workspace base + lane
+    double *lds;  // LDS area + lane
+    int ab, ka;
+    __device__ __forceinline__ double &operator[](int slot) const
+    {
+        const unsigned q = (unsigned)(slot - ab);
+        if (q < (unsigned)ka) return lds[(size_t)q * kLanes];
+        return w[(size_t)slot * kLanes];
+    }
 };
 
 __device__ __forceinline__ bool crosses(double l, double u, double point)
@@ -587,9 +599,11 @@ dag_crown_kernel(const DagJob *__restrict__ jobs, int64_t njobs, const DagFwd *_
                  const int32_t *__restrict__ gin, const double *__restrict__ tabs,
                  const gh_dag_batch *__restrict__ batches,
                  const int64_t *__restrict__ job_start, int64_t *counter, double *ws,
-                 int64_t slots)
+                 int64_t slots, int lds_slots)
 {
-    Lane W{ws + (size_t)blockIdx.x * (size_t)slots * kLanes + threadIdx.x};
+    extern __shared__ double sh_a[];
+    Lane W{ws + (size_t)blockIdx.x * (size_t)slots * kLanes + threadIdx.x, sh_a + threadIdx.x, 0,
+           lds_slots};
     const int64_t total = job_start[njobs];
     for (;;) {
         unsigned long long item = 0;
@@ -608,6 +622,7 @@ dag_crown_kernel(const DagJob *__restrict__ jobs, int64_t njobs, const DagFwd *_
         const DagJob J = jobs[lo];
         const gh_dag_batch bt = batches[J.prog];
         const int64_t count = *bt.count;
+        W.ab = uni(J.abase);
         run_job(J, fwd, passes, steps, cin, gin, tabs, bt, count, it - job_start[lo], W);
     }
 }
@@ -623,9 +638,9 @@ extern "C" size_t gh_dag_workspace_bytes(const gh_dag_plan *plan, int waves)
 }
 
 extern "C" int gh_dag_crown_f64(const gh_dag_plan *plan, const gh_dag_batch *batches, void *ws,
-                                size_t ws_bytes, int waves, hipStream_t stream)
+                                size_t ws_bytes, int waves, int lds_slots, hipStream_t stream)
 {
-    if (!plan || waves < 1) return GH_EINVAL;
+    if (!plan || waves < 1 || lds_slots < 0 || lds_slots > kMaxLdsSlots) return GH_EINVAL;
     if (plan->njobs == 0) return GH_OK;
     if (!batches || !ws || ws_bytes < gh_dag_workspace_bytes(plan, waves)) return GH_EINVAL;
     int64_t *job_start = static_cast<int64_t *>(ws);
@@ -633,8 +648,9 @@ extern "C" int gh_dag_crown_f64(const gh_dag_plan *plan, const gh_dag_batch *bat
     double *slots = reinterpret_cast<double *>(static_cast<char *>(ws) + sched_bytes(plan->njobs));
     hipLaunchKernelGGL(dag_schedule_kernel, dim3(1), dim3(1024), 0, stream, plan->jobs,
                        plan->njobs, batches, job_start, counter);
-    hipLaunchKernelGGL(dag_crown_kernel, dim3(waves), dim3(kLanes), 0, stream, plan->jobs,
+    const size_t lds = (size_t)lds_slots * kLanes * sizeof(double);
+    hipLaunchKernelGGL(dag_crown_kernel, dim3(waves), dim3(kLanes), lds, stream, plan->jobs,
                        plan->njobs, plan->fwd, plan->passes, plan->steps, plan->cin, plan->gin,
-                       plan->tabs, batches, job_start, counter, slots, plan->max_slots);
+                       plan->tabs, batches, job_start, counter, slots, plan->max_slots, lds_slots);
     return hipGetLastError() == hipSuccess ? GH_OK : GH_ELAUNCH;
 }

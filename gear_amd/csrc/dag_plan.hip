@@ -388,6 +388,7 @@ struct Builder {
             prog_io[p] += 2 * (int64_t)ni + 1;  // box endpoints in, bound out
             prog_steps[p] += cost - V;
             J.slots = abase + 2 * apeak;
+            J.abase = abase;
             if (J.slots > 32767) return GH_EUNSUPPORTED;
             J.cost = cost;
             jobs.push_back(J);
@@ -450,7 +451,11 @@ extern "C" int gh_dag_plan_create(int nprog, const gh_prog *progs, gh_dag_plan *
     P->ngin = (int64_t)B.gin.size();
     P->nprog = nprog;
     P->max_slots = 1;
-    for (auto &J : B.jobs) P->max_slots = std::max<int64_t>(P->max_slots, J.slots);
+    P->max_aslots = 0;
+    for (auto &J : B.jobs) {
+        P->max_slots = std::max<int64_t>(P->max_slots, J.slots);
+        P->max_aslots = std::max<int64_t>(P->max_aslots, J.slots - J.abase);
+    }
     P->tape_bytes = (int64_t)buf.size();
     for (int p = 0; p < nprog; ++p) {
         P->state_bytes.push_back(8 * B.prog_words[p]);
@@ -500,5 +505,12 @@ extern "C" int gh_dag_plan_work(const gh_dag_plan *plan, int64_t *flops, int64_t
         if (flops) flops[p] = plan->flops[p];
         if (transcendentals) transcendentals[p] = plan->trans[p];
     }
+    return GH_OK;
+}
+
+extern "C" int gh_dag_plan_aslots(const gh_dag_plan *plan, int64_t *max_aslots)
+{
+    if (!plan || !max_aslots) return GH_EINVAL;
+    *max_aslots = plan->max_aslots;
     return GH_OK;
 }

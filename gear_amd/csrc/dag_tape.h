@@ -19,7 +19,9 @@
 //   slot V + k      U of local node k
 //   rs .. rs+6      relaxation planes of a relaxed node (MUL: 6 + flags,
 //                   unary: 4 + flags at rs + 4)
-//   sa, sa + 1      A (lower, upper) of a node during a pass
+//   sa, sa + 1      A (lower, upper) of a node during a pass: the A area,
+//                   slots abase .. slots - 1, the most accessed per slot;
+//                   the kernel keeps its first lds_slots slots in LDS
 #ifndef GEARHIP_DAG_TAPE_H
 #define GEARHIP_DAG_TAPE_H
 
@@ -74,6 +76,7 @@ struct DagJob {
     int32_t n, R;            // program inputs (box stride) and rows
     int32_t gin_off;         // global indices of the local inputs
     int32_t slots;           // per-lane slots this job uses
+    int32_t abase, pad;      // first slot of the A area (the passes' A pairs)
     int64_t cost;            // backward steps + nodes (scheduling weight)
 };
 
@@ -84,6 +87,7 @@ struct gh_dag_plan {
     int device;
     int64_t njobs, nfwd, npass, nsteps, ncin, ngin, nprog;
     int64_t max_slots, tape_bytes;
+    int64_t max_aslots;         // largest A area of a job (slots)
     void *dbuf;                 // one device allocation holding every section
     const gh::DagJob *jobs;
     const gh::DagFwd *fwd;

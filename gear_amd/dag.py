@@ -50,6 +50,9 @@ class DagPlan:
         _lib.check(_lib.lib().gh_dag_plan_info(handle, ctypes.byref(jobs), ctypes.byref(slots),
                                                ctypes.byref(tape)), "dag plan info")
         self.jobs, self.max_slots, self.tape_bytes = jobs.value, slots.value, tape.value
+        asl = ctypes.c_int64()
+        _lib.check(_lib.lib().gh_dag_plan_aslots(handle, ctypes.byref(asl)), "dag plan aslots")
+        self.max_aslots = asl.value
         k = max(1, len(arrs))
         st, io, steps = (ctypes.c_int64 * k)(), (ctypes.c_int64 * k)(), (ctypes.c_int64 * k)()
         _lib.check(_lib.lib().gh_dag_plan_stats(handle, st, io, steps), "dag plan stats")
@@ -78,6 +81,8 @@ class DagPlan:
 
     # -- device buffers
     def workspace(self, waves):
+        if isinstance(waves, tuple):
+            waves = waves[0]
         if self._ws is None or self._ws_waves != waves:
             nbytes = _lib.lib().gh_dag_workspace_bytes(self._h, waves)
             self._ws = torch.empty(max(nbytes, 256), dtype=torch.uint8, device=self.device)
@@ -110,31 +115,57 @@ class DagPlan:
         return torch.tensor(np.asarray(rows, dtype=np.uint64).view(np.int64).reshape(-1, 7),
                             dtype=torch.int64).to(self.device)
 
-    def launch(self, batches, waves, stream=None):
+    # LDS per compute unit the A areas may take (of 160 KB)
+    LDS_BUDGET = 144 << 10
+
+    def lds_slots_for(self, waves):
+        """A-area slots kept in LDS per wavefront at `waves` resident
+        wavefronts: the plan's largest A area, within the LDS budget."""
+        props = torch.cuda.get_device_properties(self.device)
+        per_cu = max(1, -(-int(waves) // props.multi_processor_count))
+        fit = self.LDS_BUDGET // (per_cu * 64 * 8)
+        return int(max(0, min(self.max_aslots, fit, 64)))
+
+    def launch(self, batches, waves, stream=None, lds_slots=None):
         ws = self.workspace(waves)
+        if isinstance(waves, tuple):
+            waves, lds_slots = waves
+        if lds_slots is None:
+            lds_slots = self.lds_slots_for(waves)
         _lib.check(_lib.lib().gh_dag_crown_f64(self._h, batches.data_ptr(), ws.data_ptr(),
-                                               ws.numel(), waves, _lib.stream_ptr(stream)),
+                                               ws.numel(), waves, int(lds_slots),
+                                               _lib.stream_ptr(stream)),
                    "dag crown")
 
     def tune_waves(self, batches, total_boxes, tuner=None):
-        """Resident wavefronts for this plan and batch size, picked by the
-        autotuner among 2, 4, 8 and 12 waves per compute unit."""
+        """(resident wavefronts, A-area LDS slots) for this plan and batch
+        size, picked by the autotuner among 2, 4, 8 and 12 waves per compute
+        unit, each with its A areas in LDS (as many slots as fit) or in the
+        global workspace."""
         from .autotune import default_tuner
         tuner = tuner or default_tuner()
         props = torch.cuda.get_device_properties(self.device)
         cus = props.multi_processor_count
-        cands = [k * cus for k in (2, 4, 8, 12)]
+        cands = []
+        for k in (2, 4, 8, 12):
+            w = k * cus
+            cands.append((w, 0))
+            ls = self.lds_slots_for(w)
+            if ls > 0:
+                cands.append((w, ls))
         bucket = 1 << max(0, int(total_boxes).bit_length() - 1)
-        key = ("dag_crown_f64", getattr(props, "gcnArchName", ""), cus, self.jobs,
+        key = ("dag_crown_f64_lds", getattr(props, "gcnArchName", ""), cus, self.jobs,
                self.max_slots, self.tape_bytes, bucket)
-        waves, _ = tuner.pick(key, cands, lambda w: self.launch(batches, w))
-        return waves
+        choice, _ = tuner.pick(key, cands, lambda c: self.launch(batches, c[0], lds_slots=c[1]))
+        return tuple(choice)
 
     def graph(self, batches, waves):
         """A captured replay of the bound over `batches` (a
         torch.cuda.CUDAGraph).  Counts are read on the device at replay, so
-        the same graph serves changing batch sizes up to each capacity."""
-        self.workspace(waves)
+        the same graph serves changing batch sizes up to each capacity.
+        waves: a wave count or a (waves, lds_slots) pair from tune_waves."""
+        w = waves[0] if isinstance(waves, tuple) else waves
+        self.workspace(w)
         g = torch.cuda.CUDAGraph()
         s = torch.cuda.Stream()
         s.wait_stream(torch.cuda.current_stream())

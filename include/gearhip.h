@@ -558,12 +558,18 @@ size_t gh_dag_workspace_bytes(const gh_dag_plan *plan, int waves);
  * box concretization (concretize_bounds.py:267-321) and the interval
  * fallback of rows with no relaxation (backward_bound.py:826-877).
  *   batches: device array of nprog gh_dag_batch, in plan order;
- *   ws: gh_dag_workspace_bytes(plan, waves) bytes.
+ *   ws: gh_dag_workspace_bytes(plan, waves) bytes;
+ *   lds_slots (0 .. 64): per-lane slots of each row's A area (the passes'
+ *   A values, the most accessed state) kept in LDS, lds_slots x 512 bytes
+ *   per wavefront; the rest of the state stays in ws.
  * Counts are read on the device, so a captured launch follows the batch
  * sizes at replay.  Calls sharing `ws` must be ordered on one stream.
  */
 int gh_dag_crown_f64(const gh_dag_plan *plan, const gh_dag_batch *batches, void *ws,
-                     size_t ws_bytes, int waves, hipStream_t stream);
+                     size_t ws_bytes, int waves, int lds_slots, hipStream_t stream);
+/* The largest A area of a row in the plan (slots): lds_slots past it
+ * change nothing. */
+int gh_dag_plan_aslots(const gh_dag_plan *plan, int64_t *max_aslots);
 
 #ifdef __cplusplus
 }

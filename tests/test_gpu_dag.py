@@ -156,6 +156,35 @@ def test_wave_count_does_not_change_results(cuda):
     assert np.array_equal(a.view(np.int64), c.view(np.int64))
 
 
+@pytest.mark.parametrize("lds_slots", [1, 8, 64])
+def test_lds_slots_do_not_change_results(cuda, lds_slots):
+    """The A area kept in LDS (all of it, part of it, one slot) gives the
+    same bounds as the global workspace, bit for bit, and the oracle's."""
+    from gear_amd.dag import DagPlan
+    print(f"case lds slots {lds_slots}", flush=True)
+    progs = [random_nlp(14, 4, seed=21), nlp_n8_m3_problem(), _sshape_program()]
+    boxes = [random_bisection_boxes(P.xl, P.xu, 200, seed=k) if k < 2 else
+             (np.random.default_rng(3).uniform(-3, 1, (200, 3)),
+              np.random.default_rng(3).uniform(-3, 1, (200, 3)) + 1.5)
+             for k, P in enumerate(progs)]
+    plan = DagPlan(progs, device=cuda)
+    assert plan.max_aslots > 0
+    xL = [_dev(L, cuda) for L, _ in boxes]
+    xU = [_dev(U, cuda) for _, U in boxes]
+    outs = []
+    for ls in (0, lds_slots):
+        lbs = [torch.empty(len(L), P.R, dtype=torch.float64, device=cuda)
+               for (L, _), P in zip(boxes, progs)]
+        cnt = torch.tensor([len(L) for L, _ in boxes], dtype=torch.int64, device=cuda)
+        batches = plan.make_batches(xL, xU, lbs, [cnt[k:k + 1] for k in range(len(progs))])
+        plan.launch(batches, 96, lds_slots=ls)
+        torch.cuda.synchronize()
+        outs.append([lb.cpu().numpy() for lb in lbs])
+    for k, P in enumerate(progs):
+        assert np.array_equal(outs[0][k].view(np.int64), outs[1][k].view(np.int64)), k
+        _check(outs[1][k], O.dag_crown(P, *boxes[k], threads=8), f"lds program {k}")
+
+
 def test_device_counts_follow_batch_size(cuda):
     """Counts are read on the device: shrinking the count leaves the rows
     past it untouched."""
@@ -212,7 +241,9 @@ def test_autotuned_waves_give_identical_bounds(cuda):
     batches = plan.make_batches([xL], [xU], [lb], [cnt])
     tuner = Autotuner()
     w = plan.tune_waves(batches, 500, tuner=tuner)
-    assert w in [k * torch.cuda.get_device_properties(cuda).multi_processor_count for k in (2, 4, 8, 12)]
+    cus = torch.cuda.get_device_properties(cuda).multi_processor_count
+    assert w[0] in [k * cus for k in (2, 4, 8, 12)]
+    assert w[1] in (0, plan.lds_slots_for(w[0]))
     assert plan.tune_waves(batches, 500, tuner=tuner) == w      # cached
     plan.launch(batches, w)
     torch.cuda.synchronize()
